@@ -278,7 +278,7 @@ void or_reset_state(const or_grid *g, or_state *s)
 }
 
 /* vdiff2 (src/cpu/n_kernel_lib_PSV.cpp:100-131), order 2N. */
-static void vdiff(const or_grid *g, or_state *s)
+void or_vdiff(const or_grid *g, or_state *s)
 {
     const int nx = g->nx, N = g->fdorder / 2;
     const int nz1 = N, nz2 = g->nz - N, nx1 = N, nx2 = g->nx - N;
@@ -307,7 +307,7 @@ static void vdiff(const or_grid *g, or_state *s)
 
 /* sdiff2 (src/cpu/n_kernel_lib_PSV.cpp:207-236), order 2N; outputs in the
  * reference's slots dz_z = szz_z, dx_z = szx_z, dz_x = szx_x, dx_x = sxx_x. */
-static void sdiff(const or_grid *g, or_state *s)
+void or_sdiff(const or_grid *g, or_state *s)
 {
     const int nx = g->nx, N = g->fdorder / 2;
     const int nz1 = N, nz2 = g->nz - N, nx1 = N, nx2 = g->nx - N;
@@ -563,11 +563,11 @@ void or_step(const or_grid *g, const or_media *m, or_state *s, int32_t it, int32
              int32_t accu, or_snaps *snap,
              int32_t grad, double *g_lam, double *g_mu, double *g_rho)
 {
-    vdiff(g, s);
+    or_vdiff(g, s);
     if (g->pml_z || g->pml_x) pml_diff(g, s, s->mem_vz_z, s->mem_vx_z, s->mem_vz_x, s->mem_vx_x);
     update_s(g, m, s);
     if (g->surf) surf_mirror(g, m, s);
-    sdiff(g, s);
+    or_sdiff(g, s);
     if (g->pml_z || g->pml_x) pml_diff(g, s, s->mem_szz_z, s->mem_szx_z, s->mem_szx_x, s->mem_sxx_x);
     update_v(g, m, s);
     if (src && src->nsrc) vsrc(g, m, s, src, ishot, it);

@@ -94,6 +94,8 @@ void or_update_mat2(double *mat, const double *mat_old, const double *grad, doub
 
 /* ---- time stepping ------------------------------------------------------------ */
 void or_reset_state(const or_grid *g, or_state *s);
+void or_vdiff(const or_grid *g, or_state *s);
+void or_sdiff(const or_grid *g, or_state *s);
 void or_step(const or_grid *g, const or_media *m, or_state *s, int32_t it, int32_t ishot,
              const or_sources *src, or_receivers *rec, int32_t record,
              int32_t accu, or_snaps *snap,

@@ -146,7 +146,7 @@ def dtstab(lam, mu, rho, hc, dh, fw):
 def make_case(*, nt, nz, nx, dt, dz, dx, fdorder, lam, mu, rho, z_src, x_src, shot_to_fire,
               z_rec, x_rec, stf_z=None, stf_x=None, stf_type=1, fc=100.0, amp_z=0.0, amp_x=1.0,
               npml=(10, 10, 10, 10), isurf=(0, 0, 0, 0), npower=2.0, damp_v=None, rcoef=0.001,
-              k_max=1.0, snap=None, taper=None, maxrelerror=1):
+              k_max=1.0, snap=None, taper=None, maxrelerror=1, nshot=None):
     """Assemble a case the way simulate_PSV does (src/h_simulate_PSV.cpp:101-185).
 
     ``snap`` = (z1, z2, x1, x2, dt, dz, dx) and ``taper`` = (t1, t2, b1, b2, l1, l2, r1, r2)
@@ -177,6 +177,7 @@ def make_case(*, nt, nz, nx, dt, dz, dx, fdorder, lam, mu, rho, z_src, x_src, sh
              stf_x=np.ascontiguousarray(stf_x, np.float64),
              z_src=np.asarray(z_src, np.int32), x_src=np.asarray(x_src, np.int32),
              shot_to_fire=np.asarray(shot_to_fire, np.int32),
+             nshot=int(nshot if nshot is not None else max(shot_to_fire) + 1),
              z_rec=np.asarray(z_rec, np.int32), x_rec=np.asarray(x_rec, np.int32))
     for k, v in pz.items():
         c[k + "_z"] = v
@@ -403,10 +404,9 @@ def shot_gradient(c, media, ishot, obs_uz, obs_ux, which="oracle"):
 
 
 def misfit_all(c, lam, mu, rho, obs, which="oracle"):
-    media = mat_av2(c, lam, mu, rho, which="oracle")
+    media = mat_av2(c, lam, mu, rho, which=which)
     L2 = 0.0
-    nshot = int(c["shot_to_fire"].max()) + 1
-    for ishot in range(nshot):
+    for ishot in range(int(c["nshot"])):
         f = kernel_psv(c, media, ishot, grad=False, accu=False, which=which)
         _, _, l = adjsrc2(f["uz"], f["ux"], obs[ishot][0], obs[ishot][1], float(c["dt"]), which=which)
         L2 += l
@@ -458,3 +458,31 @@ def step_length(c, est, L2_0, lam_old, mu_old, rho_old, G, obs, which="oracle", 
     if step1 == 1:
         est = calc_opt_step(L2t, sl)
     return est, trials, (list(L2t), list(sl))
+
+
+def ref_step_length(c, est, L2_0, lam_old, mu_old, rho_old, G, obs):
+    """The compiled reference's step_length_PSV (update_param 0) on the same inputs."""
+    lib = load("ref")
+    keep = _Keep()
+    g = _grid(c, keep)
+    nz, nx, nt = int(c["nz"]), int(c["nx"]), int(c["nt"])
+    nshot = int(c["nshot"])
+    lam, mu, rho = lam_old.copy(), mu_old.copy(), rho_old.copy()
+    lc, mc, rc = lam_old.copy(), mu_old.copy(), rho_old.copy()
+    mzx, rz, rx = (np.zeros((nz, nx)) for _ in range(3))
+    gl, gm, gr = (np.ascontiguousarray(a, np.float64).copy() for a in G)
+    st = new_state(c)
+    s = OrState(*[keep.d(st[k]) for k in STATE_FIELDS])
+    nrec = len(c["z_rec"])
+    rec_uz, rec_ux = np.zeros((nrec, nt)), np.zeros((nrec, nt))
+    rec = OrReceivers(nrec, 0, keep.d(rec_uz), keep.d(rec_ux), keep.i(c["z_rec"]), keep.i(c["x_rec"]))
+    src = OrSources(len(c["z_src"]), int(c["stf_type"]), keep.d(c["stf_z"]), keep.d(c["stf_x"]),
+                    keep.i(c["z_src"]), keep.i(c["x_src"]), keep.i(c["shot_to_fire"]))
+    tuz = np.ascontiguousarray(np.stack([o[0] for o in obs]), np.float64)
+    tux = np.ascontiguousarray(np.stack([o[1] for o in obs]), np.float64)
+    snt, snz, snx = snap_dims(c)
+    snaps = {k: np.zeros((snt, snz, snx)) for k in ("vz", "vx", "szz", "szx", "sxx")}
+    sn = OrSnaps(*[keep.d(snaps[k]) for k in ("vz", "vx", "szz", "szx", "sxx")])
+    return lib.ref_step_length_psv(C.c_double(est), C.c_double(L2_0), nshot, C.byref(g), _d(lam), _d(mu), _d(rho),
+                                   _d(lc), _d(mc), _d(rc), _d(mzx), _d(rz), _d(rx), _d(gl), _d(gm), _d(gr),
+                                   C.byref(s), C.byref(src), C.byref(rec), _d(tuz), _d(tux), C.byref(sn))

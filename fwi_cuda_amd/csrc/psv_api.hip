@@ -1,0 +1,1260 @@
+// psv_api.hip -- context, memory plan, time loops and the C ABI of include/fwi_psv.h.
+//
+// Shots are processed in batches of B: every launch advances all B shots of a
+// batch (blockIdx.z = shot), so the media and CPML profiles are read once per
+// tile for B shots and small grids still fill the chip.  The reference loops
+// shots serially (src/cpu/n_simulate_PSV.cpp:270, src/cuda/d_simulate_PSV.cu:443).
+//
+// Gradient storage.  FULL keeps every snap_dt-th forward state of the snap box
+// (the reference's accu_* cubes, src/cpu/n_alloc_PSV.cpp:178-192).  CHECKPOINT
+// keeps the complete propagation state (velocities, stresses, CPML memory)
+// every S steps; the adjoint walks the segments backwards, recomputing each
+// segment's forward states into a segment buffer before running the segment's
+// adjoint steps.  The recomputation repeats the forward kernels on the same
+// inputs, so the checkpointed gradient is bit-identical to the FULL one.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "psv_internal.h"
+
+using namespace psv;
+
+namespace {
+
+struct EventPair {
+    int kid;
+    hipEvent_t a, b;
+};
+
+size_t round_up(size_t v, size_t m) { return (v + m - 1) / m * m; }
+
+}  // namespace
+
+struct fwi_ctx {
+    // ---- configuration
+    fwi_grid g{};
+    fwi_opts o{};
+    int N = 1, P = 0, nzs = 0, nxs = 0, snt = 0, snz = 0, snx = 0;
+    int nshot = 0, stf_type = 1, nrec_tot = 0, nsrc_tot = 0;
+    int tz = 16;
+    std::vector<int> src_off, rec_off, isurf;
+    std::vector<double> prof[12];
+    std::string err;
+    hipStream_t st = nullptr;
+    int have_model = 0, have_obs = 0;
+    double C[3] = {0, 0, 0};
+    double step_est = 0.01;
+
+    // ---- plan
+    int B = 1, storage = FWI_STORE_FULL, S = 0, nseg = 0;
+    size_t bytes = 0;
+    std::vector<void *> allocs;
+
+    // ---- device: shared
+    float *lam = nullptr, *mu = nullptr, *muzx = nullptr, *rhoz = nullptr, *rhox = nullptr;
+    double *m_lam = nullptr, *m_mu = nullptr, *m_rho = nullptr, *o_lam = nullptr, *o_mu = nullptr, *o_rho = nullptr;
+    double *G[3] = {nullptr, nullptr, nullptr};
+    double *tmp = nullptr, *red = nullptr;  // precondition scratch, reduction scratch
+    int *zmap = nullptr, *xmap = nullptr, *nonfinite = nullptr;
+    float *pf[12] = {};   // b a 1/K  b_half a_half 1/K_half, z then x
+    // acquisition
+    double *stf_z = nullptr, *stf_x = nullptr;
+    int *sg_shot = nullptr, *sg_cell = nullptr, *sg_off = nullptr, *sg_idx = nullptr;   // source groups
+    int *rg_shot = nullptr, *rg_cell = nullptr, *rg_off = nullptr, *rg_idx = nullptr;   // receiver groups (adjoint)
+    std::vector<int> sg_first, rg_first;   // first group of each shot (+ total)
+    int *rec_shot = nullptr, *rec_cell = nullptr, *rec_off_d = nullptr;
+    double *syn_z = nullptr, *syn_x = nullptr, *obs_z = nullptr, *obs_x = nullptr, *res_z = nullptr,
+           *res_x = nullptr, *acc = nullptr, *misfit = nullptr;
+    // ---- device: batch
+    float *f[9] = {};      // vz vx szz szx sxx uz ux We Wa   [B][nz][P]
+    float *psi[8] = {};    // pvz_z pvx_z pszz_z pszx_z [B][nzs][P]; pvx_x pvz_x psxx_x pszx_x [B][nz][nxs]
+    float *gr[3] = {};     // [B][snz][snx]
+    float *snap[5] = {};   // FULL: [B][snt][snz][snx]; CHECKPOINT: [B][S/snap_dt][snz][snx]
+    size_t snap_stride = 0;
+    // checkpoint mode
+    float *rf[5] = {}, *rpsi[8] = {};   // recompute state
+    float *ck = nullptr;                // [nseg][13 arrays of the batch]
+    size_t ck_bytes_per_seg = 0;
+    int last_s0 = -1, last_nb = 0;
+    // plain form: full-grid derivative scratch and CPML memory (one shot)
+    bool plain = false;
+    PlainBufs pb{};
+
+    // ---- profiling
+    int prof_on = 0;
+    std::vector<hipEvent_t> ev_pool;
+    std::vector<EventPair> ev_pending;
+    fwi_kernel_time times[FWI_K_COUNT] = {};
+
+    int fail(int code, const char *fmt, ...) {
+        char buf[512];
+        va_list ap;
+        va_start(ap, fmt);
+        vsnprintf(buf, sizeof buf, fmt, ap);
+        va_end(ap);
+        err = buf;
+        return code;
+    }
+    template <class T>
+    T *dalloc(size_t n) {
+        if (n == 0) n = 1;
+        void *p = nullptr;
+        if (hipMalloc(&p, n * sizeof(T)) != hipSuccess) return nullptr;
+        allocs.push_back(p);
+        bytes += n * sizeof(T);
+        return static_cast<T *>(p);
+    }
+    size_t fs() const { return (size_t)g.nz * P; }
+    size_t psiz() const { return (size_t)nzs * P; }
+    size_t psix() const { return (size_t)g.nz * nxs; }
+};
+
+#define HIPCHK(call)                                                                  \
+    do {                                                                              \
+        hipError_t e_ = (call);                                                       \
+        if (e_ != hipSuccess) return c->fail(FWI_EHIP, "%s: %s", #call, hipGetErrorString(e_)); \
+    } while (0)
+
+#define RETIF(x)                  \
+    do {                          \
+        int r_ = (x);             \
+        if (r_ != FWI_OK) return r_; \
+    } while (0)
+
+namespace {
+
+// ---------------------------------------------------------------------------
+// profiling
+// ---------------------------------------------------------------------------
+hipEvent_t ev_get(fwi_ctx *c) {
+    if (!c->ev_pool.empty()) {
+        hipEvent_t e = c->ev_pool.back();
+        c->ev_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e;
+    hipEventCreate(&e);
+    return e;
+}
+
+struct Timer {
+    fwi_ctx *c;
+    int kid;
+    hipEvent_t a = nullptr;
+    Timer(fwi_ctx *c_, int k) : c(c_), kid(k) {
+        if (c->prof_on) {
+            a = ev_get(c);
+            hipEventRecord(a, c->st);
+        }
+    }
+    ~Timer() {
+        if (c->prof_on) {
+            hipEvent_t b = ev_get(c);
+            hipEventRecord(b, c->st);
+            c->ev_pending.push_back({kid, a, b});
+        }
+    }
+};
+
+void prof_collect(fwi_ctx *c) {
+    if (c->ev_pending.empty()) return;
+    hipStreamSynchronize(c->st);
+    for (auto &p : c->ev_pending) {
+        float ms = 0;
+        hipEventElapsedTime(&ms, p.a, p.b);
+        c->times[p.kid].launches += 1;
+        c->times[p.kid].total_ms += ms;
+        c->ev_pool.push_back(p.a);
+        c->ev_pool.push_back(p.b);
+    }
+    c->ev_pending.clear();
+}
+
+int sync(fwi_ctx *c) {
+    hipError_t e = hipStreamSynchronize(c->st);
+    if (e != hipSuccess) return c->fail(FWI_EHIP, "stream: %s", hipGetErrorString(e));
+    e = hipGetLastError();
+    if (e != hipSuccess) return c->fail(FWI_EHIP, "launch: %s", hipGetErrorString(e));
+    prof_collect(c);
+    return FWI_OK;
+}
+
+// ---------------------------------------------------------------------------
+// kernel arguments
+// ---------------------------------------------------------------------------
+KArgs base_args(fwi_ctx *c, int nb, float *const fld[5], float *const ps[8]) {
+    KArgs a{};
+    a.nz = c->g.nz;
+    a.nx = c->g.nx;
+    a.P = c->P;
+    a.B = nb;
+    a.fs = c->fs();
+    a.dt = (float)c->g.dt;
+    for (int k = 0; k < 6; ++k) {
+        a.cz[k] = k < c->N ? (float)(c->g.hc[k + 1] / c->g.dz) : 0.f;
+        a.cx[k] = k < c->N ? (float)(c->g.hc[k + 1] / c->g.dx) : 0.f;
+    }
+    a.vz = fld[0];
+    a.vx = fld[1];
+    a.szz = fld[2];
+    a.szx = fld[3];
+    a.sxx = fld[4];
+    a.uz = c->f[5];
+    a.ux = c->f[6];
+    a.We = c->f[7];
+    a.lam = c->lam;
+    a.mu = c->mu;
+    a.muzx = c->muzx;
+    a.rhoz = c->rhoz;
+    a.rhox = c->rhox;
+    a.zmap = c->zmap;
+    a.xmap = c->xmap;
+    a.nzs = c->nzs;
+    a.nxs = c->nxs;
+    a.bz = c->pf[0]; a.az = c->pf[1]; a.ikz = c->pf[2];
+    a.bhz = c->pf[3]; a.ahz = c->pf[4]; a.ikhz = c->pf[5];
+    a.bx = c->pf[6]; a.ax = c->pf[7]; a.ikx = c->pf[8];
+    a.bhx = c->pf[9]; a.ahx = c->pf[10]; a.ikhx = c->pf[11];
+    a.pvz_z = ps[0]; a.pvx_z = ps[1]; a.pszz_z = ps[2]; a.pszx_z = ps[3];
+    a.pvx_x = ps[4]; a.pvz_x = ps[5]; a.psxx_x = ps[6]; a.pszx_x = ps[7];
+    a.snap_z1 = c->g.snap_z1;
+    a.snap_z2 = c->g.snap_z2;
+    a.snap_x1 = c->g.snap_x1;
+    a.snap_x2 = c->g.snap_x2;
+    a.snap_dz = c->g.snap_dz;
+    a.snap_dx = c->g.snap_dx;
+    a.snz = c->snz;
+    a.snx = c->snx;
+    a.g_lam = c->gr[0];
+    a.g_mu = c->gr[1];
+    a.g_rho = c->gr[2];
+    a.gw = (float)(c->o.grad_sign * (double)c->g.snap_dt * c->g.dt);
+    return a;
+}
+
+SrcArgs src_args(fwi_ctx *c, int s0, int nb) {
+    SrcArgs s{};
+    s.shot0 = s0;
+    const int g0 = c->sg_first[s0], g1 = c->sg_first[s0 + nb];
+    s.ngroups = g1 - g0;
+    s.grp_shot = c->sg_shot + g0;
+    s.grp_cell = c->sg_cell + g0;
+    s.grp_off = c->sg_off + g0;
+    s.grp_idx = c->sg_idx;
+    s.stf_z = c->stf_z;
+    s.stf_x = c->stf_x;
+    s.nt = c->g.nt;
+    s.type = c->stf_type;
+    s.dt = c->g.dt;
+    s.dzdx = c->g.dz * c->g.dx;
+    s.scale = 1.0;
+    return s;
+}
+
+// adjoint sources: the receivers of each shot, residual rows as source functions
+SrcArgs adj_args(fwi_ctx *c, int s0, int nb, double scale) {
+    SrcArgs s{};
+    s.shot0 = s0;
+    const int g0 = c->rg_first[s0], g1 = c->rg_first[s0 + nb];
+    s.ngroups = g1 - g0;
+    s.grp_shot = c->rg_shot + g0;
+    s.grp_cell = c->rg_cell + g0;
+    s.grp_off = c->rg_off + g0;
+    s.grp_idx = c->rg_idx;
+    s.stf_z = c->res_z;
+    s.stf_x = c->res_x;
+    s.nt = c->g.nt;
+    s.type = 0;
+    s.dt = c->g.dt;
+    s.dzdx = c->g.dz * c->g.dx;
+    s.scale = scale;
+    return s;
+}
+
+RecArgs rec_args(fwi_ctx *c, int s0, int nb) {
+    RecArgs r{};
+    r.shot0 = s0;
+    const int r0 = c->rec_off[s0], r1 = c->rec_off[s0 + nb];
+    r.nrec = r1 - r0;
+    r.rec_shot = c->rec_shot + r0;
+    r.rec_cell = c->rec_cell + r0;
+    r.acc_z = c->acc;
+    r.acc_x = c->acc + c->nrec_tot;
+    r.seis_z = c->syn_z + (size_t)r0 * c->g.nt;
+    r.seis_x = c->syn_x + (size_t)r0 * c->g.nt;
+    r.nt = c->g.nt;
+    r.dt = c->g.dt;
+    r.dzdx = c->g.dz * c->g.dx;
+    return r;
+}
+
+bool has_surface(const fwi_ctx *c) {
+    for (int i = 0; i < 4; ++i)
+        if (c->g.isurf[i] > 0) return true;
+    return false;
+}
+
+// stress half-step, free surface, velocity half-step on the given state
+void half_steps(fwi_ctx *c, const KArgs &a) {
+    if (c->plain) {
+        Timer t(c, FWI_K_STRESS);
+        plain_step(a, c->pb, c->N, c->g.isurf, c->g.pml_z, c->g.pml_x, c->st);
+        return;
+    }
+    {
+        Timer t(c, FWI_K_STRESS);
+        launch_stress(a, c->N, c->tz, c->st);
+    }
+    if (has_surface(c)) {
+        Timer t(c, FWI_K_SURFACE);
+        launch_surface(a, c->N, c->g.isurf, c->st);
+    }
+    {
+        Timer t(c, FWI_K_VELOCITY);
+        launch_velocity(a, c->N, c->tz, c->st);
+    }
+}
+
+// one forward-structured time step on the given state
+void fwd_step(fwi_ctx *c, KArgs a, const SrcArgs &sa, int it) {
+    half_steps(c, a);
+    {
+        Timer t(c, FWI_K_SOURCE);
+        launch_inject(a, sa, it, c->st);
+    }
+}
+
+int zero_state(fwi_ctx *c, int nb, float *const fld[5], float *const ps[8], bool u_we) {
+    for (int i = 0; i < 5; ++i) HIPCHK(hipMemsetAsync(fld[i], 0, sizeof(float) * nb * c->fs(), c->st));
+    for (int i = 0; i < 4; ++i) HIPCHK(hipMemsetAsync(ps[i], 0, sizeof(float) * nb * c->psiz(), c->st));
+    for (int i = 4; i < 8; ++i) HIPCHK(hipMemsetAsync(ps[i], 0, sizeof(float) * nb * c->psix(), c->st));
+    if (c->plain) {
+        float *m[8] = {c->pb.mvz_z, c->pb.mvx_z, c->pb.mszz_z, c->pb.mszx_z,
+                       c->pb.mvz_x, c->pb.mvx_x, c->pb.mszx_x, c->pb.msxx_x};
+        for (int i = 0; i < 8; ++i) HIPCHK(hipMemsetAsync(m[i], 0, sizeof(float) * c->fs(), c->st));
+    }
+    if (u_we)
+        for (int i = 5; i < 9; ++i) HIPCHK(hipMemsetAsync(c->f[i], 0, sizeof(float) * nb * c->fs(), c->st));
+    return FWI_OK;
+}
+
+// copy the 13-array propagation state of nb shots to / from checkpoint k
+int ckpt_copy(fwi_ctx *c, int k, int nb, float *const fld[5], float *const ps[8], bool save) {
+    char *base = reinterpret_cast<char *>(c->ck) + (size_t)k * c->ck_bytes_per_seg;
+    size_t off = 0;
+    for (int i = 0; i < 13; ++i) {
+        float *p = i < 5 ? fld[i] : ps[i - 5];
+        const size_t per = i < 5 ? c->fs() : (i < 9 ? c->psiz() : c->psix());
+        const size_t nbytes = sizeof(float) * per * nb;
+        void *dst = save ? (void *)(base + off) : (void *)p;
+        const void *src = save ? (const void *)p : (const void *)(base + off);
+        HIPCHK(hipMemcpyAsync(dst, src, nbytes, hipMemcpyDeviceToDevice, c->st));
+        off += sizeof(float) * per * c->B;
+    }
+    return FWI_OK;
+}
+
+int check_finite(fwi_ctx *c, int nb) {
+    if (!c->o.check_finite) return FWI_OK;
+    HIPCHK(hipMemsetAsync(c->nonfinite, 0, sizeof(int), c->st));
+    dev_count_nonfinite(c->f[0], nb * c->fs(), c->nonfinite, c->st);
+    dev_count_nonfinite(c->f[1], nb * c->fs(), c->nonfinite, c->st);
+    int h = 0;
+    HIPCHK(hipMemcpyAsync(&h, c->nonfinite, sizeof(int), hipMemcpyDeviceToHost, c->st));
+    RETIF(sync(c));
+    if (h) return c->fail(FWI_ENONFINITE, "%d non-finite velocity values after propagation", h);
+    return FWI_OK;
+}
+
+// Forward pass (kernel_PSV with grad = false) for shots [s0, s0 + nb).
+int run_forward(fwi_ctx *c, int s0, int nb, bool store, bool record, bool energy) {
+    float *fld[5] = {c->f[0], c->f[1], c->f[2], c->f[3], c->f[4]};
+    RETIF(zero_state(c, nb, fld, c->psi, true));
+    if (record) HIPCHK(hipMemsetAsync(c->acc, 0, sizeof(double) * 2 * c->nrec_tot, c->st));
+    KArgs a = base_args(c, nb, fld, c->psi);
+    a.energy = energy;
+    const SrcArgs sa = src_args(c, s0, nb);
+    const RecArgs ra = rec_args(c, s0, nb);
+    const bool ckpt = store && c->storage == FWI_STORE_CHECKPOINT;
+    for (int it = 0; it < c->g.nt; ++it) {
+        if (ckpt && it % c->S == 0) {
+            Timer t(c, FWI_K_STORE);
+            RETIF(ckpt_copy(c, it / c->S, nb, fld, c->psi, true));
+        }
+        fwd_step(c, a, sa, it);
+        if (record) {
+            Timer t(c, FWI_K_RECEIVER);
+            launch_record(a, ra, it, c->st);
+        }
+        if (store && !ckpt && it % c->g.snap_dt == 0) {
+            Timer t(c, FWI_K_STORE);
+            const size_t off = (size_t)(it / c->g.snap_dt) * c->snz * c->snx;
+            float *dst[5] = {c->snap[0] + off, c->snap[1] + off, c->snap[2] + off, c->snap[3] + off,
+                             c->snap[4] + off};
+            launch_store(a, dst, c->snap_stride, (float)c->g.dt, c->st);
+        }
+    }
+    c->last_s0 = s0;
+    c->last_nb = nb;
+    return check_finite(c, nb);
+}
+
+// One adjoint step (time index it); on snapshot steps the velocity kernel
+// correlates with the forward slab at slab_off (fwi_grad2 fused), or the plain
+// form runs fwi_grad2 as its own pass after the injection, as the reference does.
+void adj_step(fwi_ctx *c, KArgs a, const SrcArgs &sa, int it, size_t slab_off) {
+    const bool gstep = it % c->g.snap_dt == 0;
+    if (gstep) {
+        a.sn_vz = c->snap[0] + slab_off;
+        a.sn_vx = c->snap[1] + slab_off;
+        a.sn_szz = c->snap[2] + slab_off;
+        a.sn_szx = c->snap[3] + slab_off;
+        a.sn_sxx = c->snap[4] + slab_off;
+        a.sn_stride = c->snap_stride;
+    }
+    a.grad = gstep && !c->plain;
+    half_steps(c, a);
+    {
+        Timer t(c, FWI_K_SOURCE);
+        launch_inject(a, sa, it, c->st);
+    }
+    if (gstep && c->plain) {
+        Timer t(c, FWI_K_OTHER);
+        plain_grad(a, c->st);
+    }
+}
+
+// Adjoint pass (kernel_PSV with grad = true): residuals in res_* are injected at
+// the receivers in reversed time, scaled by `alpha` (a power of two).
+int run_adjoint(fwi_ctx *c, int s0, int nb, double alpha) {
+    float *fld[5] = {c->f[0], c->f[1], c->f[2], c->f[3], c->f[4]};
+    RETIF(zero_state(c, nb, fld, c->psi, false));
+    for (int i = 5; i < 7; ++i) HIPCHK(hipMemsetAsync(c->f[i], 0, sizeof(float) * nb * c->fs(), c->st));
+    HIPCHK(hipMemsetAsync(c->f[8], 0, sizeof(float) * nb * c->fs(), c->st));
+    for (int i = 0; i < 3; ++i)
+        HIPCHK(hipMemsetAsync(c->gr[i], 0, sizeof(float) * nb * c->snz * c->snx, c->st));
+    KArgs a = base_args(c, nb, fld, c->psi);
+    a.We = c->f[8];
+    a.energy = 1;
+    a.displ = 1;
+    const SrcArgs sa = adj_args(c, s0, nb, alpha);
+    const int nt = c->g.nt, sdt = c->g.snap_dt;
+    const size_t slab = (size_t)c->snz * c->snx;
+    if (c->storage != FWI_STORE_CHECKPOINT) {
+        for (int jt = 0; jt < nt; ++jt) {
+            const int it = nt - 1 - jt;
+            adj_step(c, a, sa, it, (size_t)(it / sdt) * slab);
+        }
+    } else {
+        KArgs r = base_args(c, nb, c->rf, c->rpsi);
+        const SrcArgs fsa = src_args(c, s0, nb);
+        for (int k = c->nseg - 1; k >= 0; --k) {
+            const int t0 = k * c->S, t1 = std::min(t0 + c->S, nt);
+            {
+                Timer t(c, FWI_K_STORE);
+                RETIF(ckpt_copy(c, k, nb, c->rf, c->rpsi, false));
+            }
+            for (int it = t0; it < t1; ++it) {
+                fwd_step(c, r, fsa, it);
+                if (it % sdt == 0) {
+                    Timer t(c, FWI_K_STORE);
+                    const size_t off = (size_t)((it - t0) / sdt) * slab;
+                    float *dst[5] = {c->snap[0] + off, c->snap[1] + off, c->snap[2] + off, c->snap[3] + off,
+                                     c->snap[4] + off};
+                    launch_store(r, dst, c->snap_stride, (float)c->g.dt, c->st);
+                }
+            }
+            for (int it = t1 - 1; it >= t0; --it) adj_step(c, a, sa, it, (size_t)((it - t0) / sdt) * slab);
+        }
+    }
+    c->last_s0 = s0;
+    c->last_nb = nb;
+    return check_finite(c, nb);
+}
+
+// power of two bringing max|x| into [1, 2)
+double pow2_scale(double m) {
+    if (!(m > 0) || !std::isfinite(m)) return 1.0;
+    int e;
+    std::frexp(m, &e);
+    return std::ldexp(1.0, 1 - e);
+}
+
+int absmax_host(fwi_ctx *c, const double *x, size_t n, double *out) {
+    dev_absmax(x, n, c->red, c->st);
+    HIPCHK(hipMemcpyAsync(out, c->red, sizeof(double), hipMemcpyDeviceToHost, c->st));
+    return sync(c);
+}
+
+int batches(const fwi_ctx *c, int shot0, int nshots, std::vector<std::pair<int, int>> &out) {
+    out.clear();
+    if (shot0 < 0 || nshots < 0 || shot0 + nshots > c->nshot) return FWI_EINVAL;
+    const int nbat = (nshots + c->B - 1) / c->B;
+    for (int b = 0, s = shot0; b < nbat; ++b) {
+        const int left = shot0 + nshots - s, rem = nbat - b;
+        const int nb = (left + rem - 1) / rem;   // balanced batches
+        out.push_back({s, nb});
+        s += nb;
+    }
+    return FWI_OK;
+}
+
+// forward + residual + adjoint for one batch; leaves raw gradients in gr (scaled by alpha)
+int batch_gradient(fwi_ctx *c, int s0, int nb, const double *adj_z, const double *adj_x, double *misfit_out,
+                   double *alpha_out) {
+    RETIF(run_forward(c, s0, nb, true, adj_z == nullptr, true));
+    const int r0 = c->rec_off[s0], r1 = c->rec_off[s0 + nb];
+    const size_t rows = (size_t)(r1 - r0) * c->g.nt, off = (size_t)r0 * c->g.nt;
+    if (adj_z) {
+        HIPCHK(hipMemcpyAsync(c->res_z + off, adj_z, sizeof(double) * rows, hipMemcpyHostToDevice, c->st));
+        HIPCHK(hipMemcpyAsync(c->res_x + off, adj_x, sizeof(double) * rows, hipMemcpyHostToDevice, c->st));
+    } else {
+        dev_residual(c->syn_z, c->syn_x, c->obs_z, c->obs_x, c->res_z, c->res_x, c->rec_off_d + s0, nb, c->g.nt,
+                     c->g.dt, c->misfit, c->st);
+        if (misfit_out)
+            HIPCHK(hipMemcpyAsync(misfit_out, c->misfit, sizeof(double) * nb, hipMemcpyDeviceToHost, c->st));
+    }
+    double mz = 0, mx = 0;
+    RETIF(absmax_host(c, c->res_z + off, rows, &mz));
+    RETIF(absmax_host(c, c->res_x + off, rows, &mx));
+    const double alpha = pow2_scale(std::max(mz, mx));
+    *alpha_out = alpha;
+    return run_adjoint(c, s0, nb, alpha);
+}
+
+int refresh_media(fwi_ctx *c) {
+    dev_mat_av(c->m_lam, c->m_mu, c->m_rho, c->lam, c->mu, c->muzx, c->rhoz, c->rhox, c->red, c->g.nz, c->g.nx,
+               c->P, c->st);
+    HIPCHK(hipMemcpyAsync(c->C, c->red, sizeof(double) * 3, hipMemcpyDeviceToHost, c->st));
+    return sync(c);
+}
+
+// ---------------------------------------------------------------------------
+// planning and allocation
+// ---------------------------------------------------------------------------
+struct Need {
+    size_t shared, per_shot;
+};
+
+Need need_bytes(const fwi_ctx *c, int storage, int S) {
+    const size_t nz = c->g.nz, nx = c->g.nx, nt = c->g.nt, fs = c->fs();
+    const size_t snapcell = (size_t)c->snz * c->snx;
+    Need n{};
+    n.shared = 5 * fs * 4 + 9 * nz * nx * 8 + 2 * nz * nx * 8 + (size_t)c->nrec_tot * nt * 8 * 6 +
+               (size_t)c->nsrc_tot * nt * 8 * 2 + (64 << 20);
+    const size_t state_ck = 5 * fs * 4 + 4 * (c->psiz() + c->psix()) * 4;
+    n.per_shot = 9 * fs * 4 + 8 * (c->psiz() + c->psix()) * 4 + 3 * snapcell * 4;
+    if (c->plain) n.per_shot += 12 * fs * 4;
+    if (storage == FWI_STORE_FULL) {
+        n.per_shot += 5 * (size_t)c->snt * snapcell * 4;
+    } else {
+        const size_t nseg = (nt + S - 1) / S;
+        n.per_shot += state_ck + nseg * state_ck + 5 * (size_t)(S / c->g.snap_dt) * snapcell * 4;
+    }
+    return n;
+}
+
+int plan(fwi_ctx *c) {
+    size_t freeb = 0, total = 0;
+    HIPCHK(hipMemGetInfo(&freeb, &total));
+    const size_t budget = c->o.mem_budget > 0 ? (size_t)c->o.mem_budget : (size_t)(0.9 * (double)freeb);
+    const int sdt = c->g.snap_dt;
+    // segment length for checkpoints: minimise nseg * state + S/snap_dt * snapshot, multiple of snap_dt
+    int S = c->o.ckpt_interval;
+    if (S <= 0) {
+        const double state_ck = 5.0 * c->fs() + 4.0 * (c->psiz() + c->psix());
+        const double snapb = 5.0 * c->snz * c->snx;
+        S = (int)std::lround(std::sqrt((double)c->g.nt * state_ck * sdt / snapb));
+    }
+    S = std::max(sdt, (S + sdt - 1) / sdt * sdt);
+    S = std::min(S, (c->g.nt + sdt - 1) / sdt * sdt);
+    int storage = c->o.storage;
+    auto fits = [&](int st_, int b) {
+        Need n = need_bytes(c, st_, S);
+        return n.shared + (size_t)b * n.per_shot <= budget;
+    };
+    if (c->plain) {
+        if (storage == FWI_STORE_CHECKPOINT) return c->fail(FWI_EINVAL, "the plain form stores full snapshots");
+        storage = FWI_STORE_FULL;
+    }
+    if (storage == FWI_STORE_AUTO) storage = fits(FWI_STORE_FULL, 1) ? FWI_STORE_FULL : FWI_STORE_CHECKPOINT;
+    Need n = need_bytes(c, storage, S);
+    if (n.shared + n.per_shot > budget)
+        return c->fail(FWI_ENOMEM, "one shot needs %.2f GB, budget %.2f GB", (n.shared + n.per_shot) / 1e9,
+                       budget / 1e9);
+    int Bmax = (int)std::min<size_t>((budget - n.shared) / n.per_shot, (size_t)c->nshot);
+    if (c->o.batch > 0) Bmax = std::min(Bmax, c->o.batch);
+    if (c->plain) Bmax = 1;
+    Bmax = std::max(1, Bmax);
+    const int nbat = (c->nshot + Bmax - 1) / Bmax;
+    c->B = (c->nshot + nbat - 1) / nbat;
+    c->storage = storage;
+    c->S = S;
+    c->nseg = (c->g.nt + S - 1) / S;
+    return FWI_OK;
+}
+
+int allocate(fwi_ctx *c) {
+    const int nz = c->g.nz, nx = c->g.nx, nt = c->g.nt, B = c->B;
+    const size_t fs = c->fs(), nn = (size_t)nz * nx, snapcell = (size_t)c->snz * c->snx;
+    bool ok = true;
+    auto F = [&](size_t n) { float *p = c->dalloc<float>(n); ok = ok && p; return p; };
+    auto D = [&](size_t n) { double *p = c->dalloc<double>(n); ok = ok && p; return p; };
+    auto I = [&](size_t n) { int *p = c->dalloc<int>(n); ok = ok && p; return p; };
+    c->lam = F(fs); c->mu = F(fs); c->muzx = F(fs); c->rhoz = F(fs); c->rhox = F(fs);
+    c->m_lam = D(nn); c->m_mu = D(nn); c->m_rho = D(nn);
+    c->o_lam = D(nn); c->o_mu = D(nn); c->o_rho = D(nn);
+    for (int i = 0; i < 3; ++i) c->G[i] = D(nn);
+    c->tmp = D(2 * nn);
+    c->red = D(4 + 3 * 1024);
+    c->nonfinite = I(1);
+    c->zmap = I(nz);
+    c->xmap = I(nx);
+    for (int i = 0; i < 6; ++i) c->pf[i] = F(nz);
+    for (int i = 6; i < 12; ++i) c->pf[i] = F(nx);
+    const size_t recrows = (size_t)c->nrec_tot * nt;
+    c->syn_z = D(recrows); c->syn_x = D(recrows);
+    c->obs_z = D(recrows); c->obs_x = D(recrows);
+    c->res_z = D(recrows); c->res_x = D(recrows);
+    c->acc = D(2 * (size_t)c->nrec_tot);
+    c->misfit = D(c->nshot);
+    for (int i = 0; i < 9; ++i) c->f[i] = F(B * fs);
+    for (int i = 0; i < 4; ++i) c->psi[i] = F(B * c->psiz());
+    for (int i = 4; i < 8; ++i) c->psi[i] = F(B * c->psix());
+    for (int i = 0; i < 3; ++i) c->gr[i] = F(B * snapcell);
+    if (c->storage == FWI_STORE_FULL) {
+        c->snap_stride = (size_t)c->snt * snapcell;
+    } else {
+        c->snap_stride = (size_t)(c->S / c->g.snap_dt) * snapcell;
+        for (int i = 0; i < 5; ++i) c->rf[i] = F(B * fs);
+        for (int i = 0; i < 4; ++i) c->rpsi[i] = F(B * c->psiz());
+        for (int i = 4; i < 8; ++i) c->rpsi[i] = F(B * c->psix());
+        c->ck_bytes_per_seg = sizeof(float) * B * (5 * fs + 4 * (c->psiz() + c->psix()));
+        c->ck = F(c->ck_bytes_per_seg / sizeof(float) * c->nseg);
+    }
+    for (int i = 0; i < 5; ++i) c->snap[i] = F(B * c->snap_stride);
+    if (c->plain) {
+        PlainBufs &p = c->pb;
+        float **m[12] = {&p.dz_z, &p.dx_z, &p.dz_x, &p.dx_x, &p.mvz_z, &p.mvx_z,
+                         &p.mszz_z, &p.mszx_z, &p.mvz_x, &p.mvx_x, &p.mszx_x, &p.msxx_x};
+        for (int i = 0; i < 12; ++i) *m[i] = F(fs);
+        p.fbz = c->pf[0]; p.faz = c->pf[1]; p.fikz = c->pf[2];
+        p.fbhz = c->pf[3]; p.fahz = c->pf[4]; p.fikhz = c->pf[5];
+        p.fbx = c->pf[6]; p.fax = c->pf[7]; p.fikx = c->pf[8];
+        p.fbhx = c->pf[9]; p.fahx = c->pf[10]; p.fikhx = c->pf[11];
+    }
+    if (!ok) return c->fail(FWI_ENOMEM, "device allocation failed after %.2f GB", c->bytes / 1e9);
+    return FWI_OK;
+}
+
+void free_all(fwi_ctx *c) {
+    for (void *p : c->allocs) hipFree(p);
+    c->allocs.clear();
+}
+
+// groups of (shot, cell) for deterministic injection in source order
+void build_groups(const std::vector<int> &off, const int32_t *z, const int32_t *x, int P, std::vector<int> &gshot,
+                  std::vector<int> &gcell, std::vector<int> &goff, std::vector<int> &gidx, std::vector<int> &first) {
+    const int nshot = (int)off.size() - 1;
+    first.assign(nshot + 1, 0);
+    goff.clear();
+    for (int s = 0; s < nshot; ++s) {
+        first[s] = (int)gshot.size();
+        std::vector<std::pair<int, int>> cells;   // (cell, index) in source order
+        for (int i = off[s]; i < off[s + 1]; ++i) cells.push_back({z[i] * P + x[i], i});
+        std::vector<char> used(cells.size(), 0);
+        for (size_t a = 0; a < cells.size(); ++a) {
+            if (used[a]) continue;
+            gshot.push_back(s);
+            gcell.push_back(cells[a].first);
+            goff.push_back((int)gidx.size());
+            for (size_t b = a; b < cells.size(); ++b)
+                if (!used[b] && cells[b].first == cells[a].first) {
+                    used[b] = 1;
+                    gidx.push_back(cells[b].second);
+                }
+        }
+    }
+    first[nshot] = (int)gshot.size();
+    goff.push_back((int)gidx.size());
+}
+
+template <class T>
+int upload(fwi_ctx *c, T *dst, const T *src, size_t n) {
+    if (n == 0) return FWI_OK;
+    HIPCHK(hipMemcpyAsync(dst, src, n * sizeof(T), hipMemcpyHostToDevice, c->st));
+    return FWI_OK;
+}
+
+int setup_device(fwi_ctx *c, const fwi_acq *q) {
+    const int nz = c->g.nz, nx = c->g.nx, N = c->N;
+    // CPML active rows / columns inside the interior
+    std::vector<int> zmap(nz, -1), xmap(nx, -1);
+    auto active = [](const fwi_ctx *c_, int base, int i) {
+        const auto &p = c_->prof;
+        return p[base + 0][i] != 0.0 || p[base + 3][i] != 0.0 || p[base + 2][i] != 1.0 || p[base + 5][i] != 1.0;
+    };
+    c->nzs = 0;
+    if (c->g.pml_z)
+        for (int i = N; i < nz - N; ++i)
+            if (active(c, 0, i)) zmap[i] = c->nzs++;
+    c->nxs = 0;
+    if (c->g.pml_x)
+        for (int i = N; i < nx - N; ++i)
+            if (active(c, 6, i)) xmap[i] = c->nxs++;
+    RETIF(plan(c));
+    RETIF(allocate(c));
+    RETIF(upload(c, c->zmap, zmap.data(), nz));
+    RETIF(upload(c, c->xmap, xmap.data(), nx));
+    // profiles as b, a, 1/K (integer then half), z then x
+    std::vector<float> hz(6 * nz), hx(6 * nx);
+    for (int i = 0; i < nz; ++i) {
+        hz[0 * nz + i] = (float)c->prof[1][i];
+        hz[1 * nz + i] = (float)c->prof[0][i];
+        hz[2 * nz + i] = (float)(1.0 / c->prof[2][i]);
+        hz[3 * nz + i] = (float)c->prof[4][i];
+        hz[4 * nz + i] = (float)c->prof[3][i];
+        hz[5 * nz + i] = (float)(1.0 / c->prof[5][i]);
+    }
+    for (int i = 0; i < nx; ++i) {
+        hx[0 * nx + i] = (float)c->prof[7][i];
+        hx[1 * nx + i] = (float)c->prof[6][i];
+        hx[2 * nx + i] = (float)(1.0 / c->prof[8][i]);
+        hx[3 * nx + i] = (float)c->prof[10][i];
+        hx[4 * nx + i] = (float)c->prof[9][i];
+        hx[5 * nx + i] = (float)(1.0 / c->prof[11][i]);
+    }
+    for (int k = 0; k < 6; ++k) RETIF(upload(c, c->pf[k], hz.data() + k * nz, nz));
+    for (int k = 0; k < 6; ++k) RETIF(upload(c, c->pf[6 + k], hx.data() + k * nx, nx));
+    // sources
+    std::vector<int> gs, gc, go, gi;
+    build_groups(c->src_off, q->src_z, q->src_x, c->P, gs, gc, go, gi, c->sg_first);
+    bool ok = true;
+    auto I = [&](size_t n) { int *p = c->dalloc<int>(n); ok = ok && p; return p; };
+    c->sg_shot = I(gs.size()); c->sg_cell = I(gc.size()); c->sg_off = I(go.size()); c->sg_idx = I(gi.size());
+    c->stf_z = c->dalloc<double>((size_t)c->nsrc_tot * c->g.nt);
+    c->stf_x = c->dalloc<double>((size_t)c->nsrc_tot * c->g.nt);
+    if (!ok || !c->stf_z || !c->stf_x) return c->fail(FWI_ENOMEM, "source tables");
+    RETIF(upload(c, c->sg_shot, gs.data(), gs.size()));
+    RETIF(upload(c, c->sg_cell, gc.data(), gc.size()));
+    RETIF(upload(c, c->sg_off, go.data(), go.size()));
+    RETIF(upload(c, c->sg_idx, gi.data(), gi.size()));
+    RETIF(upload(c, c->stf_z, q->stf_z, (size_t)c->nsrc_tot * c->g.nt));
+    RETIF(upload(c, c->stf_x, q->stf_x, (size_t)c->nsrc_tot * c->g.nt));
+    // receivers: recording list and adjoint-injection groups
+    std::vector<int> rshot(c->nrec_tot), rcell(c->nrec_tot);
+    for (int s = 0; s < c->nshot; ++s)
+        for (int r = c->rec_off[s]; r < c->rec_off[s + 1]; ++r) {
+            rshot[r] = s;
+            rcell[r] = q->rec_z[r] * c->P + q->rec_x[r];
+        }
+    c->rec_shot = I(c->nrec_tot);
+    c->rec_cell = I(c->nrec_tot);
+    c->rec_off_d = I(c->nshot + 1);
+    gs.clear(); gc.clear(); go.clear(); gi.clear();
+    build_groups(c->rec_off, q->rec_z, q->rec_x, c->P, gs, gc, go, gi, c->rg_first);
+    c->rg_shot = I(gs.size()); c->rg_cell = I(gc.size()); c->rg_off = I(go.size()); c->rg_idx = I(gi.size());
+    if (!ok) return c->fail(FWI_ENOMEM, "receiver tables");
+    RETIF(upload(c, c->rec_shot, rshot.data(), rshot.size()));
+    RETIF(upload(c, c->rec_cell, rcell.data(), rcell.size()));
+    RETIF(upload(c, c->rec_off_d, c->rec_off.data(), c->rec_off.size()));
+    RETIF(upload(c, c->rg_shot, gs.data(), gs.size()));
+    RETIF(upload(c, c->rg_cell, gc.data(), gc.size()));
+    RETIF(upload(c, c->rg_off, go.data(), go.size()));
+    RETIF(upload(c, c->rg_idx, gi.data(), gi.size()));
+    for (int i = 0; i < 3; ++i) HIPCHK(hipMemsetAsync(c->G[i], 0, sizeof(double) * (size_t)nz * nx, c->st));
+    return sync(c);
+}
+
+int validate(fwi_ctx *c, const fwi_grid *g, const fwi_acq *q) {
+    if (g->nt <= 0 || g->nz <= 0 || g->nx <= 0) return c->fail(FWI_EINVAL, "empty grid or time axis");
+    if (g->fdorder < 2 || g->fdorder > 12 || g->fdorder % 2) return c->fail(FWI_EINVAL, "fdorder %d", g->fdorder);
+    const int N = g->fdorder / 2;
+    if (g->nz < 2 * N + 1 || g->nx < 2 * N + 1) return c->fail(FWI_EINVAL, "grid smaller than the stencil");
+    if (!(g->dt > 0) || !(g->dz > 0) || !(g->dx > 0)) return c->fail(FWI_EINVAL, "dt, dz, dx must be positive");
+    if (g->snap_dt < 1 || g->snap_dz < 1 || g->snap_dx < 1) return c->fail(FWI_EINVAL, "snap strides < 1");
+    if (g->snap_z1 < 0 || g->snap_z2 >= g->nz || g->snap_z1 > g->snap_z2 || g->snap_x1 < 0 ||
+        g->snap_x2 >= g->nx || g->snap_x1 > g->snap_x2)
+        return c->fail(FWI_EINVAL, "snap box outside the grid");
+    if ((g->snap_dx > 1 && (g->snap_x2 - g->snap_x1) % g->snap_dx) ||
+        (g->snap_dz > 1 && (g->snap_z2 - g->snap_z1) % g->snap_dz))
+        return c->fail(FWI_EINVAL, "snap box must span whole snap strides");
+    for (int s = 0; s < 4; ++s) {
+        const int v = g->isurf[s];
+        if (v < 0) return c->fail(FWI_EINVAL, "isurf[%d] < 0", s);
+        if (v > 0 && (s < 2 ? (v < N || v >= g->nz - N) : (v < N || v >= g->nx - N)))
+            return c->fail(FWI_EINVAL, "free surface %d at %d is outside the interior", s, v);
+        // the image rows / columns (surf_mirror) must stay inside the grid
+        const int n = s < 2 ? g->nz : g->nx;
+        if (v > 0 && ((s % 2 == 0 && 2 * v - N > n - 1) || (s % 2 == 1 && 2 * v - n + N < 0)))
+            return c->fail(FWI_EINVAL, "free surface %d at %d mirrors beyond the grid", s, v);
+    }
+    if (g->pml_z && (!g->a_z || !g->b_z || !g->K_z || !g->a_half_z || !g->b_half_z || !g->K_half_z))
+        return c->fail(FWI_EINVAL, "missing z CPML profiles");
+    if (g->pml_x && (!g->a_x || !g->b_x || !g->K_x || !g->a_half_x || !g->b_half_x || !g->K_half_x))
+        return c->fail(FWI_EINVAL, "missing x CPML profiles");
+    if (!q || q->nshot <= 0 || !q->src_off || !q->rec_off) return c->fail(FWI_EINVAL, "no shots");
+    if (q->stf_type != 0 && q->stf_type != 1) return c->fail(FWI_EINVAL, "stf_type %d", q->stf_type);
+    for (int s = 0; s < q->nshot; ++s)
+        if (q->src_off[s + 1] < q->src_off[s] || q->rec_off[s + 1] < q->rec_off[s])
+            return c->fail(FWI_EINVAL, "CSR offsets decrease at shot %d", s);
+    if (q->src_off[0] != 0 || q->rec_off[0] != 0) return c->fail(FWI_EINVAL, "CSR offsets must start at 0");
+    const int ns = q->src_off[q->nshot], nr = q->rec_off[q->nshot];
+    if ((ns && (!q->src_z || !q->src_x || !q->stf_z || !q->stf_x)) || (nr && (!q->rec_z || !q->rec_x)))
+        return c->fail(FWI_EINVAL, "missing source or receiver arrays");
+    for (int i = 0; i < ns; ++i)
+        if (q->src_z[i] < 0 || q->src_z[i] >= g->nz || q->src_x[i] < 0 || q->src_x[i] >= g->nx)
+            return c->fail(FWI_EINVAL, "source %d at (%d, %d) outside the grid", i, q->src_z[i], q->src_x[i]);
+    for (int i = 0; i < nr; ++i)
+        if (q->rec_z[i] < 0 || q->rec_z[i] >= g->nz || q->rec_x[i] < 0 || q->rec_x[i] >= g->nx)
+            return c->fail(FWI_EINVAL, "receiver %d at (%d, %d) outside the grid", i, q->rec_z[i], q->rec_x[i]);
+    return FWI_OK;
+}
+
+thread_local std::string g_create_error;
+
+}  // namespace
+
+// ===========================================================================
+// C ABI
+// ===========================================================================
+extern "C" {
+
+int32_t fwi_abi_version(void) { return FWI_PSV_ABI_VERSION; }
+
+const char *fwi_strerror(int32_t s) {
+    switch (s) {
+        case FWI_OK: return "ok";
+        case FWI_EINVAL: return "invalid argument";
+        case FWI_ENOMEM: return "out of memory";
+        case FWI_EHIP: return "HIP runtime error";
+        case FWI_ENONFINITE: return "non-finite values";
+        case FWI_ECFL: return "time step above the stability limit";
+        case FWI_ESTEP: return "step-length search failed";
+        case FWI_ESTATE: return "call out of order";
+        case FWI_ENODEV: return "no gfx950 device";
+        default: return "unknown status";
+    }
+}
+
+const char *fwi_last_error(const fwi_ctx *c) { return c ? c->err.c_str() : g_create_error.c_str(); }
+
+void fwi_default_opts(fwi_opts *o) {
+    if (!o) return;
+    std::memset(o, 0, sizeof *o);
+    o->storage = FWI_STORE_AUTO;
+    o->form = FWI_FORM_FUSED;
+    o->grad_sign = -1;
+    o->check_finite = 1;
+}
+
+int32_t fwi_create(const fwi_grid *g, const fwi_acq *q, const fwi_opts *opts, fwi_ctx **out) {
+    if (!out || !g) return FWI_EINVAL;
+    *out = nullptr;
+    fwi_ctx *c = new (std::nothrow) fwi_ctx();
+    if (!c) return FWI_ENOMEM;
+    fwi_default_opts(&c->o);
+    if (opts) c->o = *opts;
+    if (c->o.grad_sign == 0) c->o.grad_sign = -1;
+    int r = validate(c, g, q);
+    if (r == FWI_OK) {
+        int ndev = 0;
+        if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= c->o.device)
+            r = c->fail(FWI_ENODEV, "no HIP device %d", c->o.device);
+        else if (hipSetDevice(c->o.device) != hipSuccess)
+            r = c->fail(FWI_ENODEV, "cannot select device %d", c->o.device);
+    }
+    if (r == FWI_OK) {
+        c->g = *g;
+        c->N = g->fdorder / 2;
+        c->P = (int)round_up(g->nx, kPitchAlign);
+        c->snt = 1 + (g->nt - 1) / g->snap_dt;
+        c->snz = 1 + (g->snap_z2 - g->snap_z1) / g->snap_dz;
+        c->snx = 1 + (g->snap_x2 - g->snap_x1) / g->snap_dx;
+        c->nshot = q->nshot;
+        c->stf_type = q->stf_type;
+        c->src_off.assign(q->src_off, q->src_off + q->nshot + 1);
+        c->rec_off.assign(q->rec_off, q->rec_off + q->nshot + 1);
+        c->nsrc_tot = c->src_off.back();
+        c->nrec_tot = c->rec_off.back();
+        c->tz = c->o.tile_tz > 0 ? c->o.tile_tz : 16;
+        c->plain = c->o.form == FWI_FORM_PLAIN;
+        const double *pz[6] = {g->a_z, g->b_z, g->K_z, g->a_half_z, g->b_half_z, g->K_half_z};
+        const double *px[6] = {g->a_x, g->b_x, g->K_x, g->a_half_x, g->b_half_x, g->K_half_x};
+        for (int k = 0; k < 6; ++k) {
+            c->prof[k].assign(g->nz, k % 3 == 1 || k % 3 == 2 ? 1.0 : 0.0);
+            c->prof[6 + k].assign(g->nx, k % 3 == 1 || k % 3 == 2 ? 1.0 : 0.0);
+            if (g->pml_z) c->prof[k].assign(pz[k], pz[k] + g->nz);
+            if (g->pml_x) c->prof[6 + k].assign(px[k], px[k] + g->nx);
+        }
+        // the context keeps its own copies; the caller's profile pointers are not retained
+        c->g.a_z = c->g.b_z = c->g.K_z = c->g.a_half_z = c->g.b_half_z = c->g.K_half_z = nullptr;
+        c->g.a_x = c->g.b_x = c->g.K_x = c->g.a_half_x = c->g.b_half_x = c->g.K_half_x = nullptr;
+        if (hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking) != hipSuccess)
+            r = c->fail(FWI_EHIP, "stream creation failed");
+    }
+    if (r == FWI_OK) r = setup_device(c, q);
+    if (r != FWI_OK) {
+        g_create_error = c->err;
+        fwi_destroy(c);
+        return r;
+    }
+    *out = c;
+    return FWI_OK;
+}
+
+void fwi_destroy(fwi_ctx *c) {
+    if (!c) return;
+    if (c->st) hipStreamSynchronize(c->st);
+    for (auto &p : c->ev_pending) {
+        hipEventDestroy(p.a);
+        hipEventDestroy(p.b);
+    }
+    for (auto e : c->ev_pool) hipEventDestroy(e);
+    free_all(c);
+    if (c->st) hipStreamDestroy(c->st);
+    delete c;
+}
+
+int32_t fwi_plan(const fwi_ctx *c, int32_t *batch, int32_t *storage, int32_t *ckpt_interval, int64_t *bytes) {
+    if (!c) return FWI_EINVAL;
+    if (batch) *batch = c->B;
+    if (storage) *storage = c->storage;
+    if (ckpt_interval) *ckpt_interval = c->storage == FWI_STORE_CHECKPOINT ? c->S : 0;
+    if (bytes) *bytes = (int64_t)c->bytes;
+    return FWI_OK;
+}
+
+int32_t fwi_set_model(fwi_ctx *c, const double *lam, const double *mu, const double *rho) {
+    if (!c || !lam || !mu || !rho) return FWI_EINVAL;
+    const size_t n = (size_t)c->g.nz * c->g.nx;
+    RETIF(upload(c, c->m_lam, lam, n));
+    RETIF(upload(c, c->m_mu, mu, n));
+    RETIF(upload(c, c->m_rho, rho, n));
+    RETIF(refresh_media(c));
+    c->have_model = 1;
+    return FWI_OK;
+}
+
+int32_t fwi_get_model(fwi_ctx *c, double *lam, double *mu, double *rho) {
+    if (!c || !c->have_model) return c ? c->fail(FWI_ESTATE, "no model") : FWI_EINVAL;
+    const size_t n = (size_t)c->g.nz * c->g.nx;
+    if (lam) HIPCHK(hipMemcpyAsync(lam, c->m_lam, n * 8, hipMemcpyDeviceToHost, c->st));
+    if (mu) HIPCHK(hipMemcpyAsync(mu, c->m_mu, n * 8, hipMemcpyDeviceToHost, c->st));
+    if (rho) HIPCHK(hipMemcpyAsync(rho, c->m_rho, n * 8, hipMemcpyDeviceToHost, c->st));
+    return sync(c);
+}
+
+int32_t fwi_get_media(fwi_ctx *c, double *mu_zx, double *rho_zp, double *rho_xp, double C[3]) {
+    if (!c || !c->have_model) return c ? c->fail(FWI_ESTATE, "no model") : FWI_EINVAL;
+    const int nz = c->g.nz, nx = c->g.nx, P = c->P;
+    std::vector<float> h((size_t)nz * P);
+    const float *src[3] = {c->muzx, c->rhoz, c->rhox};
+    double *dst[3] = {mu_zx, rho_zp, rho_xp};
+    for (int k = 0; k < 3; ++k) {
+        if (!dst[k]) continue;
+        HIPCHK(hipMemcpyAsync(h.data(), src[k], h.size() * 4, hipMemcpyDeviceToHost, c->st));
+        RETIF(sync(c));
+        for (int iz = 0; iz < nz; ++iz)
+            for (int ix = 0; ix < nx; ++ix) dst[k][(size_t)iz * nx + ix] = h[(size_t)iz * P + ix];
+    }
+    if (C) std::memcpy(C, c->C, sizeof(double) * 3);
+    return FWI_OK;
+}
+
+int32_t fwi_set_observed(fwi_ctx *c, const double *oz, const double *ox) {
+    if (!c || !oz || !ox) return FWI_EINVAL;
+    const size_t n = (size_t)c->nrec_tot * c->g.nt;
+    RETIF(upload(c, c->obs_z, oz, n));
+    RETIF(upload(c, c->obs_x, ox, n));
+    RETIF(sync(c));
+    c->have_obs = 1;
+    return FWI_OK;
+}
+
+int32_t fwi_forward(fwi_ctx *c, int32_t shot0, int32_t nshots, double *seis_z, double *seis_x) {
+    if (!c) return FWI_EINVAL;
+    if (!c->have_model) return c->fail(FWI_ESTATE, "fwi_forward before fwi_set_model");
+    std::vector<std::pair<int, int>> bs;
+    if (batches(c, shot0, nshots, bs) != FWI_OK) return c->fail(FWI_EINVAL, "shot range");
+    const int nt = c->g.nt;
+    for (auto [s0, nb] : bs) {
+        RETIF(run_forward(c, s0, nb, false, true, true));
+        const int r0 = c->rec_off[s0], r1 = c->rec_off[s0 + nb], rb = c->rec_off[shot0];
+        const size_t n = (size_t)(r1 - r0) * nt;
+        if (seis_z)
+            HIPCHK(hipMemcpyAsync(seis_z + (size_t)(r0 - rb) * nt, c->syn_z + (size_t)r0 * nt, n * 8,
+                                  hipMemcpyDeviceToHost, c->st));
+        if (seis_x)
+            HIPCHK(hipMemcpyAsync(seis_x + (size_t)(r0 - rb) * nt, c->syn_x + (size_t)r0 * nt, n * 8,
+                                  hipMemcpyDeviceToHost, c->st));
+        RETIF(sync(c));
+    }
+    return FWI_OK;
+}
+
+int32_t fwi_get_wavefield(fwi_ctx *c, int32_t shot, int32_t field, double *out) {
+    if (!c || !out || field < 0 || field > 8) return FWI_EINVAL;
+    if (shot < c->last_s0 || shot >= c->last_s0 + c->last_nb)
+        return c->fail(FWI_ESTATE, "shot %d is not in the last propagated batch", shot);
+    const int nz = c->g.nz, nx = c->g.nx, P = c->P;
+    std::vector<float> h((size_t)nz * P);
+    const float *src = c->f[field] + (size_t)(shot - c->last_s0) * c->fs();
+    HIPCHK(hipMemcpyAsync(h.data(), src, h.size() * 4, hipMemcpyDeviceToHost, c->st));
+    RETIF(sync(c));
+    for (int iz = 0; iz < nz; ++iz)
+        for (int ix = 0; ix < nx; ++ix) out[(size_t)iz * nx + ix] = h[(size_t)iz * P + ix];
+    return FWI_OK;
+}
+
+int32_t fwi_forward_snapshots(fwi_ctx *c, int32_t shot, double *seis_z, double *seis_x, double *vz, double *vx,
+                              double *szz, double *szx, double *sxx) {
+    if (!c) return FWI_EINVAL;
+    if (!c->have_model) return c->fail(FWI_ESTATE, "fwi_forward_snapshots before fwi_set_model");
+    if (shot < 0 || shot >= c->nshot) return c->fail(FWI_EINVAL, "shot %d", shot);
+    if (c->storage != FWI_STORE_FULL) return c->fail(FWI_ESTATE, "snapshots need FULL storage");
+    RETIF(run_forward(c, shot, 1, true, true, true));
+    const int nt = c->g.nt, r0 = c->rec_off[shot], r1 = c->rec_off[shot + 1];
+    const size_t n = (size_t)(r1 - r0) * nt;
+    if (seis_z) HIPCHK(hipMemcpyAsync(seis_z, c->syn_z + (size_t)r0 * nt, n * 8, hipMemcpyDeviceToHost, c->st));
+    if (seis_x) HIPCHK(hipMemcpyAsync(seis_x, c->syn_x + (size_t)r0 * nt, n * 8, hipMemcpyDeviceToHost, c->st));
+    double *outs[5] = {vz, vx, szz, szx, sxx};
+    std::vector<float> h(c->snap_stride);
+    for (int k = 0; k < 5; ++k) {
+        if (!outs[k]) continue;
+        HIPCHK(hipMemcpyAsync(h.data(), c->snap[k], h.size() * 4, hipMemcpyDeviceToHost, c->st));
+        RETIF(sync(c));
+        for (size_t i = 0; i < h.size(); ++i) outs[k][i] = h[i];
+    }
+    return sync(c);
+}
+
+int32_t fwi_gradient(fwi_ctx *c, int32_t shot0, int32_t nshots, double *misfit) {
+    if (!c) return FWI_EINVAL;
+    if (!c->have_model || !c->have_obs) return c->fail(FWI_ESTATE, "fwi_gradient needs a model and observed data");
+    std::vector<std::pair<int, int>> bs;
+    if (batches(c, shot0, nshots, bs) != FWI_OK) return c->fail(FWI_EINVAL, "shot range");
+    const int nz = c->g.nz, nx = c->g.nx;
+    for (auto [s0, nb] : bs) {
+        double alpha = 1;
+        RETIF(batch_gradient(c, s0, nb, nullptr, nullptr, misfit ? misfit + (s0 - shot0) : nullptr, &alpha));
+        const size_t snapcell = (size_t)c->snz * c->snx;
+        for (int s = 0; s < nb; ++s) {
+            const float *gs[3] = {c->gr[0] + s * snapcell, c->gr[1] + s * snapcell, c->gr[2] + s * snapcell};
+            dev_precondition(c->f[7] + s * c->fs(), c->f[8] + s * c->fs(), 1.0 / (alpha * alpha), gs, 1.0 / alpha,
+                             c->G, c->C, c->tmp, c->red, nz, nx, c->P, c->g.snap_z1, c->g.snap_z2, c->g.snap_x1,
+                             c->g.snap_x2, c->g.snap_dz, c->g.snap_dx, c->snz, c->snx, c->st);
+        }
+        RETIF(sync(c));
+    }
+    return FWI_OK;
+}
+
+int32_t fwi_gradient_raw(fwi_ctx *c, int32_t shot0, int32_t nshots, const double *adj_z, const double *adj_x,
+                         double *g_lam, double *g_mu, double *g_rho) {
+    if (!c || !adj_z || !adj_x) return FWI_EINVAL;
+    if (!c->have_model) return c->fail(FWI_ESTATE, "fwi_gradient_raw before fwi_set_model");
+    std::vector<std::pair<int, int>> bs;
+    if (batches(c, shot0, nshots, bs) != FWI_OK) return c->fail(FWI_EINVAL, "shot range");
+    const size_t snapcell = (size_t)c->snz * c->snx;
+    double *outs[3] = {g_lam, g_mu, g_rho};
+    for (int k = 0; k < 3; ++k)
+        if (outs[k]) std::memset(outs[k], 0, snapcell * 8);
+    std::vector<float> h(snapcell * c->B);
+    const int nt = c->g.nt, rb = c->rec_off[shot0];
+    for (auto [s0, nb] : bs) {
+        const size_t off = (size_t)(c->rec_off[s0] - rb) * nt;
+        double alpha = 1;
+        RETIF(batch_gradient(c, s0, nb, adj_z + off, adj_x + off, nullptr, &alpha));
+        for (int k = 0; k < 3; ++k) {
+            if (!outs[k]) continue;
+            HIPCHK(hipMemcpyAsync(h.data(), c->gr[k], snapcell * nb * 4, hipMemcpyDeviceToHost, c->st));
+            RETIF(sync(c));
+            for (int s = 0; s < nb; ++s)
+                for (size_t i = 0; i < snapcell; ++i) outs[k][i] += (double)h[s * snapcell + i] / alpha;
+        }
+    }
+    return FWI_OK;
+}
+
+int32_t fwi_zero_gradient(fwi_ctx *c) {
+    if (!c) return FWI_EINVAL;
+    for (int i = 0; i < 3; ++i)
+        HIPCHK(hipMemsetAsync(c->G[i], 0, sizeof(double) * (size_t)c->g.nz * c->g.nx, c->st));
+    return sync(c);
+}
+
+int32_t fwi_get_gradient(fwi_ctx *c, double *a, double *b, double *d) {
+    if (!c) return FWI_EINVAL;
+    const size_t n = (size_t)c->g.nz * c->g.nx;
+    double *o[3] = {a, b, d};
+    for (int i = 0; i < 3; ++i)
+        if (o[i]) HIPCHK(hipMemcpyAsync(o[i], c->G[i], n * 8, hipMemcpyDeviceToHost, c->st));
+    return sync(c);
+}
+
+int32_t fwi_set_gradient(fwi_ctx *c, const double *a, const double *b, const double *d) {
+    if (!c || !a || !b || !d) return FWI_EINVAL;
+    const size_t n = (size_t)c->g.nz * c->g.nx;
+    RETIF(upload(c, c->G[0], a, n));
+    RETIF(upload(c, c->G[1], b, n));
+    RETIF(upload(c, c->G[2], d, n));
+    return sync(c);
+}
+
+int32_t fwi_taper_gradient(fwi_ctx *c) {
+    if (!c) return FWI_EINVAL;
+    for (int i = 0; i < 3; ++i)
+        dev_taper(c->G[i], c->g.nz, c->g.nx, c->g.snap_z1, c->g.snap_z2, c->g.snap_x1, c->g.snap_x2, c->g.taper,
+                  c->st);
+    return sync(c);
+}
+
+int32_t fwi_misfit(fwi_ctx *c, int32_t shot0, int32_t nshots, double *L2) {
+    if (!c || !L2) return FWI_EINVAL;
+    if (!c->have_model || !c->have_obs) return c->fail(FWI_ESTATE, "fwi_misfit needs a model and observed data");
+    std::vector<std::pair<int, int>> bs;
+    if (batches(c, shot0, nshots, bs) != FWI_OK) return c->fail(FWI_EINVAL, "shot range");
+    double total = 0;
+    std::vector<double> m(c->B);
+    for (auto [s0, nb] : bs) {
+        RETIF(run_forward(c, s0, nb, false, true, false));
+        dev_residual(c->syn_z, c->syn_x, c->obs_z, c->obs_x, c->res_z, c->res_x, c->rec_off_d + s0, nb, c->g.nt,
+                     c->g.dt, c->misfit, c->st);
+        HIPCHK(hipMemcpyAsync(m.data(), c->misfit, sizeof(double) * nb, hipMemcpyDeviceToHost, c->st));
+        RETIF(sync(c));
+        for (int s = 0; s < nb; ++s) total += m[s];
+    }
+    if (!std::isfinite(total)) return c->fail(FWI_ENONFINITE, "misfit is not finite");
+    *L2 = total;
+    return FWI_OK;
+}
+
+int32_t fwi_save_model(fwi_ctx *c) {
+    if (!c || !c->have_model) return c ? c->fail(FWI_ESTATE, "no model") : FWI_EINVAL;
+    const size_t n = (size_t)c->g.nz * c->g.nx * 8;
+    HIPCHK(hipMemcpyAsync(c->o_lam, c->m_lam, n, hipMemcpyDeviceToDevice, c->st));
+    HIPCHK(hipMemcpyAsync(c->o_mu, c->m_mu, n, hipMemcpyDeviceToDevice, c->st));
+    HIPCHK(hipMemcpyAsync(c->o_rho, c->m_rho, n, hipMemcpyDeviceToDevice, c->st));
+    return sync(c);
+}
+
+// update_mat2 for the three parameters (caps and floors of src/cpu/n_simulate_PSV.cpp:542-546)
+int32_t fwi_update_model(fwi_ctx *c, double eps, double rho_factor, double rho_min) {
+    if (!c || !c->have_model) return c ? c->fail(FWI_ESTATE, "no model") : FWI_EINVAL;
+    const size_t n = (size_t)c->g.nz * c->g.nx;
+    double *m[3] = {c->m_lam, c->m_mu, c->m_rho};
+    const double *old[3] = {c->o_lam, c->o_mu, c->o_rho};
+    const double mmax[3] = {4.8e10, 2.7e10, 3000.0}, mmin[3] = {0.0, 0.0, rho_min};
+    const double step[3] = {eps, eps, rho_factor * eps};
+    for (int p = 0; p < 3; ++p) {
+        double gmax = 0, am = 0;
+        RETIF(absmax_host(c, c->G[p], n, &gmax));
+        RETIF(absmax_host(c, old[p], n, &am));
+        if (am < mmax[p]) am = mmax[p];
+        const double factor = am / gmax;
+        dev_update(m[p], old[p], c->G[p], step[p] * factor, mmax[p], mmin[p], n, c->st);
+    }
+    return refresh_media(c);
+}
+
+// step_length_PSV (src/cpu/n_step_length_PSV.cpp:13-209) with update_param 0
+int32_t fwi_step_length(fwi_ctx *c, double est, double L2_0, double *step_out, int32_t *ntrials) {
+    if (!c || !step_out) return FWI_EINVAL;
+    const double scalefac = 2.0;
+    const int stepmax_1 = 30, stepmax_2 = 4;
+    double L2t[3] = {L2_0, 0, 0}, sl[3] = {0, 0, 0};
+    int step1 = 0, step2 = 0, itests = 1, iteste = 1, countstep = 0, trials = 0;
+    while (step2 != 1 || step1 != 1) {
+        for (int itest = itests; itest <= iteste; ++itest) {
+            RETIF(fwi_update_model(c, est, 0.5, 1.5));
+            double L2 = 0;
+            RETIF(fwi_misfit(c, 0, c->nshot, &L2));
+            ++trials;
+            L2t[itest] = L2;
+            sl[itest] = est;
+            sl[0] = 0.0;
+        }
+        if (step1 == 0 && L2t[0] <= L2t[1]) {
+            est = est / scalefac;
+            countstep++;
+        }
+        if (step1 == 1 && L2t[1] < L2t[2]) {
+            sl[2] = est;
+            step2 = 1;
+        }
+        if (step1 == 1 && L2t[1] >= L2t[2]) {
+            sl[2] = est;
+            est = est + est / scalefac;
+            countstep++;
+        }
+        if (step1 == 0 && L2t[0] > L2t[1]) {
+            sl[1] = est;
+            step1 = 1;
+            iteste = itests = 2;
+            countstep = 0;
+            est = est + est / scalefac;
+        }
+        if (step1 == 0 && countstep > stepmax_1) {
+            if (ntrials) *ntrials = trials;
+            return c->fail(FWI_ESTEP, "no step in %d halvings lowered the misfit", stepmax_1);
+        }
+        if (step1 == 1 && countstep > stepmax_2) step1 = step2 = 1;
+    }
+    // calc_opt_step: vertex of the parabola through the three (step, misfit) pairs
+    const double d01 = (L2t[1] - L2t[0]) / (sl[1] - sl[0]);
+    const double d12 = (L2t[2] - L2t[1]) / (sl[2] - sl[1]);
+    const double x2 = (d12 - d01) / (sl[2] - sl[0]);
+    const double x1 = d01 - x2 * (sl[0] + sl[1]);
+    double opt = -x1 / (2.0 * x2);
+    if (opt > sl[2]) opt = sl[2];
+    if (opt < 0.0) opt = sl[1];
+    if (L2t[1] < L2t[0] && L2t[2] < L2t[1]) opt = sl[2];
+    *step_out = opt;
+    if (ntrials) *ntrials = trials;
+    return FWI_OK;
+}
+
+int32_t fwi_iteration(fwi_ctx *c, double *L2_out, double *step_out) {
+    if (!c) return FWI_EINVAL;
+    RETIF(fwi_save_model(c));
+    RETIF(refresh_media(c));
+    std::vector<double> m(c->nshot);
+    RETIF(fwi_gradient(c, 0, c->nshot, m.data()));
+    double L2 = 0;
+    for (double v : m) L2 += v;
+    RETIF(fwi_taper_gradient(c));
+    double step = 0;
+    RETIF(fwi_step_length(c, c->step_est, L2, &step, nullptr));
+    RETIF(fwi_update_model(c, step, 0.5, 1.25));
+    c->step_est = step;
+    if (L2_out) *L2_out = L2;
+    if (step_out) *step_out = step;
+    return FWI_OK;
+}
+
+int32_t fwi_profile(fwi_ctx *c, int32_t enable) {
+    if (!c) return FWI_EINVAL;
+    RETIF(sync(c));
+    c->prof_on = enable != 0;
+    if (enable) std::memset(c->times, 0, sizeof c->times);
+    return FWI_OK;
+}
+
+int32_t fwi_kernel_times(fwi_ctx *c, fwi_kernel_time t[FWI_K_COUNT]) {
+    if (!c || !t) return FWI_EINVAL;
+    RETIF(sync(c));
+    std::memcpy(t, c->times, sizeof c->times);
+    return FWI_OK;
+}
+
+int32_t fwi_sync(fwi_ctx *c) { return c ? sync(c) : FWI_EINVAL; }
+
+}  // extern "C"

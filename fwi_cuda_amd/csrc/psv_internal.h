@@ -1,0 +1,120 @@
+// psv_internal.h -- shared declarations of the gfx950 P/SV library.
+//
+// Device data layout (one context, a batch of B shots advanced together):
+//   float fields [B][nz][P]      vz vx szz szx sxx uz ux We We_adj; P = nx rounded up to 64
+//   float psi_z  [B][nzs][P]     CPML memory of the z-derivatives, only the nzs rows
+//                                where the z profile is active (zmap[iz] >= 0)
+//   float psi_x  [B][nz][nxs]    CPML memory of the x-derivatives, only the nxs active columns
+//   float media  [nz][P]         lam mu mu_zx rho_zp rho_xp (shared by all shots)
+//   float snaps  [B][snt][snz][snx] x5 (full storage) or a checkpoint segment buffer
+//   float grad   [B][snz][snx] x3  raw correlation sums of the batch's shots
+//   double model [nz][nx]        lam mu rho and the saved copies; global gradient x3
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "fwi_psv.h"
+
+namespace psv {
+
+constexpr int kPitchAlign = 64;   // floats: rows start on 256-byte boundaries
+constexpr int kBX = 64;           // stencil block: one wavefront across x ...
+constexpr int kBY = 4;            // ... times four z-segments
+
+// Everything a stencil / injection kernel reads or writes, passed by value.
+struct KArgs {
+    int nz, nx, P, B;
+    size_t fs;                 // field stride per shot = nz * P
+    float dt;
+    float cz[6], cx[6];        // hc[k] / dz, hc[k] / dx
+    // wavefield of the batch
+    float *vz, *vx, *szz, *szx, *sxx, *uz, *ux, *We;
+    // media
+    const float *lam, *mu, *muzx, *rhoz, *rhox;
+    // CPML: maps and profiles (b, a, 1/K) for integer and half points
+    const int *zmap, *xmap;
+    int nzs, nxs;
+    const float *bz, *az, *ikz, *bhz, *ahz, *ikhz;
+    const float *bx, *ax, *ikx, *bhx, *ahx, *ikhx;
+    float *pvz_z, *pvx_z, *pszz_z, *pszx_z;   // [B][nzs][P]
+    float *pvx_x, *pvz_x, *psxx_x, *pszx_x;   // [B][nz][nxs]
+    // velocity-kernel options
+    int energy;                // We += vx^2 + vz^2
+    int displ;                 // u += dt v (adjoint)
+    int grad;                  // correlate with the snapshot slab this step
+    float gw;                  // grad_sign * snap_dt * dt
+    const float *sn_vz, *sn_vx, *sn_szz, *sn_szx, *sn_sxx;  // slab of shot 0 at this step
+    size_t sn_stride;          // between shots
+    float *g_lam, *g_mu, *g_rho;  // [B][snz][snx]
+    int snap_z1, snap_z2, snap_x1, snap_x2, snap_dz, snap_dx, snz, snx;
+};
+
+// Point injection / recording (vsrc2, urec2) for the batch.
+struct SrcArgs {
+    int shot0;                 // first shot of the batch (grp_shot is absolute)
+    int ngroups;               // cells with at least one source, over the batch
+    const int *grp_shot;       // shot of each group
+    const int *grp_cell;       // iz * P + ix
+    const int *grp_off;        // [ngroups + 1] into grp_idx
+    const int *grp_idx;        // source rows into stf (absolute)
+    const double *stf_z, *stf_x;  // [rows][nt]
+    int nt, type;
+    double dt, dzdx, scale;    // scale multiplies the source amplitude (adjoint rescaling)
+};
+
+struct RecArgs {
+    int shot0;                 // first shot of the batch (rec_shot is absolute)
+    int nrec;                  // receivers of the batch
+    const int *rec_shot;       // shot of each receiver
+    const int *rec_cell;       // iz * P + ix
+    double *acc_z, *acc_x;     // running sums [nrec]
+    double *seis_z, *seis_x;   // [nrec][nt] (rows of the batch's first receiver onwards)
+    int nt;
+    double dt, dzdx;
+};
+
+// ---- launchers (psv_kernels.hip) ------------------------------------------------
+void launch_stress(const KArgs &a, int N, int tz, hipStream_t st);
+void launch_velocity(const KArgs &a, int N, int tz, hipStream_t st);
+void launch_surface(const KArgs &a, int N, const int isurf[4], hipStream_t st);
+void launch_inject(const KArgs &a, const SrcArgs &s, int it, hipStream_t st);
+void launch_record(const KArgs &a, const RecArgs &r, int it, hipStream_t st);
+void launch_store(const KArgs &a, float *const dst[5], size_t dst_stride, float dt, hipStream_t st);
+
+// plain reference-structured path (psv_plain.hip)
+struct PlainBufs {
+    float *dz_z, *dx_z, *dz_x, *dx_x;                 // [nz][P] derivative scratch
+    float *mvz_z, *mvx_z, *mszz_z, *mszx_z;           // [nz][P] full-grid CPML memory
+    float *mvz_x, *mvx_x, *mszx_x, *msxx_x;
+    const float *fbz, *faz, *fikz, *fbhz, *fahz, *fikhz;   // full-length profiles
+    const float *fbx, *fax, *fikx, *fbhx, *fahx, *fikhx;
+};
+void plain_step(const KArgs &a, const PlainBufs &p, int N, const int isurf[4], int pml_z, int pml_x,
+                hipStream_t st);
+void plain_grad(const KArgs &a, hipStream_t st);
+
+// ---- small kernels on the model and gradient (psv_misc.hip) ------------------------
+void dev_mat_av(const double *lam, const double *mu, const double *rho, float *flam, float *fmu,
+                float *muzx, float *rhoz, float *rhox, double *sums3, int nz, int nx, int P,
+                hipStream_t st);
+void dev_residual(const double *seis_z, const double *seis_x, const double *obs_z, const double *obs_x,
+                  double *res_z, double *res_x, const int *rec_off_local, int nshots, int nt, double dt,
+                  double *misfit, hipStream_t st);
+void dev_absmax(const double *x, size_t n, double *out, hipStream_t st);
+void dev_precondition(const float *We, const float *Wa, double wa_scale, const float *const g[3],
+                      double g_scale, double *const G[3], const double C[3], double *tmp,
+                      double *wmax, int nz, int nx, int P, int z1, int z2, int x1, int x2, int sdz,
+                      int sdx, int snz, int snx, hipStream_t st);
+void dev_taper(double *A, int nz, int nx, int z1, int z2, int x1, int x2, const int taper[8],
+               hipStream_t st);
+void dev_update(double *mat, const double *old, const double *G, double factor, double mat_max,
+                double mat_min, size_t n, hipStream_t st);
+void dev_count_nonfinite(const float *x, size_t n, int *out, hipStream_t st);
+void dev_fill(float *p, float v, size_t n, hipStream_t st);
+
+}  // namespace psv

@@ -1,0 +1,395 @@
+// psv_kernels.hip -- gfx950 time-step kernels of the elastic P/SV path.
+//
+// One time step is two fused half-step kernels plus O(nx + nrec) epilogues:
+//
+//   k_stress    vdiff2 + pml_diff2 (velocity pass) + update_s2
+//               (src/cpu/n_kernel_lib_PSV.cpp:100-131, 134-178, 181-205)
+//   k_surface   surf_mirror (:272-395), one launch per side in the reference's order
+//   k_velocity  sdiff2 + pml_diff2 (stress pass) + update_v2 (:207-270), and on
+//               snapshot steps of the adjoint the fwi_grad2 correlation (:439-484)
+//   k_inject    vsrc2 (:487-536)       k_record  urec2 (:538-576)
+//   k_store     gard_fwd_storage2 (:398-436)
+//
+// Stencil kernels: a thread owns one column ix and marches down TZ rows.  The
+// z-derivative taps live in a register queue of 2N values per field, so each
+// row of vz, vx (or szz, szx) is fetched once per segment; the x taps are
+// neighbouring lanes of the same wavefront (one 256-byte row segment per load,
+// served by L1/L2).  CPML memory variables exist only in the active strips
+// (zmap / xmap >= 0); elsewhere the CPML update is the identity, as it is in
+// the reference where a = 0, b = 1, K = 1 outside the strips.
+#include "psv_internal.h"
+
+namespace psv {
+
+namespace {
+
+__device__ __forceinline__ size_t fidx(const KArgs &a, int s, int z, int x) {
+    return (size_t)s * a.fs + (size_t)z * a.P + x;
+}
+
+template <int N, int TZ>
+__global__ __launch_bounds__(kBX *kBY) void k_stress(const KArgs a) {
+    const int ix = N + blockIdx.x * kBX + threadIdx.x;
+    const int z0 = N + (blockIdx.y * kBY + threadIdx.y) * TZ;
+    const int nzi = a.nz - N;
+    if (ix >= a.nx - N || z0 >= nzi) return;
+    const int zend = min(z0 + TZ, nzi);
+    const int s = blockIdx.z;
+    const int P = a.P;
+    const size_t fo = (size_t)s * a.fs;
+    const float *__restrict__ vz = a.vz + fo;
+    const float *__restrict__ vx = a.vx + fo;
+    float *__restrict__ szz = a.szz + fo;
+    float *__restrict__ szx = a.szx + fo;
+    float *__restrict__ sxx = a.sxx + fo;
+    const float dt = a.dt;
+
+    const int cxm = a.xmap[ix];
+    float bx = 1.f, ax = 0.f, ikx = 1.f, bhx = 1.f, ahx = 0.f, ikhx = 1.f;
+    if (cxm >= 0) {
+        bx = a.bx[ix]; ax = a.ax[ix]; ikx = a.ikx[ix];
+        bhx = a.bhx[ix]; ahx = a.ahx[ix]; ikhx = a.ikhx[ix];
+    }
+
+    // qz[j] = vz[z - N + j], qx[j] = vx[z - N + 1 + j] for the current row z
+    float qz[2 * N], qx[2 * N];
+#pragma unroll
+    for (int j = 0; j < 2 * N - 1; ++j) {
+        qz[j + 1] = vz[(size_t)(z0 - N + j) * P + ix];
+        qx[j + 1] = vx[(size_t)(z0 - N + 1 + j) * P + ix];
+    }
+    for (int z = z0; z < zend; ++z) {
+#pragma unroll
+        for (int j = 0; j < 2 * N - 1; ++j) {
+            qz[j] = qz[j + 1];
+            qx[j] = qx[j + 1];
+        }
+        qz[2 * N - 1] = vz[(size_t)(z + N - 1) * P + ix];
+        qx[2 * N - 1] = vx[(size_t)(z + N) * P + ix];
+        const size_t r = (size_t)z * P + ix;
+        float vz_z = 0.f, vx_z = 0.f, vz_x = 0.f, vx_x = 0.f;
+#pragma unroll
+        for (int k = 1; k <= N; ++k) {
+            vz_z += a.cz[k - 1] * (qz[N + k - 1] - qz[N - k]);
+            vx_z += a.cz[k - 1] * (qx[N + k - 1] - qx[N - k]);
+            vz_x += a.cx[k - 1] * (vz[r + k] - vz[r - k + 1]);
+            vx_x += a.cx[k - 1] * (vx[r + k - 1] - vx[r - k]);
+        }
+        const int rz = a.zmap[z];
+        if (rz >= 0) {
+            const size_t pi = ((size_t)s * a.nzs + rz) * P + ix;
+            const float p1 = a.bz[z] * a.pvz_z[pi] + a.az[z] * vz_z;
+            const float p2 = a.bhz[z] * a.pvx_z[pi] + a.ahz[z] * vx_z;
+            a.pvz_z[pi] = p1;
+            a.pvx_z[pi] = p2;
+            vz_z = vz_z * a.ikz[z] + p1;
+            vx_z = vx_z * a.ikhz[z] + p2;
+        }
+        if (cxm >= 0) {
+            const size_t pi = ((size_t)s * a.nz + z) * a.nxs + cxm;
+            const float p1 = bx * a.pvx_x[pi] + ax * vx_x;
+            const float p2 = bhx * a.pvz_x[pi] + ahx * vz_x;
+            a.pvx_x[pi] = p1;
+            a.pvz_x[pi] = p2;
+            vx_x = vx_x * ikx + p1;
+            vz_x = vz_x * ikhx + p2;
+        }
+        const float lam = a.lam[r], mu = a.mu[r], muzx = a.muzx[r];
+        const float div = lam * (vx_x + vz_z);
+        szz[r] += dt * (div + 2.f * mu * vz_z);
+        szx[r] += dt * muzx * (vz_x + vx_z);
+        sxx[r] += dt * (div + 2.f * mu * vx_x);
+    }
+}
+
+template <int N, int TZ>
+__global__ __launch_bounds__(kBX *kBY) void k_velocity(const KArgs a) {
+    const int ix = N + blockIdx.x * kBX + threadIdx.x;
+    const int z0 = N + (blockIdx.y * kBY + threadIdx.y) * TZ;
+    const int nzi = a.nz - N;
+    if (ix >= a.nx - N || z0 >= nzi) return;
+    const int zend = min(z0 + TZ, nzi);
+    const int s = blockIdx.z;
+    const int P = a.P;
+    const size_t fo = (size_t)s * a.fs;
+    float *__restrict__ vz = a.vz + fo;
+    float *__restrict__ vx = a.vx + fo;
+    const float *__restrict__ szz = a.szz + fo;
+    const float *__restrict__ szx = a.szx + fo;
+    const float *__restrict__ sxx = a.sxx + fo;
+    const float dt = a.dt;
+
+    const int cxm = a.xmap[ix];
+    float bx = 1.f, ax = 0.f, ikx = 1.f, bhx = 1.f, ahx = 0.f, ikhx = 1.f;
+    if (cxm >= 0) {
+        bx = a.bx[ix]; ax = a.ax[ix]; ikx = a.ikx[ix];
+        bhx = a.bhx[ix]; ahx = a.ahx[ix]; ikhx = a.ikhx[ix];
+    }
+    const bool gcol = a.grad && ix >= a.snap_x1 && ix <= a.snap_x2 && (ix - a.snap_x1) % a.snap_dx == 0;
+    const int jx = gcol ? (ix - a.snap_x1) / a.snap_dx : 0;
+
+    // qzz[j] = szz[z - N + 1 + j], qzx[j] = szx[z - N + j]
+    float qzz[2 * N], qzx[2 * N];
+#pragma unroll
+    for (int j = 0; j < 2 * N - 1; ++j) {
+        qzz[j + 1] = szz[(size_t)(z0 - N + 1 + j) * P + ix];
+        qzx[j + 1] = szx[(size_t)(z0 - N + j) * P + ix];
+    }
+    for (int z = z0; z < zend; ++z) {
+#pragma unroll
+        for (int j = 0; j < 2 * N - 1; ++j) {
+            qzz[j] = qzz[j + 1];
+            qzx[j] = qzx[j + 1];
+        }
+        qzz[2 * N - 1] = szz[(size_t)(z + N) * P + ix];
+        qzx[2 * N - 1] = szx[(size_t)(z + N - 1) * P + ix];
+        const size_t r = (size_t)z * P + ix;
+        float szz_z = 0.f, szx_z = 0.f, szx_x = 0.f, sxx_x = 0.f;
+#pragma unroll
+        for (int k = 1; k <= N; ++k) {
+            szz_z += a.cz[k - 1] * (qzz[N + k - 1] - qzz[N - k]);
+            szx_z += a.cz[k - 1] * (qzx[N + k - 1] - qzx[N - k]);
+            szx_x += a.cx[k - 1] * (szx[r + k - 1] - szx[r - k]);
+            sxx_x += a.cx[k - 1] * (sxx[r + k] - sxx[r - k + 1]);
+        }
+        // the reference reuses the velocity pass's slot-to-profile mapping here
+        // (src/cpu/n_kernel_PSV.cpp:142-146): szz_z and sxx_x take the integer-point
+        // profiles, szx_z and szx_x the half-point ones.
+        const int rz = a.zmap[z];
+        if (rz >= 0) {
+            const size_t pi = ((size_t)s * a.nzs + rz) * P + ix;
+            const float p1 = a.bz[z] * a.pszz_z[pi] + a.az[z] * szz_z;
+            const float p2 = a.bhz[z] * a.pszx_z[pi] + a.ahz[z] * szx_z;
+            a.pszz_z[pi] = p1;
+            a.pszx_z[pi] = p2;
+            szz_z = szz_z * a.ikz[z] + p1;
+            szx_z = szx_z * a.ikhz[z] + p2;
+        }
+        if (cxm >= 0) {
+            const size_t pi = ((size_t)s * a.nz + z) * a.nxs + cxm;
+            const float p1 = bx * a.psxx_x[pi] + ax * sxx_x;
+            const float p2 = bhx * a.pszx_x[pi] + ahx * szx_x;
+            a.psxx_x[pi] = p1;
+            a.pszx_x[pi] = p2;
+            sxx_x = sxx_x * ikx + p1;
+            szx_x = szx_x * ikhx + p2;
+        }
+        const float vz0 = vz[r], vx0 = vx[r];
+        const float vz1 = vz0 + dt * a.rhoz[r] * (szx_x + szz_z);
+        const float vx1 = vx0 + dt * a.rhox[r] * (sxx_x + szx_z);
+        vz[r] = vz1;
+        vx[r] = vx1;
+        float uz = 0.f, ux = 0.f;
+        if (a.displ) {
+            uz = a.uz[fo + r] + dt * vz0;
+            ux = a.ux[fo + r] + dt * vx0;
+            a.uz[fo + r] = uz;
+            a.ux[fo + r] = ux;
+        }
+        if (a.energy) a.We[fo + r] += vx1 * vx1 + vz1 * vz1;
+        if (gcol && z >= a.snap_z1 && z <= a.snap_z2 && (z - a.snap_z1) % a.snap_dz == 0) {
+            const int jz = (z - a.snap_z1) / a.snap_dz;
+            const size_t q = (size_t)s * a.snz * a.snx + (size_t)jz * a.snx + jx;
+            const size_t o = (size_t)s * a.sn_stride + (size_t)jz * a.snx + jx;
+            const float Szz = a.sn_szz[o], Sxx = a.sn_sxx[o], Szx = a.sn_szx[o];
+            const float Vz = a.sn_vz[o], Vx = a.sn_vx[o];
+            const float lam = a.lam[r], mu = a.mu[r];
+            const float c_zz = qzz[N - 1], c_zx = qzx[N], c_xx = sxx[r];
+            const float lm = lam + mu;
+            const float s1 = 0.25f * (Szz + Sxx) * (c_zz + c_xx) / (lm * lm);
+            const float s2 = 0.25f * (Szz - Sxx) * (c_zz - c_xx) / (mu * mu);
+            const float s3 = (Szx * c_zx) / (mu * mu);
+            const float s4 = ux * Vx + uz * Vz;
+            a.g_lam[q] += a.gw * s1;
+            a.g_mu[q] += a.gw * (s3 + s1 + s2);
+            a.g_rho[q] += a.gw * s4;
+        }
+    }
+}
+
+// surf_mirror, one side per launch.  side 0/1: top/bottom row, one thread per
+// column; side 2/3: left/right column, one thread per row.  The tangential
+// velocity derivative is recomputed with the CPML correction the stress kernel
+// applied (its memory variable already holds this step's value).
+template <int N>
+__global__ void k_surface(const KArgs a, int side, int is) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int s = blockIdx.y;
+    const size_t fo = (size_t)s * a.fs;
+    const int P = a.P;
+    const float dt = a.dt;
+    if (side < 2) {
+        const int ix = N + t;
+        if (ix >= a.nx - N) return;
+        const size_t r = (size_t)is * P + ix;
+        float d = 0.f;
+#pragma unroll
+        for (int k = 1; k <= N; ++k) d += a.cx[k - 1] * (a.vx[fo + r + k - 1] - a.vx[fo + r - k]);
+        const int cxm = a.xmap[ix];
+        if (cxm >= 0) d = d * a.ikx[ix] + a.pvx_x[((size_t)s * a.nz + is) * a.nxs + cxm];
+        const float l = a.lam[r], u = a.mu[r];
+        a.szz[fo + r] = 0.f;
+        a.szx[fo + r] = 0.f;
+        a.sxx[fo + r] = 4.f * dt * d * (l * u + u * u) / (l + 2.f * u);
+        if (side == 0) {
+            for (int k = 1; k < is - N + 1; ++k) {
+                a.szx[fo + r - (size_t)k * P] = -a.szx[fo + r + (size_t)k * P];
+                a.szz[fo + r - (size_t)k * P] = -a.szz[fo + r + (size_t)k * P];
+            }
+        } else {
+            for (int k = 1; k <= a.nz - N - is; ++k) {
+                a.szx[fo + r + (size_t)k * P] = -a.szx[fo + r - (size_t)k * P];
+                a.szz[fo + r + (size_t)k * P] = -a.szz[fo + r - (size_t)k * P];
+            }
+        }
+    } else {
+        const int iz = N + t;
+        if (iz >= a.nz - N) return;
+        const size_t r = (size_t)iz * P + is;
+        float d = 0.f;
+#pragma unroll
+        for (int k = 1; k <= N; ++k)
+            d += a.cz[k - 1] * (a.vz[fo + r + (size_t)(k - 1) * P] - a.vz[fo + r - (size_t)k * P]);
+        const int rz = a.zmap[iz];
+        if (rz >= 0) d = d * a.ikz[iz] + a.pvz_z[((size_t)s * a.nzs + rz) * P + is];
+        const float l = a.lam[r], u = a.mu[r];
+        a.sxx[fo + r] = 0.f;
+        a.szx[fo + r] = 0.f;
+        a.szz[fo + r] = 4.f * dt * d * (l * u + u * u) / (l + 2.f * u);
+        if (side == 2) {
+            for (int k = 1; k < is - N + 1; ++k) {
+                a.szx[fo + r - k] = -a.szx[fo + r + k];
+                a.sxx[fo + r - k] = -a.sxx[fo + r + k];
+            }
+        } else {
+            for (int k = 1; k <= a.nx - N - is; ++k) {
+                a.szx[fo + r + k] = -a.szx[fo + r - k];
+                a.sxx[fo + r + k] = -a.sxx[fo + r - k];
+            }
+        }
+    }
+}
+
+// vsrc2: one thread per (shot, cell) group; a group's sources are added in
+// source order, as the reference's serial loop does for sources sharing a cell.
+__global__ void k_inject(const KArgs a, const SrcArgs sa, int it) {
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= sa.ngroups) return;
+    const int s = sa.grp_shot[g] - sa.shot0;
+    const int cell = sa.grp_cell[g];
+    const size_t o = (size_t)s * a.fs + cell;
+    float vz = a.vz[o], vx = a.vx[o];
+    for (int j = sa.grp_off[g]; j < sa.grp_off[g + 1]; ++j) {
+        const size_t row = (size_t)sa.grp_idx[j] * sa.nt + it;
+        const double fz = sa.stf_z[row] * sa.scale, fx = sa.stf_x[row] * sa.scale;
+        if (sa.type == 0) {
+            vz += (float)(sa.dt * (double)a.rhoz[cell] * fz / sa.dzdx);
+            vx += (float)(sa.dt * (double)a.rhox[cell] * fx / sa.dzdx);
+        } else {
+            vz += (float)fz;
+            vx += (float)fx;
+        }
+    }
+    a.vz[o] = vz;
+    a.vx[o] = vx;
+}
+
+// urec2: displacement as the FP64 running sum of dt v / (dz dx).
+__global__ void k_record(const KArgs a, const RecArgs ra, int it) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= ra.nrec) return;
+    const size_t o = (size_t)(ra.rec_shot[r] - ra.shot0) * a.fs + ra.rec_cell[r];
+    const double uz = ra.acc_z[r] + ra.dt * (double)a.vz[o] / ra.dzdx;
+    const double ux = ra.acc_x[r] + ra.dt * (double)a.vx[o] / ra.dzdx;
+    ra.acc_z[r] = uz;
+    ra.acc_x[r] = ux;
+    ra.seis_z[(size_t)r * ra.nt + it] = uz;
+    ra.seis_x[(size_t)r * ra.nt + it] = ux;
+}
+
+// gard_fwd_storage2: stresses as they are, velocities / dt, on the strided box.
+__global__ void k_store(const KArgs a, float *d0, float *d1, float *d2, float *d3, float *d4, size_t dst_stride,
+                        float dt) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    const int s = blockIdx.y;
+    if (q >= a.snz * a.snx) return;
+    const int jz = q / a.snx, jx = q - jz * a.snx;
+    const int iz = a.snap_z1 + jz * a.snap_dz, ix = a.snap_x1 + jx * a.snap_dx;
+    const size_t i = fidx(a, s, iz, ix);
+    const size_t o = (size_t)s * dst_stride + q;
+    d0[o] = a.vz[i] / dt;
+    d1[o] = a.vx[i] / dt;
+    d2[o] = a.szz[i];
+    d3[o] = a.szx[i];
+    d4[o] = a.sxx[i];
+}
+
+}  // namespace
+
+#define PSV_TZ_CASES(KER, N)                                                     \
+    switch (tz) {                                                                \
+        case 4: hipLaunchKernelGGL((KER<N, 4>), grid, block, 0, st, a); break;   \
+        case 8: hipLaunchKernelGGL((KER<N, 8>), grid, block, 0, st, a); break;   \
+        case 32: hipLaunchKernelGGL((KER<N, 32>), grid, block, 0, st, a); break; \
+        default: hipLaunchKernelGGL((KER<N, 16>), grid, block, 0, st, a); break; \
+    }
+
+#define PSV_N_CASES(KER)                   \
+    switch (N) {                           \
+        case 1: PSV_TZ_CASES(KER, 1) break; \
+        case 2: PSV_TZ_CASES(KER, 2) break; \
+        case 3: PSV_TZ_CASES(KER, 3) break; \
+        case 4: PSV_TZ_CASES(KER, 4) break; \
+        case 5: PSV_TZ_CASES(KER, 5) break; \
+        default: PSV_TZ_CASES(KER, 6) break; \
+    }
+
+static dim3 stencil_grid(const KArgs &a, int N, int tz) {
+    const int nxi = a.nx - 2 * N, nzi = a.nz - 2 * N;
+    const int segs = (nzi + tz - 1) / tz;
+    return dim3((nxi + kBX - 1) / kBX, (segs + kBY - 1) / kBY, a.B);
+}
+
+void launch_stress(const KArgs &a, int N, int tz, hipStream_t st) {
+    const dim3 grid = stencil_grid(a, N, tz), block(kBX, kBY);
+    PSV_N_CASES(k_stress)
+}
+
+void launch_velocity(const KArgs &a, int N, int tz, hipStream_t st) {
+    const dim3 grid = stencil_grid(a, N, tz), block(kBX, kBY);
+    PSV_N_CASES(k_velocity)
+}
+
+void launch_surface(const KArgs &a, int N, const int isurf[4], hipStream_t st) {
+    for (int side = 0; side < 4; ++side) {
+        if (isurf[side] <= 0) continue;
+        const int n = side < 2 ? a.nx - 2 * N : a.nz - 2 * N;
+        const dim3 grid((n + 255) / 256, a.B), block(256);
+        switch (N) {
+            case 1: hipLaunchKernelGGL(k_surface<1>, grid, block, 0, st, a, side, isurf[side]); break;
+            case 2: hipLaunchKernelGGL(k_surface<2>, grid, block, 0, st, a, side, isurf[side]); break;
+            case 3: hipLaunchKernelGGL(k_surface<3>, grid, block, 0, st, a, side, isurf[side]); break;
+            case 4: hipLaunchKernelGGL(k_surface<4>, grid, block, 0, st, a, side, isurf[side]); break;
+            case 5: hipLaunchKernelGGL(k_surface<5>, grid, block, 0, st, a, side, isurf[side]); break;
+            default: hipLaunchKernelGGL(k_surface<6>, grid, block, 0, st, a, side, isurf[side]); break;
+        }
+    }
+}
+
+void launch_inject(const KArgs &a, const SrcArgs &s, int it, hipStream_t st) {
+    if (s.ngroups <= 0) return;
+    hipLaunchKernelGGL(k_inject, dim3((s.ngroups + 127) / 128), dim3(128), 0, st, a, s, it);
+}
+
+void launch_record(const KArgs &a, const RecArgs &r, int it, hipStream_t st) {
+    if (r.nrec <= 0) return;
+    hipLaunchKernelGGL(k_record, dim3((r.nrec + 127) / 128), dim3(128), 0, st, a, r, it);
+}
+
+void launch_store(const KArgs &a, float *const dst[5], size_t dst_stride, float dt, hipStream_t st) {
+    const int n = a.snz * a.snx;
+    hipLaunchKernelGGL(k_store, dim3((n + 255) / 256, a.B), dim3(256), 0, st, a, dst[0], dst[1], dst[2], dst[3],
+                       dst[4], dst_stride, dt);
+}
+
+}  // namespace psv

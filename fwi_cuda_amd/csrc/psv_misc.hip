@@ -1,0 +1,312 @@
+// psv_misc.hip -- per-model and per-shot kernels around the time loop.
+//
+// These run once per shot or per model, not per time step, so they favour
+// exactness (FP64, deterministic reduction order) over speed:
+//   mat_av2          src/cpu/n_kernel_lib_PSV.cpp:853-922
+//   adjsrc2          :579-626
+//   energy_weights2  :696-757    interpol_grad2 :628-693    scale_grad_E2 :760-783
+//   taper2           :942-1010   update_mat2    :785-832
+#include "psv_internal.h"
+
+namespace psv {
+
+namespace {
+
+constexpr int kRB = 256;   // reduction block
+constexpr double kPiRef = 3.14159265;   // the reference's truncated pi (include/h_globvar.hpp:13)
+
+template <typename Op>
+__device__ double block_reduce(double v, Op op, double *sh) {
+    const int t = threadIdx.x;
+    sh[t] = v;
+    __syncthreads();
+    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+        if (t < w) sh[t] = op(sh[t], sh[t + w]);
+        __syncthreads();
+    }
+    const double r = sh[0];
+    __syncthreads();
+    return r;
+}
+
+struct OpSum { __device__ double operator()(double a, double b) const { return a + b; } };
+struct OpMax { __device__ double operator()(double a, double b) const { return a > b ? a : b; } };
+
+// mat_av2: harmonic mu at (i+1/2, j+1/2), inverse mean rho at (i+1/2, j) and
+// (i, j+1/2); the rho_xp zero guard writes rho_zp as the reference does.
+__global__ void k_mat_av(const double *lam, const double *mu, const double *rho, float *flam, float *fmu,
+                         float *muzx, float *rhoz, float *rhox, double *part, int nz, int nx, int P) {
+    __shared__ double sh[kRB];
+    const size_t n = (size_t)nz * nx;
+    double sl = 0, sm = 0, sr = 0;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const int iz = (int)(i / nx), ix = (int)(i - (size_t)iz * nx);
+        const size_t o = (size_t)iz * P + ix;
+        flam[o] = (float)lam[i];
+        fmu[o] = (float)mu[i];
+        if (iz < nz - 1 && ix < nx - 1) {
+            const double m00 = mu[i], m01 = mu[i + 1], m10 = mu[i + nx], m11 = mu[i + nx + 1];
+            double v = 4.0 / ((1.0 / m00) + (1.0 / m01) + (1.0 / m10) + (1.0 / m11));
+            if (m00 == 0.0 || m01 == 0.0 || m10 == 0.0 || m11 == 0.0) v = 0.0;
+            const double r00 = rho[i], r10 = rho[i + nx], r01 = rho[i + 1];
+            double rz = 1.0 / (0.5 * (r00 + r10));
+            const double rx = 1.0 / (0.5 * (r00 + r01));
+            if (r00 < 1e-4 && r10 < 1e-4) rz = 0.0;
+            if (r00 < 1e-4 && r01 < 1e-4) rz = 0.0;
+            muzx[o] = (float)v;
+            rhoz[o] = (float)rz;
+            rhox[o] = (float)rx;
+            sl += lam[i];
+            sm += mu[i];
+            sr += rho[i];
+        } else {
+            muzx[o] = 0.f;
+            rhoz[o] = 0.f;
+            rhox[o] = 0.f;
+        }
+    }
+    sl = block_reduce(sl, OpSum(), sh);
+    sm = block_reduce(sm, OpSum(), sh);
+    sr = block_reduce(sr, OpSum(), sh);
+    if (threadIdx.x == 0) {
+        part[3 * blockIdx.x + 0] = sl;
+        part[3 * blockIdx.x + 1] = sm;
+        part[3 * blockIdx.x + 2] = sr;
+    }
+}
+
+__global__ void k_sum_parts3(const double *part, int nparts, double *out, double ncell) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    double a = 0, b = 0, c = 0;
+    for (int i = 0; i < nparts; ++i) {
+        a += part[3 * i];
+        b += part[3 * i + 1];
+        c += part[3 * i + 2];
+    }
+    out[0] = a / ncell;
+    out[1] = b / ncell;
+    out[2] = c / ncell;
+}
+
+// adjsrc2 for a batch: block s handles shot s's receivers.
+__global__ void k_residual(const double *seis_z, const double *seis_x, const double *obs_z, const double *obs_x,
+                           double *res_z, double *res_x, const int *rec_off, int nt, double dt, double *misfit) {
+    __shared__ double sh[kRB];
+    const int s = blockIdx.x;
+    const size_t b = (size_t)rec_off[s] * nt, e = (size_t)rec_off[s + 1] * nt;
+    double acc = 0;
+    for (size_t i = b + threadIdx.x; i < e; i += blockDim.x) {
+        const double rz = seis_z[i] - obs_z[i], rx = seis_x[i] - obs_x[i];
+        res_z[i] = rz;
+        res_x[i] = rx;
+        acc += 0.5 * dt * (rz * rz) + 0.5 * dt * (rx * rx);
+    }
+    acc = block_reduce(acc, OpSum(), sh);
+    if (threadIdx.x == 0) misfit[s] = acc;
+}
+
+__global__ void k_absmax(const double *x, size_t n, double *part) {
+    __shared__ double sh[kRB];
+    double m = 0;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        m = fmax(m, fabs(x[i]));
+    m = block_reduce(m, OpMax(), sh);
+    if (threadIdx.x == 0) part[blockIdx.x] = m;
+}
+
+__global__ void k_max_parts(const double *part, int n, double *out) {
+    __shared__ double sh[kRB];
+    double m = 0;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) m = fmax(m, part[i]);
+    m = block_reduce(m, OpMax(), sh);
+    if (threadIdx.x == 0) *out = m;
+}
+
+// energy_weights2, first pass: We = sqrt(We_fwd * We_adj) on [z1,z2) x [x1,x2), block maxima.
+__global__ void k_energy(const float *We, const float *Wa, double wa_scale, double *Wc, double *part, int nx, int P,
+                         int z1, int z2, int x1, int x2) {
+    __shared__ double sh[kRB];
+    const int bw = x2 - x1;
+    const size_t n = (size_t)(z2 - z1) * bw;
+    double m = 0;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const int iz = z1 + (int)(i / bw), ix = x1 + (int)(i % bw);
+        const size_t f = (size_t)iz * P + ix;
+        const double w = sqrt((double)We[f] * ((double)Wa[f] * wa_scale));
+        Wc[(size_t)iz * nx + ix] = w;
+        m = fmax(m, w);
+    }
+    m = block_reduce(m, OpMax(), sh);
+    if (threadIdx.x == 0) part[blockIdx.x] = m;
+}
+
+__global__ void k_energy_add(double *Wc, const double *wmax, int nx, int z1, int z2, int x1, int x2) {
+    const int bw = x2 - x1;
+    const size_t n = (size_t)(z2 - z1) * bw;
+    const double add = 0.005 * (*wmax);
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const int iz = z1 + (int)(i / bw), ix = x1 + (int)(i % bw);
+        Wc[(size_t)iz * nx + ix] += add;
+    }
+}
+
+// interpol_grad2 phase 1: snap points onto a zeroed full grid
+__global__ void k_interp_points(const float *g, double g_scale, double *T, int nx, int z1, int x1, int sdz, int sdx,
+                                int snz, int snx) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= snz * snx) return;
+    const int jz = q / snx, jx = q - jz * snx;
+    T[(size_t)(z1 + jz * sdz) * nx + x1 + jx * sdx] = (double)g[q] * g_scale;
+}
+
+// phase 2: along x on snap rows iz < z2 (one thread per row segment)
+__global__ void k_interp_x(double *T, int nx, int z1, int z2, int x1, int x2, int sdz, int sdx) {
+    const int nrow = (z2 - z1 + sdz - 1) / sdz, nseg = (x2 - x1 + sdx - 1) / sdx;
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= nrow * nseg) return;
+    const int iz = z1 + (q / nseg) * sdz, ix = x1 + (q % nseg) * sdx;
+    double *row = T + (size_t)iz * nx;
+    const double t = (row[ix + sdx] - row[ix]) / sdx;
+    for (int k = 1; k < sdx; ++k) row[ix + k] = row[ix] + t * k;
+}
+
+// phase 3: along z for every column x1 <= ix < x2 (one thread per segment column)
+__global__ void k_interp_z(double *T, int nx, int z1, int z2, int x1, int x2, int sdz) {
+    const int nseg = (z2 - z1 + sdz - 1) / sdz, bw = x2 - x1;
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= nseg * bw) return;
+    const int iz = z1 + (q / bw) * sdz, ix = x1 + q % bw;
+    const double t = (T[(size_t)(iz + sdz) * nx + ix] - T[(size_t)iz * nx + ix]) / sdz;
+    for (int k = 1; k < sdz; ++k) T[(size_t)(iz + k) * nx + ix] = T[(size_t)iz * nx + ix] + t * k;
+}
+
+// scale_grad_E2: G += T / (We C^2) on [z1,z2) x [x1,x2)
+__global__ void k_scale(double *G, const double *T, const double *Wc, double Cm, int nx, int z1, int z2, int x1,
+                        int x2) {
+    const int bw = x2 - x1;
+    const size_t n = (size_t)(z2 - z1) * bw;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const size_t o = (size_t)(z1 + (int)(i / bw)) * nx + x1 + (int)(i % bw);
+        G[o] += T[o] / (Wc[o] * Cm * Cm);
+    }
+}
+
+// taper2: x factor then z factor, per cell
+__global__ void k_taper(double *A, int nz, int nx, int z1, int z2, int x1, int x2, int t1, int t2, int b1, int b2,
+                        int l1, int l2, int r1, int r2) {
+    const size_t n = (size_t)nz * nx;
+    const int tl = l2 - l1, tr = r1 - r2, tt = t2 - t1, tb = b1 - b2;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const int iz = (int)(i / nx), ix = (int)(i - (size_t)iz * nx);
+        double a = A[i];
+        if (ix >= x1 && ix < l1) a *= 0.0;
+        else if (ix >= l1 && ix < l2) a *= 0.5 * (1.0 - cos(kPiRef * (ix - l1) / tl));
+        else if (ix > r2 && ix < r1) a *= 0.5 * (1.0 - cos(kPiRef * (r1 - ix) / tr));
+        else if (ix >= r1 && ix <= x2) a *= 0.0;
+        if (iz >= z1 && iz < t1) a *= 0.0;
+        else if (iz >= t1 && iz < t2) a *= 0.5 * (1.0 - cos(kPiRef * (iz - t1) / tt));
+        else if (iz > b2 && iz < b1) a *= 0.5 * (1.0 - cos(kPiRef * (b1 - iz) / tb));
+        else if (iz >= b1 && iz <= z2) a *= 0.0;
+        A[i] = a;
+    }
+}
+
+__global__ void k_update(double *mat, const double *old, const double *G, double f, double mmax, double mmin,
+                         size_t n) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        double v = old[i] + f * G[i];
+        if (v > mmax) v = mmax;
+        if (v < mmin) v = mmin;
+        mat[i] = v;
+    }
+}
+
+__global__ void k_nonfinite(const float *x, size_t n, int *out) {
+    int c = 0;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        c += !isfinite(x[i]);
+    if (c) atomicAdd(out, c);
+}
+
+__global__ void k_fill(float *p, float v, size_t n) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+inline int nblocks(size_t n, int cap = 1024) {
+    size_t b = (n + kRB - 1) / kRB;
+    return (int)(b < (size_t)cap ? (b ? b : 1) : cap);
+}
+
+}  // namespace
+
+void dev_mat_av(const double *lam, const double *mu, const double *rho, float *flam, float *fmu, float *muzx,
+                float *rhoz, float *rhox, double *sums3, int nz, int nx, int P, hipStream_t st) {
+    const int nb = nblocks((size_t)nz * nx, 512);
+    // partial sums live right after the three means
+    double *part = sums3 + 4;
+    hipLaunchKernelGGL(k_mat_av, dim3(nb), dim3(kRB), 0, st, lam, mu, rho, flam, fmu, muzx, rhoz, rhox, part, nz,
+                       nx, P);
+    hipLaunchKernelGGL(k_sum_parts3, dim3(1), dim3(1), 0, st, part, nb, sums3, (double)((nz - 1) * (nx - 1)));
+}
+
+void dev_residual(const double *seis_z, const double *seis_x, const double *obs_z, const double *obs_x,
+                  double *res_z, double *res_x, const int *rec_off_local, int nshots, int nt, double dt,
+                  double *misfit, hipStream_t st) {
+    hipLaunchKernelGGL(k_residual, dim3(nshots), dim3(kRB), 0, st, seis_z, seis_x, obs_z, obs_x, res_z, res_x,
+                       rec_off_local, nt, dt, misfit);
+}
+
+void dev_absmax(const double *x, size_t n, double *out, hipStream_t st) {
+    const int nb = nblocks(n, 512);
+    double *part = out + 1;
+    hipLaunchKernelGGL(k_absmax, dim3(nb), dim3(kRB), 0, st, x, n, part);
+    hipLaunchKernelGGL(k_max_parts, dim3(1), dim3(kRB), 0, st, part, nb, out);
+}
+
+void dev_precondition(const float *We, const float *Wa, double wa_scale, const float *const g[3], double g_scale,
+                      double *const G[3], const double C[3], double *tmp, double *wmax, int nz, int nx, int P,
+                      int z1, int z2, int x1, int x2, int sdz, int sdx, int snz, int snx, hipStream_t st) {
+    double *Wc = tmp, *T = tmp + (size_t)nz * nx;
+    const size_t nbox = (size_t)(z2 - z1) * (x2 - x1);
+    const int nb = nblocks(nbox, 512);
+    double *part = wmax + 1;
+    hipLaunchKernelGGL(k_energy, dim3(nb), dim3(kRB), 0, st, We, Wa, wa_scale, Wc, part, nx, P, z1, z2, x1, x2);
+    hipLaunchKernelGGL(k_max_parts, dim3(1), dim3(kRB), 0, st, part, nb, wmax);
+    hipLaunchKernelGGL(k_energy_add, dim3(nb), dim3(kRB), 0, st, Wc, wmax, nx, z1, z2, x1, x2);
+    for (int p = 0; p < 3; ++p) {
+        if (!(C[p] > 0)) continue;
+        (void)hipMemsetAsync(T, 0, sizeof(double) * (size_t)nz * nx, st);
+        hipLaunchKernelGGL(k_interp_points, dim3((snz * snx + 255) / 256), dim3(256), 0, st, g[p], g_scale, T, nx,
+                           z1, x1, sdz, sdx, snz, snx);
+        if (sdx > 1) {
+            const int n = ((z2 - z1 + sdz - 1) / sdz) * ((x2 - x1 + sdx - 1) / sdx);
+            hipLaunchKernelGGL(k_interp_x, dim3((n + 255) / 256), dim3(256), 0, st, T, nx, z1, z2, x1, x2, sdz, sdx);
+        }
+        if (sdz > 1) {
+            const int n = ((z2 - z1 + sdz - 1) / sdz) * (x2 - x1);
+            hipLaunchKernelGGL(k_interp_z, dim3((n + 255) / 256), dim3(256), 0, st, T, nx, z1, z2, x1, x2, sdz);
+        }
+        hipLaunchKernelGGL(k_scale, dim3(nb), dim3(kRB), 0, st, G[p], T, Wc, C[p], nx, z1, z2, x1, x2);
+    }
+}
+
+void dev_taper(double *A, int nz, int nx, int z1, int z2, int x1, int x2, const int taper[8], hipStream_t st) {
+    hipLaunchKernelGGL(k_taper, dim3(nblocks((size_t)nz * nx)), dim3(kRB), 0, st, A, nz, nx, z1, z2, x1, x2, taper[0],
+                       taper[1], taper[2], taper[3], taper[4], taper[5], taper[6], taper[7]);
+}
+
+void dev_update(double *mat, const double *old, const double *G, double factor, double mat_max, double mat_min,
+                size_t n, hipStream_t st) {
+    hipLaunchKernelGGL(k_update, dim3(nblocks(n)), dim3(kRB), 0, st, mat, old, G, factor, mat_max, mat_min, n);
+}
+
+void dev_count_nonfinite(const float *x, size_t n, int *out, hipStream_t st) {
+    hipLaunchKernelGGL(k_nonfinite, dim3(nblocks(n)), dim3(kRB), 0, st, x, n, out);
+}
+
+void dev_fill(float *p, float v, size_t n, hipStream_t st) {
+    hipLaunchKernelGGL(k_fill, dim3(nblocks(n)), dim3(kRB), 0, st, p, v, n);
+}
+
+}  // namespace psv

@@ -1,0 +1,221 @@
+/*
+ * fwi_psv.h -- C ABI of the MI355X (gfx950) elastic P/SV FWI hot path.
+ *
+ * One library, libfwi_psv.so, built from the HIP sources in fwi_cuda_amd/csrc.  Plain C
+ * types only: host pointers to row-major arrays, sizes, and an opaque context.
+ *
+ * What each entry point replaces in the reference (Mohammad-debug/FWI_CUDA):
+ *
+ *   fwi_holbergcoeff      holbergcoeff            src/h_holbergcoeff.cpp:21-80
+ *   fwi_cpml              cpml_PSV                src/h_preproc_PSV.cpp:112-263
+ *   fwi_wavelet           wavelet                 src/h_preproc_PSV.cpp:267-286
+ *   fwi_dtstab            checkfd_ssg_elastic     src/h_checkfd_ssg_elastic.cpp:14-144
+ *   fwi_create            alloc_varmain_PSV + the host->device copy
+ *                                                 src/cpu/n_alloc_PSV.cpp:96-216,
+ *                                                 src/cuda/d_preproc.cu:97-202
+ *   fwi_set_model         mat_av2 on a new model  src/cpu/n_kernel_lib_PSV.cpp:853-922
+ *   fwi_forward           kernel_PSV, forward     src/cpu/n_kernel_PSV.cpp:20-213 (grad = false)
+ *                         (per shot as in simulate_fwd_PSV, src/cpu/n_simulate_PSV.cpp:105-147)
+ *   fwi_gradient          one shot of the FWI loop: kernel_PSV forward, adjsrc2,
+ *                         kernel_PSV adjoint, energy_weights2, interpol_grad2,
+ *                         scale_grad_E2           src/cpu/n_simulate_PSV.cpp:270-418
+ *   fwi_gradient_raw      kernel_PSV adjoint with caller-given adjoint sources,
+ *                         fwi_grad2 sums          src/cpu/n_kernel_lib_PSV.cpp:439-484
+ *   fwi_taper_gradient    taper2 x3               src/cpu/n_kernel_lib_PSV.cpp:942-1010
+ *   fwi_save_model        copy_mat                src/cpu/n_kernel_lib_PSV.cpp:834-851
+ *   fwi_update_model      update_mat2 x3          src/cpu/n_simulate_PSV.cpp:542-546
+ *   fwi_misfit            trial forwards + adjsrc2 sum
+ *                                                 src/cpu/n_step_length_PSV.cpp:97-122
+ *   fwi_step_length       step_length_PSV + calc_opt_step
+ *                                                 src/cpu/n_step_length_PSV.cpp:13-209,461-518
+ *   fwi_iteration         one iteration of simulate_fwi_PSV
+ *                                                 src/cpu/n_simulate_PSV.cpp:249-582
+ *
+ * Conventions (the reference has none: void functions that exit(0) on errors):
+ *   - every call returns FWI_OK (0) or a negative FWI_E* code and never exits;
+ *     fwi_last_error(ctx) gives the message of the last failure;
+ *   - the context owns all device memory; the caller owns host buffers;
+ *   - calls on one context run in order on the context's HIP stream and
+ *     return when their host outputs are written; one context per host thread.
+ *
+ * Array layouts follow the reference: fields row-major [nz][nx] (z = depth =
+ * row, x fastest, src/cpu/n_alloc_PSV.cpp:129-170), seismograms [nrec][nt]
+ * per shot (src/h_io_PSV.cpp:292-317).  Shots' receivers are concatenated
+ * (CSR over shots), so shots may have different receiver counts.
+ */
+#ifndef FWI_PSV_H
+#define FWI_PSV_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FWI_PSV_ABI_VERSION 1
+
+enum fwi_status {
+    FWI_OK = 0,
+    FWI_EINVAL = -1,     /* bad argument or inconsistent sizes */
+    FWI_ENOMEM = -2,     /* device or host allocation failed, or the budget is too small */
+    FWI_EHIP = -3,       /* HIP runtime error */
+    FWI_ENONFINITE = -4, /* NaN or Inf in a wavefield or a misfit */
+    FWI_ECFL = -5,       /* dt above the stability limit of the stencil */
+    FWI_ESTEP = -6,      /* step-length search found no decrease (the reference exits) */
+    FWI_ESTATE = -7,     /* call out of order, e.g. a gradient before fwi_set_observed */
+    FWI_ENODEV = -8      /* no gfx950 device */
+};
+
+/* Wavefield storage for the gradient. */
+enum fwi_storage {
+    FWI_STORE_AUTO = 0,        /* full snapshots if they fit the budget, else checkpoints */
+    FWI_STORE_FULL = 1,        /* every snap_dt-th forward state kept (the reference's scheme) */
+    FWI_STORE_CHECKPOINT = 2   /* checkpoints + forward recomputation per segment */
+};
+
+/* Kernel structure of a time step. */
+enum fwi_form {
+    FWI_FORM_FUSED = 0,  /* two fused half-step kernels, CPML in strips only */
+    FWI_FORM_PLAIN = 1   /* the reference GPU path's structure: one kernel per sub-step,
+                            CPML over the whole interior, a device sync after each launch
+                            (src/cuda/d_kernel_PSV.cu:16-220) */
+};
+
+typedef struct fwi_ctx fwi_ctx;
+
+/* Grid, stencil, boundaries: everything simulate_PSV derives from metaint /
+ * metafloat before calling kernel_PSV (src/h_simulate_PSV.cpp:101-185). */
+typedef struct {
+    int32_t nt, nz, nx;
+    double dt, dz, dx;
+    int32_t fdorder;            /* 2, 4, ..., 12; the stencil uses hc[1..fdorder/2] */
+    double hc[7];               /* fwi_holbergcoeff output */
+    int32_t isurf[4];           /* free-surface index: top, bottom, left, right; 0 = none */
+    int32_t pml_z, pml_x;       /* CPML present along z / x */
+    const double *a_z, *b_z, *K_z, *a_half_z, *b_half_z, *K_half_z;  /* [nz] (fwi_cpml) */
+    const double *a_x, *b_x, *K_x, *a_half_x, *b_half_x, *K_half_x;  /* [nx] */
+    int32_t snap_z1, snap_z2, snap_x1, snap_x2;  /* gradient box (inclusive for storage) */
+    int32_t snap_dt, snap_dz, snap_dx;           /* storage strides */
+    int32_t taper[8];           /* t1 t2 b1 b2 l1 l2 r1 r2 (metaint[20..27]) */
+} fwi_grid;
+
+/* Acquisition as CSR lists over shots.  The reference's form (nsrc sources
+ * with src_shot_to_fire, every shot recording the same nrec receivers,
+ * src/h_io_PSV.cpp:123-153) is the special case rec_off[s] = s * nrec. */
+typedef struct {
+    int32_t nshot;
+    int32_t stf_type;             /* 0: v += dt rho^-1 s / (dz dx); 1: v += s (vsrc2) */
+    const int32_t *src_off;       /* [nshot + 1] */
+    const int32_t *src_z, *src_x; /* [src_off[nshot]] grid indices */
+    const double *stf_z, *stf_x;  /* [src_off[nshot]][nt] */
+    const int32_t *rec_off;       /* [nshot + 1] */
+    const int32_t *rec_z, *rec_x; /* [rec_off[nshot]] */
+} fwi_acq;
+
+typedef struct {
+    int32_t batch;          /* shots advanced together in one launch; 0 = as many as memory allows */
+    int32_t storage;        /* enum fwi_storage */
+    int32_t ckpt_interval;  /* time steps per checkpoint segment; 0 = chosen from memory */
+    int64_t mem_budget;     /* device bytes the context may allocate; 0 = 90 % of free memory */
+    int32_t form;           /* enum fwi_form */
+    int32_t grad_sign;      /* -1 (default): the CPU path's "grad -= ..." (n_kernel_lib_PSV.cpp:477-479);
+                               +1: the GPU path's "+=" (d_kernel_lib_PSV.cu:838-840) */
+    int32_t use_graph;      /* capture time-step chunks in hipGraphs and replay them */
+    int32_t tile_tz;        /* rows per thread in the stencil kernels; 0 = tuned */
+    int32_t device;         /* HIP device ordinal */
+    int32_t check_finite;   /* scan the wavefield for NaN/Inf after each propagation */
+} fwi_opts;
+
+/* Per-kernel timing collected while profiling is on (HIP events on the
+ * context's stream around every launch of that kernel). */
+typedef struct {
+    int64_t launches;
+    double total_ms;
+} fwi_kernel_time;
+
+enum fwi_kernel_id {
+    FWI_K_STRESS = 0,     /* stress half-step (vdiff2 + pml_diff2 + update_s2) */
+    FWI_K_VELOCITY = 1,   /* velocity half-step (sdiff2 + pml_diff2 + update_v2 [+ fwi_grad2]) */
+    FWI_K_SURFACE = 2,    /* surf_mirror */
+    FWI_K_SOURCE = 3,     /* vsrc2 */
+    FWI_K_RECEIVER = 4,   /* urec2 */
+    FWI_K_STORE = 5,      /* gard_fwd_storage2 / checkpoint copies */
+    FWI_K_OTHER = 6,
+    FWI_K_COUNT = 7
+};
+
+/* ---- library ------------------------------------------------------------- */
+int32_t fwi_abi_version(void);
+const char *fwi_strerror(int32_t status);
+const char *fwi_last_error(const fwi_ctx *ctx);
+void fwi_default_opts(fwi_opts *opts);
+
+/* ---- host preprocessing (no device needed) -------------------------------- */
+int32_t fwi_holbergcoeff(int32_t fdorder, int32_t maxrelerror, double hc[7]);
+int32_t fwi_cpml(double *a, double *b, double *K, double *a_half, double *b_half, double *K_half,
+                 double npower, double damp_v, double rcoef, double k_max, double freq,
+                 int32_t npml_h1, int32_t npml_h2, int32_t fpad, int32_t nh, double dt, double dh);
+int32_t fwi_wavelet(double *signal, int32_t nt, double dt, double amp, double fc, double ts, int32_t shape);
+/* Stability limit dh / (sqrt 2 sum|hc| cmax); cmax/cmin returned if non-null. */
+double fwi_dtstab(int32_t nz, int32_t nx, double dh, int32_t fw, const double *rho, const double *lam,
+                  const double *mu, const double hc[7], double *cmax, double *cmin);
+
+/* ---- context ----------------------------------------------------------------- */
+int32_t fwi_create(const fwi_grid *grid, const fwi_acq *acq, const fwi_opts *opts, fwi_ctx **out);
+void fwi_destroy(fwi_ctx *ctx);
+/* Batch size and storage the context chose, and its device memory in bytes. */
+int32_t fwi_plan(const fwi_ctx *ctx, int32_t *batch, int32_t *storage, int32_t *ckpt_interval,
+                 int64_t *device_bytes);
+
+/* Model lam, mu, rho [nz][nx]; derives mu_zx, rho_zp, rho_xp and the scalar means
+ * (mat_av2) on the device. */
+int32_t fwi_set_model(fwi_ctx *ctx, const double *lam, const double *mu, const double *rho);
+int32_t fwi_get_model(fwi_ctx *ctx, double *lam, double *mu, double *rho);
+/* Averaged media as used by the stencil ([nz][nx]) and the means C[3] (lam, mu, rho). */
+int32_t fwi_get_media(fwi_ctx *ctx, double *mu_zx, double *rho_zp, double *rho_xp, double C[3]);
+/* Observed displacement seismograms, CSR over shots: [rec_off[nshot]][nt]. */
+int32_t fwi_set_observed(fwi_ctx *ctx, const double *obs_z, const double *obs_x);
+
+/* Forward modelling of shots [shot0, shot0 + nshots): receiver displacement
+ * seismograms into seis_z / seis_x (CSR rows rec_off[shot0] .. rec_off[shot0 + nshots]). */
+int32_t fwi_forward(fwi_ctx *ctx, int32_t shot0, int32_t nshots, double *seis_z, double *seis_x);
+/* Final wavefield of one shot of the last propagation: field 0..8 = vz vx uz ux szz szx sxx We We_adj. */
+int32_t fwi_get_wavefield(fwi_ctx *ctx, int32_t shot, int32_t field, double *out);
+/* Forward of one shot keeping its snapshots (the accu_* cubes that write_accu saves,
+ * src/h_io_PSV.cpp:246-288): each output is [snap_nt][snap_nz][snap_nx], velocities
+ * divided by dt as gard_fwd_storage2 stores them.  Needs FULL storage. */
+int32_t fwi_forward_snapshots(fwi_ctx *ctx, int32_t shot, double *seis_z, double *seis_x, double *vz,
+                              double *vx, double *szz, double *szx, double *sxx);
+
+/* Gradient of shots [shot0, shot0 + nshots) against the observed data:
+ * misfit[i] (may be null) receives each shot's L2, and the energy-weighted,
+ * mean-scaled shot gradients are added to the context's global gradient. */
+int32_t fwi_gradient(fwi_ctx *ctx, int32_t shot0, int32_t nshots, double *misfit);
+/* Raw correlation gradient sum over the shots (fwi_grad2 output, snap grid
+ * [snap_nz][snap_nx]) for caller-given adjoint sources adj_z / adj_x
+ * (CSR like the seismograms, injected as type-0 sources in reversed time). */
+int32_t fwi_gradient_raw(fwi_ctx *ctx, int32_t shot0, int32_t nshots, const double *adj_z,
+                         const double *adj_x, double *g_lam, double *g_mu, double *g_rho);
+int32_t fwi_zero_gradient(fwi_ctx *ctx);
+int32_t fwi_get_gradient(fwi_ctx *ctx, double *g_lam, double *g_mu, double *g_rho);
+int32_t fwi_set_gradient(fwi_ctx *ctx, const double *g_lam, const double *g_mu, const double *g_rho);
+int32_t fwi_taper_gradient(fwi_ctx *ctx);
+
+/* Sum of the shots' L2 misfits for the current model (forward only). */
+int32_t fwi_misfit(fwi_ctx *ctx, int32_t shot0, int32_t nshots, double *L2);
+int32_t fwi_save_model(fwi_ctx *ctx);
+/* m = clip(m_old + eps f G): lam, mu with eps, rho with rho_factor * eps. */
+int32_t fwi_update_model(fwi_ctx *ctx, double eps, double rho_factor, double rho_min);
+int32_t fwi_step_length(fwi_ctx *ctx, double est, double L2_0, double *step_out, int32_t *ntrials);
+/* One simulate_fwi_PSV iteration over all shots; returns its misfit and step. */
+int32_t fwi_iteration(fwi_ctx *ctx, double *L2, double *step);
+
+/* ---- profiling ------------------------------------------------------------------ */
+int32_t fwi_profile(fwi_ctx *ctx, int32_t enable);
+int32_t fwi_kernel_times(fwi_ctx *ctx, fwi_kernel_time times[FWI_K_COUNT]);
+int32_t fwi_sync(fwi_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* FWI_PSV_H */

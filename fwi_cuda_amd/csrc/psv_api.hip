@@ -35,6 +35,9 @@ struct EventPair {
 
 size_t round_up(size_t v, size_t m) { return (v + m - 1) / m * m; }
 
+// gain on the lam / mu correlation terms (divided out in FP64 on the host side)
+constexpr double kGainLM = 1152921504606846976.0;   // 2^60
+
 }  // namespace
 
 struct fwi_ctx {
@@ -72,6 +75,8 @@ struct fwi_ctx {
     int *rec_shot = nullptr, *rec_cell = nullptr, *rec_off_d = nullptr;
     double *syn_z = nullptr, *syn_x = nullptr, *obs_z = nullptr, *obs_x = nullptr, *res_z = nullptr,
            *res_x = nullptr, *acc = nullptr, *misfit = nullptr;
+    double *alpha_d = nullptr, *shotmax = nullptr;   // per-shot adjoint source scale, residual maxima
+    std::vector<double> alpha;
     // ---- device: batch
     float *f[9] = {};      // vz vx szz szx sxx uz ux We Wa   [B][nz][P]
     float *psi[8] = {};    // pvz_z pvx_z pszz_z pszx_z [B][nzs][P]; pvx_x pvz_x psxx_x pszx_x [B][nz][nxs]
@@ -186,6 +191,13 @@ int sync(fwi_ctx *c) {
     return FWI_OK;
 }
 
+template <class T>
+int upload(fwi_ctx *c, T *dst, const T *src, size_t n) {
+    if (n == 0) return FWI_OK;
+    HIPCHK(hipMemcpyAsync(dst, src, n * sizeof(T), hipMemcpyHostToDevice, c->st));
+    return FWI_OK;
+}
+
 // ---------------------------------------------------------------------------
 // kernel arguments
 // ---------------------------------------------------------------------------
@@ -236,6 +248,7 @@ KArgs base_args(fwi_ctx *c, int nb, float *const fld[5], float *const ps[8]) {
     a.g_mu = c->gr[1];
     a.g_rho = c->gr[2];
     a.gw = (float)(c->o.grad_sign * (double)c->g.snap_dt * c->g.dt);
+    a.gain_lm = (float)kGainLM;
     return a;
 }
 
@@ -254,12 +267,12 @@ SrcArgs src_args(fwi_ctx *c, int s0, int nb) {
     s.type = c->stf_type;
     s.dt = c->g.dt;
     s.dzdx = c->g.dz * c->g.dx;
-    s.scale = 1.0;
+    s.scale_shot = nullptr;
     return s;
 }
 
 // adjoint sources: the receivers of each shot, residual rows as source functions
-SrcArgs adj_args(fwi_ctx *c, int s0, int nb, double scale) {
+SrcArgs adj_args(fwi_ctx *c, int s0, int nb) {
     SrcArgs s{};
     s.shot0 = s0;
     const int g0 = c->rg_first[s0], g1 = c->rg_first[s0 + nb];
@@ -274,7 +287,7 @@ SrcArgs adj_args(fwi_ctx *c, int s0, int nb, double scale) {
     s.type = 0;
     s.dt = c->g.dt;
     s.dzdx = c->g.dz * c->g.dx;
-    s.scale = scale;
+    s.scale_shot = c->alpha_d;
     return s;
 }
 
@@ -432,8 +445,8 @@ void adj_step(fwi_ctx *c, KArgs a, const SrcArgs &sa, int it, size_t slab_off) {
 }
 
 // Adjoint pass (kernel_PSV with grad = true): residuals in res_* are injected at
-// the receivers in reversed time, scaled by `alpha` (a power of two).
-int run_adjoint(fwi_ctx *c, int s0, int nb, double alpha) {
+// the receivers in reversed time, each shot's scaled by its alpha (a power of two).
+int run_adjoint(fwi_ctx *c, int s0, int nb) {
     float *fld[5] = {c->f[0], c->f[1], c->f[2], c->f[3], c->f[4]};
     RETIF(zero_state(c, nb, fld, c->psi, false));
     for (int i = 5; i < 7; ++i) HIPCHK(hipMemsetAsync(c->f[i], 0, sizeof(float) * nb * c->fs(), c->st));
@@ -444,7 +457,7 @@ int run_adjoint(fwi_ctx *c, int s0, int nb, double alpha) {
     a.We = c->f[8];
     a.energy = 1;
     a.displ = 1;
-    const SrcArgs sa = adj_args(c, s0, nb, alpha);
+    const SrcArgs sa = adj_args(c, s0, nb);
     const int nt = c->g.nt, sdt = c->g.snap_dt;
     const size_t slab = (size_t)c->snz * c->snx;
     if (c->storage != FWI_STORE_CHECKPOINT) {
@@ -506,9 +519,9 @@ int batches(const fwi_ctx *c, int shot0, int nshots, std::vector<std::pair<int, 
     return FWI_OK;
 }
 
-// forward + residual + adjoint for one batch; leaves raw gradients in gr (scaled by alpha)
-int batch_gradient(fwi_ctx *c, int s0, int nb, const double *adj_z, const double *adj_x, double *misfit_out,
-                   double *alpha_out) {
+// forward + residual + adjoint for one batch; leaves raw gradients in gr, each
+// shot's scaled by alpha[shot] (and the lam / mu sums by kGainLM)
+int batch_gradient(fwi_ctx *c, int s0, int nb, const double *adj_z, const double *adj_x, double *misfit_out) {
     RETIF(run_forward(c, s0, nb, true, adj_z == nullptr, true));
     const int r0 = c->rec_off[s0], r1 = c->rec_off[s0 + nb];
     const size_t rows = (size_t)(r1 - r0) * c->g.nt, off = (size_t)r0 * c->g.nt;
@@ -521,12 +534,14 @@ int batch_gradient(fwi_ctx *c, int s0, int nb, const double *adj_z, const double
         if (misfit_out)
             HIPCHK(hipMemcpyAsync(misfit_out, c->misfit, sizeof(double) * nb, hipMemcpyDeviceToHost, c->st));
     }
-    double mz = 0, mx = 0;
-    RETIF(absmax_host(c, c->res_z + off, rows, &mz));
-    RETIF(absmax_host(c, c->res_x + off, rows, &mx));
-    const double alpha = pow2_scale(std::max(mz, mx));
-    *alpha_out = alpha;
-    return run_adjoint(c, s0, nb, alpha);
+    (void)rows;
+    std::vector<double> mx(nb);
+    dev_shot_absmax(c->res_z, c->res_x, c->rec_off_d + s0, nb, c->g.nt, c->shotmax, c->st);
+    HIPCHK(hipMemcpyAsync(mx.data(), c->shotmax, sizeof(double) * nb, hipMemcpyDeviceToHost, c->st));
+    RETIF(sync(c));
+    for (int s = 0; s < nb; ++s) c->alpha[s0 + s] = pow2_scale(mx[s]);
+    RETIF(upload(c, c->alpha_d + s0, c->alpha.data() + s0, nb));
+    return run_adjoint(c, s0, nb);
 }
 
 int refresh_media(fwi_ctx *c) {
@@ -625,6 +640,9 @@ int allocate(fwi_ctx *c) {
     c->res_z = D(recrows); c->res_x = D(recrows);
     c->acc = D(2 * (size_t)c->nrec_tot);
     c->misfit = D(c->nshot);
+    c->alpha_d = D(c->nshot);
+    c->shotmax = D(c->nshot);
+    c->alpha.assign(c->nshot, 1.0);
     for (int i = 0; i < 9; ++i) c->f[i] = F(B * fs);
     for (int i = 0; i < 4; ++i) c->psi[i] = F(B * c->psiz());
     for (int i = 4; i < 8; ++i) c->psi[i] = F(B * c->psix());
@@ -684,13 +702,6 @@ void build_groups(const std::vector<int> &off, const int32_t *z, const int32_t *
     }
     first[nshot] = (int)gshot.size();
     goff.push_back((int)gidx.size());
-}
-
-template <class T>
-int upload(fwi_ctx *c, T *dst, const T *src, size_t n) {
-    if (n == 0) return FWI_OK;
-    HIPCHK(hipMemcpyAsync(dst, src, n * sizeof(T), hipMemcpyHostToDevice, c->st));
-    return FWI_OK;
 }
 
 int setup_device(fwi_ctx *c, const fwi_acq *q) {
@@ -1006,7 +1017,9 @@ int32_t fwi_get_wavefield(fwi_ctx *c, int32_t shot, int32_t field, double *out) 
         return c->fail(FWI_ESTATE, "shot %d is not in the last propagated batch", shot);
     const int nz = c->g.nz, nx = c->g.nx, P = c->P;
     std::vector<float> h((size_t)nz * P);
-    const float *src = c->f[field] + (size_t)(shot - c->last_s0) * c->fs();
+    // ABI order vz vx uz ux szz szx sxx We We_adj -> device arrays vz vx szz szx sxx uz ux We Wa
+    static const int dev_index[9] = {0, 1, 5, 6, 2, 3, 4, 7, 8};
+    const float *src = c->f[dev_index[field]] + (size_t)(shot - c->last_s0) * c->fs();
     HIPCHK(hipMemcpyAsync(h.data(), src, h.size() * 4, hipMemcpyDeviceToHost, c->st));
     RETIF(sync(c));
     for (int iz = 0; iz < nz; ++iz)
@@ -1043,12 +1056,13 @@ int32_t fwi_gradient(fwi_ctx *c, int32_t shot0, int32_t nshots, double *misfit) 
     if (batches(c, shot0, nshots, bs) != FWI_OK) return c->fail(FWI_EINVAL, "shot range");
     const int nz = c->g.nz, nx = c->g.nx;
     for (auto [s0, nb] : bs) {
-        double alpha = 1;
-        RETIF(batch_gradient(c, s0, nb, nullptr, nullptr, misfit ? misfit + (s0 - shot0) : nullptr, &alpha));
+        RETIF(batch_gradient(c, s0, nb, nullptr, nullptr, misfit ? misfit + (s0 - shot0) : nullptr));
         const size_t snapcell = (size_t)c->snz * c->snx;
         for (int s = 0; s < nb; ++s) {
+            const double alpha = c->alpha[s0 + s];
             const float *gs[3] = {c->gr[0] + s * snapcell, c->gr[1] + s * snapcell, c->gr[2] + s * snapcell};
-            dev_precondition(c->f[7] + s * c->fs(), c->f[8] + s * c->fs(), 1.0 / (alpha * alpha), gs, 1.0 / alpha,
+            const double gsc[3] = {1.0 / (alpha * kGainLM), 1.0 / (alpha * kGainLM), 1.0 / alpha};
+            dev_precondition(c->f[7] + s * c->fs(), c->f[8] + s * c->fs(), 1.0 / (alpha * alpha), gs, gsc,
                              c->G, c->C, c->tmp, c->red, nz, nx, c->P, c->g.snap_z1, c->g.snap_z2, c->g.snap_x1,
                              c->g.snap_x2, c->g.snap_dz, c->g.snap_dx, c->snz, c->snx, c->st);
         }
@@ -1071,14 +1085,15 @@ int32_t fwi_gradient_raw(fwi_ctx *c, int32_t shot0, int32_t nshots, const double
     const int nt = c->g.nt, rb = c->rec_off[shot0];
     for (auto [s0, nb] : bs) {
         const size_t off = (size_t)(c->rec_off[s0] - rb) * nt;
-        double alpha = 1;
-        RETIF(batch_gradient(c, s0, nb, adj_z + off, adj_x + off, nullptr, &alpha));
+        RETIF(batch_gradient(c, s0, nb, adj_z + off, adj_x + off, nullptr));
         for (int k = 0; k < 3; ++k) {
             if (!outs[k]) continue;
             HIPCHK(hipMemcpyAsync(h.data(), c->gr[k], snapcell * nb * 4, hipMemcpyDeviceToHost, c->st));
             RETIF(sync(c));
-            for (int s = 0; s < nb; ++s)
-                for (size_t i = 0; i < snapcell; ++i) outs[k][i] += (double)h[s * snapcell + i] / alpha;
+            for (int s = 0; s < nb; ++s) {
+                const double den = c->alpha[s0 + s] * (k < 2 ? kGainLM : 1.0);
+                for (size_t i = 0; i < snapcell; ++i) outs[k][i] += (double)h[s * snapcell + i] / den;
+            }
         }
     }
     return FWI_OK;

@@ -48,6 +48,7 @@ struct KArgs {
     int displ;                 // u += dt v (adjoint)
     int grad;                  // correlate with the snapshot slab this step
     float gw;                  // grad_sign * snap_dt * dt
+    float gain_lm;             // power of two on the lam / mu terms, keeps FP32 sums off denormals
     const float *sn_vz, *sn_vx, *sn_szz, *sn_szx, *sn_sxx;  // slab of shot 0 at this step
     size_t sn_stride;          // between shots
     float *g_lam, *g_mu, *g_rho;  // [B][snz][snx]
@@ -64,7 +65,8 @@ struct SrcArgs {
     const int *grp_idx;        // source rows into stf (absolute)
     const double *stf_z, *stf_x;  // [rows][nt]
     int nt, type;
-    double dt, dzdx, scale;    // scale multiplies the source amplitude (adjoint rescaling)
+    double dt, dzdx;
+    const double *scale_shot;  // per-shot amplitude factor (adjoint rescaling), null = 1
 };
 
 struct RecArgs {
@@ -106,8 +108,10 @@ void dev_residual(const double *seis_z, const double *seis_x, const double *obs_
                   double *res_z, double *res_x, const int *rec_off_local, int nshots, int nt, double dt,
                   double *misfit, hipStream_t st);
 void dev_absmax(const double *x, size_t n, double *out, hipStream_t st);
+void dev_shot_absmax(const double *a, const double *b, const int *rec_off_local, int nshots, int nt, double *out,
+                     hipStream_t st);
 void dev_precondition(const float *We, const float *Wa, double wa_scale, const float *const g[3],
-                      double g_scale, double *const G[3], const double C[3], double *tmp,
+                      const double g_scale[3], double *const G[3], const double C[3], double *tmp,
                       double *wmax, int nz, int nx, int P, int z1, int z2, int x1, int x2, int sdz,
                       int sdx, int snz, int snx, hipStream_t st);
 void dev_taper(double *A, int nz, int nx, int z1, int z2, int x1, int x2, const int taper[8],

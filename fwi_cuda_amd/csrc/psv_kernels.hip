@@ -196,9 +196,12 @@ __global__ __launch_bounds__(kBX *kBY) void k_velocity(const KArgs a) {
             const float lam = a.lam[r], mu = a.mu[r];
             const float c_zz = qzz[N - 1], c_zx = qzx[N], c_xx = sxx[r];
             const float lm = lam + mu;
-            const float s1 = 0.25f * (Szz + Sxx) * (c_zz + c_xx) / (lm * lm);
-            const float s2 = 0.25f * (Szz - Sxx) * (c_zz - c_xx) / (mu * mu);
-            const float s3 = (Szx * c_zx) / (mu * mu);
+            // the 1/(lam+mu)^2 and 1/mu^2 factors (~1e-20) carry the gain so the
+            // products stay in the FP32 normal range; the host divides it out in FP64
+            const float il = a.gain_lm / (lm * lm), im = a.gain_lm / (mu * mu);
+            const float s1 = 0.25f * il * (Szz + Sxx) * (c_zz + c_xx);
+            const float s2 = 0.25f * im * (Szz - Sxx) * (c_zz - c_xx);
+            const float s3 = im * (Szx * c_zx);
             const float s4 = ux * Vx + uz * Vz;
             a.g_lam[q] += a.gw * s1;
             a.g_mu[q] += a.gw * (s3 + s1 + s2);
@@ -279,9 +282,10 @@ __global__ void k_inject(const KArgs a, const SrcArgs sa, int it) {
     const int cell = sa.grp_cell[g];
     const size_t o = (size_t)s * a.fs + cell;
     float vz = a.vz[o], vx = a.vx[o];
+    const double sc = sa.scale_shot ? sa.scale_shot[sa.grp_shot[g]] : 1.0;
     for (int j = sa.grp_off[g]; j < sa.grp_off[g + 1]; ++j) {
         const size_t row = (size_t)sa.grp_idx[j] * sa.nt + it;
-        const double fz = sa.stf_z[row] * sa.scale, fx = sa.stf_x[row] * sa.scale;
+        const double fz = sa.stf_z[row] * sc, fx = sa.stf_x[row] * sc;
         if (sa.type == 0) {
             vz += (float)(sa.dt * (double)a.rhoz[cell] * fz / sa.dzdx);
             vx += (float)(sa.dt * (double)a.rhox[cell] * fx / sa.dzdx);

@@ -105,6 +105,17 @@ __global__ void k_residual(const double *seis_z, const double *seis_x, const dou
     if (threadIdx.x == 0) misfit[s] = acc;
 }
 
+// per-shot max(|a|, |b|) over the shot's receiver rows (block s = shot s)
+__global__ void k_shot_absmax(const double *a, const double *b, const int *rec_off, int nt, double *out) {
+    __shared__ double sh[kRB];
+    const int s = blockIdx.x;
+    const size_t lo = (size_t)rec_off[s] * nt, hi = (size_t)rec_off[s + 1] * nt;
+    double m = 0;
+    for (size_t i = lo + threadIdx.x; i < hi; i += blockDim.x) m = fmax(m, fmax(fabs(a[i]), fabs(b[i])));
+    m = block_reduce(m, OpMax(), sh);
+    if (threadIdx.x == 0) out[s] = m;
+}
+
 __global__ void k_absmax(const double *x, size_t n, double *part) {
     __shared__ double sh[kRB];
     double m = 0;
@@ -264,8 +275,13 @@ void dev_absmax(const double *x, size_t n, double *out, hipStream_t st) {
     hipLaunchKernelGGL(k_max_parts, dim3(1), dim3(kRB), 0, st, part, nb, out);
 }
 
-void dev_precondition(const float *We, const float *Wa, double wa_scale, const float *const g[3], double g_scale,
-                      double *const G[3], const double C[3], double *tmp, double *wmax, int nz, int nx, int P,
+void dev_shot_absmax(const double *a, const double *b, const int *rec_off_local, int nshots, int nt, double *out,
+                     hipStream_t st) {
+    hipLaunchKernelGGL(k_shot_absmax, dim3(nshots), dim3(kRB), 0, st, a, b, rec_off_local, nt, out);
+}
+
+void dev_precondition(const float *We, const float *Wa, double wa_scale, const float *const g[3],
+                      const double g_scale[3], double *const G[3], const double C[3], double *tmp, double *wmax, int nz, int nx, int P,
                       int z1, int z2, int x1, int x2, int sdz, int sdx, int snz, int snx, hipStream_t st) {
     double *Wc = tmp, *T = tmp + (size_t)nz * nx;
     const size_t nbox = (size_t)(z2 - z1) * (x2 - x1);
@@ -277,7 +293,7 @@ void dev_precondition(const float *We, const float *Wa, double wa_scale, const f
     for (int p = 0; p < 3; ++p) {
         if (!(C[p] > 0)) continue;
         (void)hipMemsetAsync(T, 0, sizeof(double) * (size_t)nz * nx, st);
-        hipLaunchKernelGGL(k_interp_points, dim3((snz * snx + 255) / 256), dim3(256), 0, st, g[p], g_scale, T, nx,
+        hipLaunchKernelGGL(k_interp_points, dim3((snz * snx + 255) / 256), dim3(256), 0, st, g[p], g_scale[p], T, nx,
                            z1, x1, sdz, sdx, snz, snx);
         if (sdx > 1) {
             const int n = ((z2 - z1 + sdz - 1) / sdz) * ((x2 - x1 + sdx - 1) / sdx);

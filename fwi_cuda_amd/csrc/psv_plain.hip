@@ -158,9 +158,10 @@ __global__ void k_pgrad(const KArgs a) {
     const float Szz = a.sn_szz[q], Sxx = a.sn_sxx[q], Szx = a.sn_szx[q], Vz = a.sn_vz[q], Vx = a.sn_vx[q];
     const float lam = a.lam[r], mu = a.mu[r], lm = lam + mu;
     const float szz = a.szz[r], sxx = a.sxx[r], szx = a.szx[r];
-    const float s1 = 0.25f * (Szz + Sxx) * (szz + sxx) / (lm * lm);
-    const float s2 = 0.25f * (Szz - Sxx) * (szz - sxx) / (mu * mu);
-    const float s3 = (Szx * szx) / (mu * mu);
+    const float il = a.gain_lm / (lm * lm), im = a.gain_lm / (mu * mu);
+    const float s1 = 0.25f * il * (Szz + Sxx) * (szz + sxx);
+    const float s2 = 0.25f * im * (Szz - Sxx) * (szz - sxx);
+    const float s3 = im * (Szx * szx);
     const float s4 = a.ux[r] * Vx + a.uz[r] * Vz;
     a.g_lam[q] += a.gw * s1;
     a.g_mu[q] += a.gw * (s3 + s1 + s2);

@@ -13,9 +13,10 @@ def layered_model(nz, nx, seed, nlayer=4, cp_range=(2200.0, 3600.0), inclusion=T
     for b, v in zip(bounds, vals[1:]):
         cp[np.broadcast_to(z >= b, (nz, nx))] = v
     if inclusion:
-        zc, xc = rng.integers(nz // 3, 2 * nz // 3), rng.integers(nx // 3, 2 * nx // 3)
+        # shallow enough that its reflection reaches surface receivers within the test records
+        zc, xc = rng.integers(nz // 4, nz // 2), rng.integers(nx // 3, 2 * nx // 3)
         zz, xx = np.mgrid[0:nz, 0:nx]
-        cp[(zz - zc) ** 2 + (xx - xc) ** 2 < (min(nz, nx) // 8) ** 2] *= 1.1
+        cp[(zz - zc) ** 2 + (xx - xc) ** 2 < (min(nz, nx) // 7) ** 2] *= 1.15
     cs = cp / vs_ratio
     rho = 310.0 * cp ** 0.25
     mu = rho * cs * cs

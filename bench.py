@@ -1,0 +1,298 @@
+#!/usr/bin/env python3
+"""Benchmark: forward + adjoint gradient throughput of the elastic P/SV path.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--config C3]
+
+One step = one full gradient (forward, residual, checkpointed adjoint and the
+energy-weighted preconditioning) over every shot of the configuration, the
+C3 workload by default: 1001 x 2001 grid, 64 shots with 20-200 receivers each,
+nt = 4000, order-8 Holberg stencil, shots batched per launch, checkpointed
+adjoint.  Synthetic data: a Gaussian-random-field true model generates the
+observed seismograms (untimed); the gradient is taken at its smoothed version.
+
+Printed JSON (one line): Gcell-updates/s = 2 nz nx nt nshot / wall (forward
+and adjoint cells, recomputation not counted), with
+  roofline      the dominant kernel's algorithmic bytes per launch over its
+                average duration (HIP events on the library's stream), against
+                MI355X's 8 TB/s;
+  hbm_frac      the whole step's algorithmic bytes (fwd 68 B, adjoint 84 B,
+                gradient 44 B, recomputed step 60 B per cell-step, CPML strips
+                32 B per cell-step per direction) over wall time and 8 TB/s;
+  plain         the reference-structured per-shot form on the same grid
+                (shots/s) and the speed-up in shots/s;
+  cpu_baseline  the reference's CPU path (compiled from its sources) on a
+                bounded sample of the same grid.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK = 8.0e12
+# per-cell algorithmic bytes of each fused kernel (FP32 fields, read once, write once)
+B_STRESS = 44          # R vz vx, RW szz szx sxx, R lam mu mu_zx
+B_VEL_FWD = 44         # R szz szx sxx, RW vz vx, R rho_zp rho_xp, RW We
+B_VEL_RECOMP = 36      # as forward without We
+B_VEL_ADJ = 60 + 52    # + RW uz ux; gradient: R 5 snapshot fields, RW 3 sums, R lam mu
+B_STORE = 40           # R 5 fields, W 5 snapshot fields
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def build_workload(name, nt, nshot):
+    from fwi_cuda_amd import configs
+    from scipy.ndimage import gaussian_filter
+    kw = {}
+    if nt:
+        kw["nt"] = nt
+    if nshot:
+        kw["nshot"] = nshot
+    c = configs.BUILDERS[name](**kw)
+    # start model: the true model smoothed with a 20-cell Gaussian
+    lam0, mu0, rho0 = (gaussian_filter(c[k], 20.0, mode="nearest") for k in ("lam", "mu", "rho"))
+    return c, (lam0, mu0, rho0)
+
+
+class quiet_stdout:
+    """Send file descriptor 1 to /dev/null (the reference prints progress from C++)."""
+
+    def __enter__(self):
+        sys.stdout.flush()
+        self.saved = os.dup(1)
+        self.null = os.open(os.devnull, os.O_WRONLY)
+        os.dup2(self.null, 1)
+
+    def __exit__(self, *exc):
+        sys.stdout.flush()
+        os.dup2(self.saved, 1)
+        os.close(self.saved)
+        os.close(self.null)
+
+
+def cpu_baseline(c, budget_s=20.0):
+    """The reference's kernel_PSV (compiled from its sources into oracle/_ref)
+    on one shot of the same grid, forward then adjoint, over a bounded number
+    of time steps chosen to take about budget_s seconds."""
+    from oracle import oracle as O
+    kind = "reference" if O.ref_available() else "port"
+    which = "ref" if kind == "reference" else "oracle"
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    nz, nx = int(c["nz"]), int(c["nx"])
+    cc = dict(c)
+    cc.pop("rec_off", None)
+    cc["z_rec"], cc["x_rec"] = c["z_rec"][:20].copy(), c["x_rec"][:20].copy()
+    cc["z_src"], cc["x_src"] = c["z_src"][:1].copy(), c["x_src"][:1].copy()
+    cc["shot_to_fire"] = np.zeros(1, np.int32)
+    cc["nshot"] = 1
+    if which == "ref":
+        cc["fdorder"] = 2   # the reference's kernels compute order 2 whatever fdorder says
+        cc["hc"] = O.holberg(2)
+    med = O.mat_av2(cc, which="oracle")
+
+    def run(nts):
+        d = dict(cc)
+        d["nt"] = nts
+        d["stf_z"] = np.ascontiguousarray(cc["stf_z"][:1, :nts])
+        d["stf_x"] = np.ascontiguousarray(cc["stf_x"][:1, :nts])
+        t0 = time.perf_counter()
+        f = O.kernel_psv(d, med, 0, grad=False, accu=True, which=which)
+        adj = (np.ones_like(f["uz"]) * 1e-9, np.ones_like(f["ux"]) * 1e-9)
+        O.kernel_psv(d, med, 0, grad=True, accu=False, snaps=f["snaps"], adj=adj, which=which)
+        return time.perf_counter() - t0
+
+    with quiet_stdout():
+        t_probe = run(4)
+        # bounded by time and by the snapshot memory of the reference (5 FP64 fields per step)
+        nts = int(max(8, min(120, 4 * budget_s / max(t_probe, 1e-3))))
+        t = run(nts)
+    rate = 2.0 * nz * nx * nts / t / 1e9
+    return {"value": rate, "unit": "Gcell-updates/s", "cores": threads, "kind": kind,
+            "sample": f"one shot of the {nz}x{nx} grid, {nts} forward + {nts} adjoint steps "
+                      f"({'order 2, the reference kernels' if which == 'ref' else 'order 8 FP64 oracle'}), "
+                      f"{t:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--config", default="C3")
+    ap.add_argument("--nt", type=int, default=0, help="shorter time axis (profiling runs)")
+    ap.add_argument("--nshot", type=int, default=0)
+    ap.add_argument("--batch", type=int, default=0)
+    ap.add_argument("--tile-tz", type=int, default=0)
+    ap.add_argument("--variant", type=int, default=0, help="stencil kernel variant (0 = tuned)")
+    ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only run a device copy of MIB MiB five times (calibrates rocprof byte counters)")
+    args = ap.parse_args()
+    if args.copy_probe:
+        import torch
+        n = args.copy_probe * (1 << 20) // 4
+        src = torch.ones(n, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)                  # the runtime's blit kernel
+            torch.mul(src, 2.0, out=dst)    # an ordinary elementwise kernel: one load, one store per element
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_MiB": args.copy_probe, "bytes_read_per_copy": 4 * n,
+                          "bytes_written_per_copy": 4 * n}))
+        return
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print(f"bench.py measures one GPU: --gpus {args.gpus} (WORLD_SIZE {world}) is not run. Shots are "
+              "independent; the multi-GPU design shards shots and all-reduces the gradient once.", file=sys.stderr)
+        sys.exit(2)
+
+    from fwi_cuda_amd import _lib, psv
+
+    c, start = build_workload(args.config, args.nt, args.nshot)
+    nz, nx, nt, nshot = int(c["nz"]), int(c["nx"]), int(c["nt"]), int(c["nshot"])
+    N = int(c["fdorder"]) // 2
+    log(f"workload {args.config}: {nz}x{nx}, nt {nt}, {nshot} shots, {int(c['rec_off'][-1]) if 'rec_off' in c else ''} "
+        "receivers")
+    sim = psv.Simulation(c, batch=args.batch, tile_tz=args.tile_tz, kernel_variant=args.variant)
+    plan = sim.plan()
+    log("plan", plan)
+    t0 = time.perf_counter()
+    obs = sim.forward()
+    log(f"observed data generated in {time.perf_counter() - t0:.1f} s")
+    sim.set_observed(*obs)
+    sim.set_model(*start)
+
+    def step():
+        sim.zero_gradient()
+        return sim.gradient()
+
+    for w in range(args.warmup):
+        t0 = time.perf_counter()
+        step()
+        log(f"warmup {w}: {time.perf_counter() - t0:.2f} s")
+    sim.profile(True)
+    try:
+        import torch
+        sync_torch = torch.cuda.synchronize if torch.cuda.is_available() else (lambda: None)
+    except Exception:
+        sync_torch = lambda: None  # noqa: E731
+    sim.sync()
+    sync_torch()
+    t0 = time.perf_counter()
+    misfit = None
+    for k in range(args.steps):
+        misfit = step()
+    sim.sync()
+    sync_torch()
+    wall = time.perf_counter() - t0
+    kt = sim.kernel_times()
+    sim.profile(False)
+    tz, tune_log = sim.tuning()
+    log(f"timed {args.steps} step(s): {wall:.2f} s, misfit {float(np.sum(misfit)):.4e}; tile rows {tz} ({tune_log})")
+    log("kernel times", {k: (n, round(ms, 1)) for k, (n, ms) in kt.items()})
+
+    K = args.steps
+    cells = nz * nx
+    interior = (nz - 2 * N) * (nx - 2 * N)
+    value = 2.0 * cells * nt * nshot * K / wall / 1e9
+    shots_per_s = nshot * K / wall
+
+    # executed algorithmic bytes of the whole step (the survey's byte model)
+    ckpt = plan["storage"] == "checkpoint"
+    zs = int(np.count_nonzero((c["a_z"] != 0) | (c["a_half_z"] != 0))) if c["pml_z"] else 0
+    xs = int(np.count_nonzero((c["a_x"] != 0) | (c["a_half_x"] != 0))) if c["pml_x"] else 0
+    pml_cells = zs * nx + xs * nz
+    per_shot_step = cells * (68 + 84 + 44 + (60 if ckpt else 20)) + pml_cells * 32 * (3 if ckpt else 2)
+    hbm_frac = per_shot_step * nt * nshot * K / wall / HBM_PEAK
+
+    # roofline of the dominant kernel from its live per-launch times
+    batches = plan["batch"]
+    launches_v, ms_v = kt["velocity"]
+    launches_s, ms_s = kt["stress"]
+    # every launch advances one batch; bytes summed over the launch modes the step runs
+    shot_steps = nshot * nt * K
+    bytes_stress = B_STRESS * interior * shot_steps * (3 if ckpt else 2)
+    bytes_vel = interior * shot_steps * (B_VEL_FWD + B_VEL_ADJ + (B_VEL_RECOMP if ckpt else 0))
+    kern = {
+        "stress": (bytes_stress, ms_s, launches_s),
+        "velocity": (bytes_vel, ms_v, launches_v),
+    }
+    dom = max(kern, key=lambda k: kern[k][1])
+    b, ms, n = kern[dom]
+    achieved = b / (ms / 1e3) / 1e9
+    roofline = {"kernel": dom, "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK / 1e9,
+                "unit": "GB/s", "frac": round(achieved * 1e9 / HBM_PEAK, 4), "traffic": None,
+                "avg_launch_ms": round(ms / max(n, 1), 4), "algorithmic_bytes_per_launch": int(b / max(n, 1))}
+    per_kernel = {k: {"achieved_GBps": round(v[0] / (v[1] / 1e3) / 1e9, 1) if v[1] else None,
+                      "launches": v[2], "total_ms": round(v[1], 1)} for k, v in kern.items()}
+    sim.close()
+
+    plain = None
+    if not args.no_plain:
+        try:
+            with psv.Simulation(c, form=_lib.FORM_PLAIN, storage=_lib.STORE_FULL) as ps:
+                ps.set_observed(*obs)
+                ps.set_model(*start)
+                ps.zero_gradient()
+                ps.gradient(0, 1)   # warm-up shot
+                ps.zero_gradient()
+                t0 = time.perf_counter()
+                ps.gradient(1, 1)
+                tp = time.perf_counter() - t0
+            plain = {"shots_per_s": round(1.0 / tp, 4), "s_per_shot": round(tp, 3),
+                     "Gcell_per_s": round(2.0 * cells * nt / tp / 1e9, 3),
+                     "speedup_shots_per_s": round(shots_per_s * tp, 2)}
+            log("plain per-shot form:", plain)
+        except Exception as e:   # noqa: BLE001
+            plain = {"error": str(e)}
+            log("plain form failed:", e)
+
+    cpu = None
+    if not args.no_cpu:
+        try:
+            cpu = cpu_baseline(c)
+            log("cpu baseline:", cpu)
+        except Exception as e:   # noqa: BLE001
+            cpu = {"error": str(e)}
+
+    out = {
+        "metric": "fwd+adj gradient Gcell-updates/s (batched, checkpointed)",
+        "value": round(value, 3),
+        "unit": "Gcell-updates/s",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": args.warmup,
+        "ms_per_step": round(wall / K * 1e3, 2),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded Gaussian-random-field model, observed data modelled on the GPU)",
+        "config": {"workload": f"{args.config}: {nz}x{nx} grid, dx {c['dx']} m, nt {nt}, {nshot} shots, "
+                               f"uneven receivers, order {int(c['fdorder'])}",
+                   "batch": plan["batch"], "storage": plan["storage"], "ckpt_interval": plan["ckpt_interval"],
+                   "tile_rows": tz,
+                   "parallelism": "single GPU, shots batched per launch"},
+        "shots_per_s": round(shots_per_s, 3),
+        "hbm_frac": round(hbm_frac, 4),
+        "roofline": roofline,
+        "kernels": per_kernel,
+        "plain": plain,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -42,7 +42,7 @@ class FwiOpts(C.Structure):
     _fields_ = [("batch", C.c_int32), ("storage", C.c_int32), ("ckpt_interval", C.c_int32),
                 ("mem_budget", C.c_int64), ("form", C.c_int32), ("grad_sign", C.c_int32),
                 ("use_graph", C.c_int32), ("tile_tz", C.c_int32), ("device", C.c_int32),
-                ("check_finite", C.c_int32)]
+                ("check_finite", C.c_int32), ("kernel_variant", C.c_int32)]
 
 
 class FwiKernelTime(C.Structure):
@@ -85,6 +85,7 @@ def load() -> C.CDLL:
         "fwi_set_observed": (C.c_int32, [vp, _dp, _dp]),
         "fwi_forward": (C.c_int32, [vp, C.c_int32, C.c_int32, _dp, _dp]),
         "fwi_get_wavefield": (C.c_int32, [vp, C.c_int32, C.c_int32, _dp]),
+        "fwi_forward_snapshots": (C.c_int32, [vp, C.c_int32] + [_dp] * 7),
         "fwi_gradient": (C.c_int32, [vp, C.c_int32, C.c_int32, _dp]),
         "fwi_gradient_raw": (C.c_int32, [vp, C.c_int32, C.c_int32, _dp, _dp, _dp, _dp, _dp]),
         "fwi_zero_gradient": (C.c_int32, [vp]),
@@ -96,6 +97,7 @@ def load() -> C.CDLL:
         "fwi_update_model": (C.c_int32, [vp, C.c_double, C.c_double, C.c_double]),
         "fwi_step_length": (C.c_int32, [vp, C.c_double, C.c_double, _dp, _ip]),
         "fwi_iteration": (C.c_int32, [vp, _dp, _dp]),
+        "fwi_tuning": (C.c_int32, [vp, _ip, C.c_char_p, C.c_int32]),
         "fwi_profile": (C.c_int32, [vp, C.c_int32]),
         "fwi_kernel_times": (C.c_int32, [vp, C.POINTER(FwiKernelTime)]),
         "fwi_sync": (C.c_int32, [vp]),
@@ -136,4 +138,4 @@ EXPORTED = ("fwi_abi_version", "fwi_strerror", "fwi_last_error", "fwi_default_op
             "fwi_get_model", "fwi_get_media", "fwi_set_observed", "fwi_forward", "fwi_get_wavefield",
             "fwi_gradient", "fwi_gradient_raw", "fwi_zero_gradient", "fwi_get_gradient", "fwi_set_gradient",
             "fwi_taper_gradient", "fwi_misfit", "fwi_save_model", "fwi_update_model", "fwi_step_length",
-            "fwi_iteration", "fwi_profile", "fwi_kernel_times", "fwi_sync")
+            "fwi_iteration", "fwi_tuning", "fwi_profile", "fwi_kernel_times", "fwi_sync", "fwi_forward_snapshots")

@@ -19,7 +19,10 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <string>
+#include <tuple>
 #include <vector>
 
 #include "psv_internal.h"
@@ -91,6 +94,10 @@ struct fwi_ctx {
     // plain form: full-grid derivative scratch and CPML memory (one shot)
     bool plain = false;
     PlainBufs pb{};
+    bool tuned = false;
+    int var = 2;   // stencil kernel variant (autotuned)
+    int order = 0; // stencil work order (autotuned)
+    std::string tune_log;
 
     // ---- profiling
     int prof_on = 0;
@@ -249,6 +256,7 @@ KArgs base_args(fwi_ctx *c, int nb, float *const fld[5], float *const ps[8]) {
     a.g_rho = c->gr[2];
     a.gw = (float)(c->o.grad_sign * (double)c->g.snap_dt * c->g.dt);
     a.gain_lm = (float)kGainLM;
+    a.order = c->order;
     return a;
 }
 
@@ -323,7 +331,7 @@ void half_steps(fwi_ctx *c, const KArgs &a) {
     }
     {
         Timer t(c, FWI_K_STRESS);
-        launch_stress(a, c->N, c->tz, c->st);
+        launch_stress(a, c->N, c->tz, c->var, c->st);
     }
     if (has_surface(c)) {
         Timer t(c, FWI_K_SURFACE);
@@ -331,7 +339,7 @@ void half_steps(fwi_ctx *c, const KArgs &a) {
     }
     {
         Timer t(c, FWI_K_VELOCITY);
-        launch_velocity(a, c->N, c->tz, c->st);
+        launch_velocity(a, c->N, c->tz, c->var, c->st);
     }
 }
 
@@ -386,9 +394,92 @@ int check_finite(fwi_ctx *c, int nb) {
     return FWI_OK;
 }
 
+// Autotuner: kernel variant, rows per thread (TZ) and work order of the two
+// stencil kernels, timed on the batch (zeroed state, a few steps per candidate);
+// the best is kept per (grid, order, batch) for the process.
+struct TuneKey {
+    int nz, nx, N, B;
+    bool operator<(const TuneKey &o) const {
+        return std::tie(nz, nx, N, B) < std::tie(o.nz, o.nx, o.N, o.B);
+    }
+};
+struct TuneVal {
+    int var, tz, order;
+};
+std::map<TuneKey, TuneVal> g_tuned;
+std::mutex g_tune_mu;
+
+int autotune(fwi_ctx *c, int nb) {
+    if (c->plain || c->tuned) return FWI_OK;
+    const TuneKey key{c->g.nz, c->g.nx, c->N, nb};
+    const bool forced = c->o.kernel_variant >= 1 && c->o.kernel_variant <= 3;
+    {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        auto it = g_tuned.find(key);
+        if (it != g_tuned.end() && (!forced || c->o.kernel_variant == it->second.var) &&
+            (c->o.tile_tz <= 0 || c->o.tile_tz == it->second.tz)) {
+            c->var = it->second.var;
+            c->tz = it->second.tz;
+            c->order = it->second.order;
+            c->tuned = true;
+            return FWI_OK;
+        }
+    }
+    float *fld[5] = {c->f[0], c->f[1], c->f[2], c->f[3], c->f[4]};
+    RETIF(zero_state(c, nb, fld, c->psi, true));
+    KArgs a = base_args(c, nb, fld, c->psi);
+    a.energy = 1;
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    std::vector<int> tzs = {8, 16, 32, 64};
+    if (c->o.tile_tz > 0) tzs = {c->o.tile_tz};
+    std::vector<int> vars = {1, 2, 3};
+    if (forced) vars = {c->o.kernel_variant};
+    float best = 1e30f;
+    TuneVal bv{vars[0], tzs[0], 0};
+    for (int order = 0; order < 3; ++order) {
+        a.order = order;
+        for (int var : vars) {
+            for (int tz : tzs) {
+                for (int w = 0; w < 2; ++w) {
+                    launch_stress(a, c->N, tz, var, c->st);
+                    launch_velocity(a, c->N, tz, var, c->st);
+                }
+                HIPCHK(hipEventRecord(e0, c->st));
+                for (int w = 0; w < 5; ++w) {
+                    launch_stress(a, c->N, tz, var, c->st);
+                    launch_velocity(a, c->N, tz, var, c->st);
+                }
+                HIPCHK(hipEventRecord(e1, c->st));
+                HIPCHK(hipEventSynchronize(e1));
+                float ms = 0;
+                HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+                char buf[64];
+                std::snprintf(buf, sizeof buf, "o%d/v%d/tz%d %.3f; ", order, var, tz, ms / 5);
+                c->tune_log += buf;
+                if (ms < best) {
+                    best = ms;
+                    bv = {var, tz, order};
+                }
+            }
+        }
+    }
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    c->var = bv.var;
+    c->tz = bv.tz;
+    c->order = bv.order;
+    c->tuned = true;
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_tuned[key] = bv;
+    return FWI_OK;
+}
+
 // Forward pass (kernel_PSV with grad = false) for shots [s0, s0 + nb).
 int run_forward(fwi_ctx *c, int s0, int nb, bool store, bool record, bool energy) {
     float *fld[5] = {c->f[0], c->f[1], c->f[2], c->f[3], c->f[4]};
+    RETIF(autotune(c, nb));
     RETIF(zero_state(c, nb, fld, c->psi, true));
     if (record) HIPCHK(hipMemsetAsync(c->acc, 0, sizeof(double) * 2 * c->nrec_tot, c->st));
     KArgs a = base_args(c, nb, fld, c->psi);
@@ -599,7 +690,19 @@ int plan(fwi_ctx *c) {
         if (storage == FWI_STORE_CHECKPOINT) return c->fail(FWI_EINVAL, "the plain form stores full snapshots");
         storage = FWI_STORE_FULL;
     }
-    if (storage == FWI_STORE_AUTO) storage = fits(FWI_STORE_FULL, 1) ? FWI_STORE_FULL : FWI_STORE_CHECKPOINT;
+    if (storage == FWI_STORE_AUTO) {
+        // Full storage moves 40 B per cell-step (store + load) against the ~100 B of a
+        // recomputed step, but its footprint caps the batch; batching is worth more
+        // than the recomputation, so full storage is kept only if it still allows
+        // the batch checkpointing would (or every shot).
+        auto max_batch = [&](int st_) {
+            Need n = need_bytes(c, st_, S);
+            return n.shared + n.per_shot > budget ? 0 : (int)std::min<size_t>((budget - n.shared) / n.per_shot, c->nshot);
+        };
+        const int bf = max_batch(FWI_STORE_FULL), bc = max_batch(FWI_STORE_CHECKPOINT);
+        const int want = c->o.batch > 0 ? std::min(c->o.batch, c->nshot) : c->nshot;
+        storage = (bf >= std::min(want, bc) && bf > 0) ? FWI_STORE_FULL : FWI_STORE_CHECKPOINT;
+    }
     Need n = need_bytes(c, storage, S);
     if (n.shared + n.per_shot > budget)
         return c->fail(FWI_ENOMEM, "one shot needs %.2f GB, budget %.2f GB", (n.shared + n.per_shot) / 1e9,
@@ -1252,6 +1355,16 @@ int32_t fwi_iteration(fwi_ctx *c, double *L2_out, double *step_out) {
     c->step_est = step;
     if (L2_out) *L2_out = L2;
     if (step_out) *step_out = step;
+    return FWI_OK;
+}
+
+int32_t fwi_tuning(const fwi_ctx *c, int32_t *tile_tz, char *log, int32_t log_len) {
+    if (!c) return FWI_EINVAL;
+    if (tile_tz) *tile_tz = c->tz;
+    if (log && log_len > 0) {
+        std::strncpy(log, c->tune_log.c_str(), (size_t)log_len - 1);
+        log[log_len - 1] = 0;
+    }
     return FWI_OK;
 }
 

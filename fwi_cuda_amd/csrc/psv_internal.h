@@ -29,6 +29,7 @@ constexpr int kBY = 4;            // ... times four z-segments
 // Everything a stencil / injection kernel reads or writes, passed by value.
 struct KArgs {
     int nz, nx, P, B;
+    int order;                 // work order of the stencil launches (tuned)
     size_t fs;                 // field stride per shot = nz * P
     float dt;
     float cz[6], cx[6];        // hc[k] / dz, hc[k] / dx
@@ -81,8 +82,9 @@ struct RecArgs {
 };
 
 // ---- launchers (psv_kernels.hip) ------------------------------------------------
-void launch_stress(const KArgs &a, int N, int tz, hipStream_t st);
-void launch_velocity(const KArgs &a, int N, int tz, hipStream_t st);
+// var 1: x taps as shifted global loads; var 2: x taps from neighbouring lanes, next row prefetched
+void launch_stress(const KArgs &a, int N, int tz, int var, hipStream_t st);
+void launch_velocity(const KArgs &a, int N, int tz, int var, hipStream_t st);
 void launch_surface(const KArgs &a, int N, const int isurf[4], hipStream_t st);
 void launch_inject(const KArgs &a, const SrcArgs &s, int it, hipStream_t st);
 void launch_record(const KArgs &a, const RecArgs &r, int it, hipStream_t st);

@@ -27,14 +27,46 @@ __device__ __forceinline__ size_t fidx(const KArgs &a, int s, int z, int x) {
     return (size_t)s * a.fs + (size_t)z * a.P + x;
 }
 
+// Workgroups are dispatched round-robin over the 8 XCDs (each with its own L2).
+// The 1-D launch index is remapped so that every XCD owns one contiguous range
+// of work, ordered tile-major and shot-minor: the B shots of a tile run back to
+// back on one XCD (the tile's media and CPML profiles stay in that L2), then the
+// vertically adjacent segment (its z halo rows are the previous tile's rows).
+template <int TZ, int OUTW = kBX>
+__device__ __forceinline__ void tile_of(const KArgs &a, int N, int &bx, int &by, int &s) {
+    const int gx = (a.nx - 2 * N + OUTW - 1) / OUTW;
+    const int gy = ((a.nz - 2 * N + TZ - 1) / TZ + kBY - 1) / kBY;
+    const int n = gx * gy * a.B;
+    const int L = blockIdx.x, xcd = L & 7, loc = L >> 3;
+    const int q = n >> 3, r = n & 7;
+    const int W = xcd * q + min(xcd, r) + loc;
+    if (a.order == 0) {          // shot fastest, then z segments, then x tiles
+        s = W % a.B;
+        const int t = W / a.B;
+        by = t % gy;
+        bx = t / gy;
+    } else if (a.order == 1) {   // shot fastest, then x tiles, then z segments
+        s = W % a.B;
+        const int t = W / a.B;
+        bx = t % gx;
+        by = t / gx;
+    } else {                     // x tiles fastest, then z segments, shots outermost
+        const int t = W % (gx * gy);
+        s = W / (gx * gy);
+        bx = t % gx;
+        by = t / gx;
+    }
+}
+
 template <int N, int TZ>
 __global__ __launch_bounds__(kBX *kBY) void k_stress(const KArgs a) {
-    const int ix = N + blockIdx.x * kBX + threadIdx.x;
-    const int z0 = N + (blockIdx.y * kBY + threadIdx.y) * TZ;
+    int tbx, tby, s;
+    tile_of<TZ>(a, N, tbx, tby, s);
+    const int ix = N + tbx * kBX + threadIdx.x;
+    const int z0 = N + (tby * kBY + threadIdx.y) * TZ;
     const int nzi = a.nz - N;
     if (ix >= a.nx - N || z0 >= nzi) return;
     const int zend = min(z0 + TZ, nzi);
-    const int s = blockIdx.z;
     const int P = a.P;
     const size_t fo = (size_t)s * a.fs;
     const float *__restrict__ vz = a.vz + fo;
@@ -104,12 +136,13 @@ __global__ __launch_bounds__(kBX *kBY) void k_stress(const KArgs a) {
 
 template <int N, int TZ>
 __global__ __launch_bounds__(kBX *kBY) void k_velocity(const KArgs a) {
-    const int ix = N + blockIdx.x * kBX + threadIdx.x;
-    const int z0 = N + (blockIdx.y * kBY + threadIdx.y) * TZ;
+    int tbx, tby, s;
+    tile_of<TZ>(a, N, tbx, tby, s);
+    const int ix = N + tbx * kBX + threadIdx.x;
+    const int z0 = N + (tby * kBY + threadIdx.y) * TZ;
     const int nzi = a.nz - N;
     if (ix >= a.nx - N || z0 >= nzi) return;
     const int zend = min(z0 + TZ, nzi);
-    const int s = blockIdx.z;
     const int P = a.P;
     const size_t fo = (size_t)s * a.fs;
     float *__restrict__ vz = a.vz + fo;
@@ -202,6 +235,249 @@ __global__ __launch_bounds__(kBX *kBY) void k_velocity(const KArgs a) {
             const float s1 = 0.25f * il * (Szz + Sxx) * (c_zz + c_xx);
             const float s2 = 0.25f * im * (Szz - Sxx) * (c_zz - c_xx);
             const float s3 = im * (Szx * c_zx);
+            const float s4 = ux * Vx + uz * Vz;
+            a.g_lam[q] += a.gw * s1;
+            a.g_mu[q] += a.gw * (s3 + s1 + s2);
+            a.g_rho[q] += a.gw * s4;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Variant 2: x taps from neighbouring lanes, next row prefetched.
+//
+// A wavefront covers 64 consecutive columns; its N lanes at each end are halo
+// lanes that only load, so OUTW = 64 - 2N columns are written per wavefront.
+// The z-queue registers already hold every lane's value of the current row,
+// so the x-derivative taps are cross-lane reads of those registers
+// (ds_bpermute) instead of 2N shifted global loads per field.  The global
+// loads of row z + 1 (queue entries, stresses, media) are issued before row z
+// is computed, which doubles the loads in flight per wavefront.
+// ---------------------------------------------------------------------------
+
+// value of v in lane (lane + d); d is a compile-time constant after unrolling
+__device__ __forceinline__ float lane_off(float v, int lane, int d) {
+    return d == 0 ? v : __shfl(v, lane + d);
+}
+
+template <int N, int TZ, int MINW = 1>
+__global__ __launch_bounds__(kBX *kBY, MINW) void k_stress2(const KArgs a) {
+    constexpr int OUTW = kBX - 2 * N;
+    int tbx, tby, s;
+    tile_of<TZ, OUTW>(a, N, tbx, tby, s);
+    const int lane = threadIdx.x;
+    const int ix = tbx * OUTW + lane;   // lane N holds the first output column N + tbx * OUTW
+    const int z0 = N + (tby * kBY + threadIdx.y) * TZ;
+    const int nzi = a.nz - N;
+    if (z0 >= nzi) return;              // uniform over the wavefront
+    const int zend = min(z0 + TZ, nzi);
+    const bool out = lane >= N && lane < kBX - N && ix < a.nx - N;
+    const int cx = min(ix, a.nx - 1);   // halo lanes past the grid read a valid column
+    const int P = a.P;
+    const size_t fo = (size_t)s * a.fs;
+    const float *__restrict__ vz = a.vz + fo;
+    const float *__restrict__ vx = a.vx + fo;
+    float *__restrict__ szz = a.szz + fo;
+    float *__restrict__ szx = a.szx + fo;
+    float *__restrict__ sxx = a.sxx + fo;
+    const float dt = a.dt;
+
+    const int cxm = out ? a.xmap[ix] : -1;
+    float bx = 1.f, ax = 0.f, ikx = 1.f, bhx = 1.f, ahx = 0.f, ikhx = 1.f;
+    if (cxm >= 0) {
+        bx = a.bx[ix]; ax = a.ax[ix]; ikx = a.ikx[ix];
+        bhx = a.bhx[ix]; ahx = a.ahx[ix]; ikhx = a.ikhx[ix];
+    }
+    // qz[j] = vz[z - N + j], qx[j] = vx[z - N + 1 + j]
+    float qz[2 * N], qx[2 * N];
+#pragma unroll
+    for (int j = 0; j < 2 * N - 1; ++j) {
+        qz[j + 1] = vz[(size_t)(z0 - N + j) * P + cx];
+        qx[j + 1] = vx[(size_t)(z0 - N + 1 + j) * P + cx];
+    }
+    // prefetched row
+    float nvz = vz[(size_t)(z0 + N - 1) * P + cx], nvx = vx[(size_t)(z0 + N) * P + cx];
+    float n_zz = 0.f, n_zx = 0.f, n_xx = 0.f, n_l = 0.f, n_m = 0.f, n_mzx = 0.f;
+    if (out) {
+        const size_t r = (size_t)z0 * P + ix;
+        n_zz = szz[r]; n_zx = szx[r]; n_xx = sxx[r];
+        n_l = a.lam[r]; n_m = a.mu[r]; n_mzx = a.muzx[r];
+    }
+    for (int z = z0; z < zend; ++z) {
+#pragma unroll
+        for (int j = 0; j < 2 * N - 1; ++j) {
+            qz[j] = qz[j + 1];
+            qx[j] = qx[j + 1];
+        }
+        qz[2 * N - 1] = nvz;
+        qx[2 * N - 1] = nvx;
+        const float c_zz = n_zz, c_zx = n_zx, c_xx = n_xx, lam = n_l, mu = n_m, muzx = n_mzx;
+        if (z + 1 < zend) {
+            nvz = vz[(size_t)(z + N) * P + cx];
+            nvx = vx[(size_t)(z + N + 1) * P + cx];
+            if (out) {
+                const size_t r1 = (size_t)(z + 1) * P + ix;
+                n_zz = szz[r1]; n_zx = szx[r1]; n_xx = sxx[r1];
+                n_l = a.lam[r1]; n_m = a.mu[r1]; n_mzx = a.muzx[r1];
+            }
+        }
+        const float vzc = qz[N], vxc = qx[N - 1];   // this lane's vz, vx on row z
+        float vz_z = 0.f, vx_z = 0.f, vz_x = 0.f, vx_x = 0.f;
+#pragma unroll
+        for (int k = 1; k <= N; ++k) {
+            vz_z += a.cz[k - 1] * (qz[N + k - 1] - qz[N - k]);
+            vx_z += a.cz[k - 1] * (qx[N + k - 1] - qx[N - k]);
+            vz_x += a.cx[k - 1] * (lane_off(vzc, lane, k) - lane_off(vzc, lane, 1 - k));
+            vx_x += a.cx[k - 1] * (lane_off(vxc, lane, k - 1) - lane_off(vxc, lane, -k));
+        }
+        if (!out) continue;
+        const size_t r = (size_t)z * P + ix;
+        const int rz = a.zmap[z];
+        if (rz >= 0) {
+            const size_t pi = ((size_t)s * a.nzs + rz) * P + ix;
+            const float p1 = a.bz[z] * a.pvz_z[pi] + a.az[z] * vz_z;
+            const float p2 = a.bhz[z] * a.pvx_z[pi] + a.ahz[z] * vx_z;
+            a.pvz_z[pi] = p1;
+            a.pvx_z[pi] = p2;
+            vz_z = vz_z * a.ikz[z] + p1;
+            vx_z = vx_z * a.ikhz[z] + p2;
+        }
+        if (cxm >= 0) {
+            const size_t pi = ((size_t)s * a.nz + z) * a.nxs + cxm;
+            const float p1 = bx * a.pvx_x[pi] + ax * vx_x;
+            const float p2 = bhx * a.pvz_x[pi] + ahx * vz_x;
+            a.pvx_x[pi] = p1;
+            a.pvz_x[pi] = p2;
+            vx_x = vx_x * ikx + p1;
+            vz_x = vz_x * ikhx + p2;
+        }
+        const float div = lam * (vx_x + vz_z);
+        szz[r] = c_zz + dt * (div + 2.f * mu * vz_z);
+        szx[r] = c_zx + dt * muzx * (vz_x + vx_z);
+        sxx[r] = c_xx + dt * (div + 2.f * mu * vx_x);
+    }
+}
+
+template <int N, int TZ, int MINW = 1>
+__global__ __launch_bounds__(kBX *kBY, MINW) void k_velocity2(const KArgs a) {
+    constexpr int OUTW = kBX - 2 * N;
+    int tbx, tby, s;
+    tile_of<TZ, OUTW>(a, N, tbx, tby, s);
+    const int lane = threadIdx.x;
+    const int ix = tbx * OUTW + lane;
+    const int z0 = N + (tby * kBY + threadIdx.y) * TZ;
+    const int nzi = a.nz - N;
+    if (z0 >= nzi) return;
+    const int zend = min(z0 + TZ, nzi);
+    const bool out = lane >= N && lane < kBX - N && ix < a.nx - N;
+    const int cx = min(ix, a.nx - 1);
+    const int P = a.P;
+    const size_t fo = (size_t)s * a.fs;
+    float *__restrict__ vz = a.vz + fo;
+    float *__restrict__ vx = a.vx + fo;
+    const float *__restrict__ szz = a.szz + fo;
+    const float *__restrict__ szx = a.szx + fo;
+    const float *__restrict__ sxx = a.sxx + fo;
+    const float dt = a.dt;
+
+    const int cxm = out ? a.xmap[ix] : -1;
+    float bx = 1.f, ax = 0.f, ikx = 1.f, bhx = 1.f, ahx = 0.f, ikhx = 1.f;
+    if (cxm >= 0) {
+        bx = a.bx[ix]; ax = a.ax[ix]; ikx = a.ikx[ix];
+        bhx = a.bhx[ix]; ahx = a.ahx[ix]; ikhx = a.ikhx[ix];
+    }
+    const bool gcol = out && a.grad && ix >= a.snap_x1 && ix <= a.snap_x2 && (ix - a.snap_x1) % a.snap_dx == 0;
+    const int jx = gcol ? (ix - a.snap_x1) / a.snap_dx : 0;
+
+    // qzz[j] = szz[z - N + 1 + j], qzx[j] = szx[z - N + j]
+    float qzz[2 * N], qzx[2 * N];
+#pragma unroll
+    for (int j = 0; j < 2 * N - 1; ++j) {
+        qzz[j + 1] = szz[(size_t)(z0 - N + 1 + j) * P + cx];
+        qzx[j + 1] = szx[(size_t)(z0 - N + j) * P + cx];
+    }
+    float n_szz = szz[(size_t)(z0 + N) * P + cx], n_szx = szx[(size_t)(z0 + N - 1) * P + cx];
+    float n_sxx = sxx[(size_t)z0 * P + cx];
+    float n_vz = 0.f, n_vx = 0.f, n_rz = 0.f, n_rx = 0.f;
+    if (out) {
+        const size_t r = (size_t)z0 * P + ix;
+        n_vz = vz[r]; n_vx = vx[r]; n_rz = a.rhoz[r]; n_rx = a.rhox[r];
+    }
+    for (int z = z0; z < zend; ++z) {
+#pragma unroll
+        for (int j = 0; j < 2 * N - 1; ++j) {
+            qzz[j] = qzz[j + 1];
+            qzx[j] = qzx[j + 1];
+        }
+        qzz[2 * N - 1] = n_szz;
+        qzx[2 * N - 1] = n_szx;
+        const float sxxc = n_sxx, vz0 = n_vz, vx0 = n_vx, rz_ = n_rz, rx_ = n_rx;
+        if (z + 1 < zend) {
+            n_szz = szz[(size_t)(z + 1 + N) * P + cx];
+            n_szx = szx[(size_t)(z + N) * P + cx];
+            n_sxx = sxx[(size_t)(z + 1) * P + cx];
+            if (out) {
+                const size_t r1 = (size_t)(z + 1) * P + ix;
+                n_vz = vz[r1]; n_vx = vx[r1]; n_rz = a.rhoz[r1]; n_rx = a.rhox[r1];
+            }
+        }
+        const float szxc = qzx[N];   // this lane's szx on row z
+        float szz_z = 0.f, szx_z = 0.f, szx_x = 0.f, sxx_x = 0.f;
+#pragma unroll
+        for (int k = 1; k <= N; ++k) {
+            szz_z += a.cz[k - 1] * (qzz[N + k - 1] - qzz[N - k]);
+            szx_z += a.cz[k - 1] * (qzx[N + k - 1] - qzx[N - k]);
+            szx_x += a.cx[k - 1] * (lane_off(szxc, lane, k - 1) - lane_off(szxc, lane, -k));
+            sxx_x += a.cx[k - 1] * (lane_off(sxxc, lane, k) - lane_off(sxxc, lane, 1 - k));
+        }
+        if (!out) continue;
+        const size_t r = (size_t)z * P + ix;
+        // integer-point profiles for szz_z and sxx_x, half-point for szx_z and szx_x,
+        // the reference's slot mapping (src/cpu/n_kernel_PSV.cpp:142-146)
+        const int rzm = a.zmap[z];
+        if (rzm >= 0) {
+            const size_t pi = ((size_t)s * a.nzs + rzm) * P + ix;
+            const float p1 = a.bz[z] * a.pszz_z[pi] + a.az[z] * szz_z;
+            const float p2 = a.bhz[z] * a.pszx_z[pi] + a.ahz[z] * szx_z;
+            a.pszz_z[pi] = p1;
+            a.pszx_z[pi] = p2;
+            szz_z = szz_z * a.ikz[z] + p1;
+            szx_z = szx_z * a.ikhz[z] + p2;
+        }
+        if (cxm >= 0) {
+            const size_t pi = ((size_t)s * a.nz + z) * a.nxs + cxm;
+            const float p1 = bx * a.psxx_x[pi] + ax * sxx_x;
+            const float p2 = bhx * a.pszx_x[pi] + ahx * szx_x;
+            a.psxx_x[pi] = p1;
+            a.pszx_x[pi] = p2;
+            sxx_x = sxx_x * ikx + p1;
+            szx_x = szx_x * ikhx + p2;
+        }
+        const float vz1 = vz0 + dt * rz_ * (szx_x + szz_z);
+        const float vx1 = vx0 + dt * rx_ * (sxx_x + szx_z);
+        vz[r] = vz1;
+        vx[r] = vx1;
+        float uz = 0.f, ux = 0.f;
+        if (a.displ) {
+            uz = a.uz[fo + r] + dt * vz0;
+            ux = a.ux[fo + r] + dt * vx0;
+            a.uz[fo + r] = uz;
+            a.ux[fo + r] = ux;
+        }
+        if (a.energy) a.We[fo + r] += vx1 * vx1 + vz1 * vz1;
+        if (gcol && z >= a.snap_z1 && z <= a.snap_z2 && (z - a.snap_z1) % a.snap_dz == 0) {
+            const int jz = (z - a.snap_z1) / a.snap_dz;
+            const size_t q = (size_t)s * a.snz * a.snx + (size_t)jz * a.snx + jx;
+            const size_t o = (size_t)s * a.sn_stride + (size_t)jz * a.snx + jx;
+            const float Szz = a.sn_szz[o], Sxx = a.sn_sxx[o], Szx = a.sn_szx[o];
+            const float Vz = a.sn_vz[o], Vx = a.sn_vx[o];
+            const float lam = a.lam[r], mu = a.mu[r];
+            const float c_zz = qzz[N - 1], c_xx = sxxc;
+            const float lm = lam + mu;
+            const float il = a.gain_lm / (lm * lm), im = a.gain_lm / (mu * mu);
+            const float s1 = 0.25f * il * (Szz + Sxx) * (c_zz + c_xx);
+            const float s2 = 0.25f * im * (Szz - Sxx) * (c_zz - c_xx);
+            const float s3 = im * (Szx * szxc);
             const float s4 = ux * Vx + uz * Vz;
             a.g_lam[q] += a.gw * s1;
             a.g_mu[q] += a.gw * (s3 + s1 + s2);
@@ -330,38 +606,55 @@ __global__ void k_store(const KArgs a, float *d0, float *d1, float *d2, float *d
 
 }  // namespace
 
-#define PSV_TZ_CASES(KER, N)                                                     \
-    switch (tz) {                                                                \
-        case 4: hipLaunchKernelGGL((KER<N, 4>), grid, block, 0, st, a); break;   \
-        case 8: hipLaunchKernelGGL((KER<N, 8>), grid, block, 0, st, a); break;   \
-        case 32: hipLaunchKernelGGL((KER<N, 32>), grid, block, 0, st, a); break; \
-        default: hipLaunchKernelGGL((KER<N, 16>), grid, block, 0, st, a); break; \
+// KER<N, TZ, EXTRA...>: EXTRA is empty or the min-waves bound of variant 3
+#define PSV_TZ_CASES(KER, N, ...)                                                             \
+    switch (tz) {                                                                             \
+        case 8: hipLaunchKernelGGL((KER<N, 8 __VA_ARGS__>), grid, block, 0, st, a); break;   \
+        case 32: hipLaunchKernelGGL((KER<N, 32 __VA_ARGS__>), grid, block, 0, st, a); break; \
+        case 64: hipLaunchKernelGGL((KER<N, 64 __VA_ARGS__>), grid, block, 0, st, a); break; \
+        default: hipLaunchKernelGGL((KER<N, 16 __VA_ARGS__>), grid, block, 0, st, a); break; \
     }
 
-#define PSV_N_CASES(KER)                   \
-    switch (N) {                           \
-        case 1: PSV_TZ_CASES(KER, 1) break; \
-        case 2: PSV_TZ_CASES(KER, 2) break; \
-        case 3: PSV_TZ_CASES(KER, 3) break; \
-        case 4: PSV_TZ_CASES(KER, 4) break; \
-        case 5: PSV_TZ_CASES(KER, 5) break; \
-        default: PSV_TZ_CASES(KER, 6) break; \
+#define PSV_N_CASES(KER, ...)                            \
+    switch (N) {                                         \
+        case 1: PSV_TZ_CASES(KER, 1, __VA_ARGS__) break; \
+        case 2: PSV_TZ_CASES(KER, 2, __VA_ARGS__) break; \
+        case 3: PSV_TZ_CASES(KER, 3, __VA_ARGS__) break; \
+        case 4: PSV_TZ_CASES(KER, 4, __VA_ARGS__) break; \
+        case 5: PSV_TZ_CASES(KER, 5, __VA_ARGS__) break; \
+        default: PSV_TZ_CASES(KER, 6, __VA_ARGS__) break; \
     }
 
-static dim3 stencil_grid(const KArgs &a, int N, int tz) {
+// one-dimensional launch; tile_of() maps the index to (tile, shot).  Variant 1
+// writes 64 columns per wavefront, variants 2 and 3 write 64 - 2N.  Variant 3 is
+// variant 2 compiled for at least 6 wavefronts per SIMD (a register cap).
+static dim3 stencil_grid(const KArgs &a, int N, int tz, int var) {
     const int nxi = a.nx - 2 * N, nzi = a.nz - 2 * N;
     const int segs = (nzi + tz - 1) / tz;
-    return dim3((nxi + kBX - 1) / kBX, (segs + kBY - 1) / kBY, a.B);
+    const int outw = var >= 2 ? kBX - 2 * N : kBX;
+    return dim3(((nxi + outw - 1) / outw) * ((segs + kBY - 1) / kBY) * a.B);
 }
 
-void launch_stress(const KArgs &a, int N, int tz, hipStream_t st) {
-    const dim3 grid = stencil_grid(a, N, tz), block(kBX, kBY);
-    PSV_N_CASES(k_stress)
+void launch_stress(const KArgs &a, int N, int tz, int var, hipStream_t st) {
+    const dim3 grid = stencil_grid(a, N, tz, var), block(kBX, kBY);
+    if (var == 3) {
+        PSV_N_CASES(k_stress2, , 6)
+    } else if (var == 2) {
+        PSV_N_CASES(k_stress2, )
+    } else {
+        PSV_N_CASES(k_stress, )
+    }
 }
 
-void launch_velocity(const KArgs &a, int N, int tz, hipStream_t st) {
-    const dim3 grid = stencil_grid(a, N, tz), block(kBX, kBY);
-    PSV_N_CASES(k_velocity)
+void launch_velocity(const KArgs &a, int N, int tz, int var, hipStream_t st) {
+    const dim3 grid = stencil_grid(a, N, tz, var), block(kBX, kBY);
+    if (var == 3) {
+        PSV_N_CASES(k_velocity2, , 6)
+    } else if (var == 2) {
+        PSV_N_CASES(k_velocity2, )
+    } else {
+        PSV_N_CASES(k_velocity, )
+    }
 }
 
 void launch_surface(const KArgs &a, int N, const int isurf[4], hipStream_t st) {

@@ -135,7 +135,7 @@ class Simulation:
     """
 
     def __init__(self, case: dict, *, batch=0, storage=_lib.STORE_AUTO, ckpt_interval=0, mem_budget=0,
-                 form=_lib.FORM_FUSED, grad_sign=-1, tile_tz=0, device=0, check_finite=1):
+                 form=_lib.FORM_FUSED, grad_sign=-1, tile_tz=0, device=0, check_finite=1, kernel_variant=0):
         lib = _lib.load()
         self.lib = lib
         self.case = case
@@ -168,6 +168,7 @@ class Simulation:
         o.batch, o.storage, o.ckpt_interval, o.mem_budget = int(batch), int(storage), int(ckpt_interval), int(mem_budget)
         o.form, o.grad_sign, o.tile_tz, o.device = int(form), int(grad_sign), int(tile_tz), int(device)
         o.check_finite = int(check_finite)
+        o.kernel_variant = int(kernel_variant)
         h = C.c_void_p()
         check(lib.fwi_create(C.byref(g), C.byref(q), C.byref(o), C.byref(h)), None)
         self.ctx = h
@@ -334,6 +335,22 @@ class Simulation:
         return hist
 
     # -- profiling --------------------------------------------------------------
+    def tuning(self):
+        tz = C.c_int32()
+        buf = C.create_string_buffer(512)
+        self._chk(self.lib.fwi_tuning(self.ctx, C.byref(tz), buf, 512))
+        return tz.value, buf.value.decode()
+
+    def forward_snapshots(self, shot):
+        """One shot's forward with its snapshot cubes (write_accu's arrays)."""
+        z1, z2, x1, x2, sdt, sdz, sdx = (int(v) for v in self.case["snap"])
+        snt = 1 + (self.nt - 1) // sdt
+        cubes = [np.zeros((snt,) + self.snap_shape) for _ in range(5)]
+        n = self._rows(shot, 1)
+        uz, ux = np.zeros((n, self.nt)), np.zeros((n, self.nt))
+        self._chk(self.lib.fwi_forward_snapshots(self.ctx, shot, dptr(uz), dptr(ux), *[dptr(a) for a in cubes]))
+        return (uz, ux), dict(zip(("vz", "vx", "szz", "szx", "sxx"), cubes))
+
     def profile(self, on=True):
         self._chk(self.lib.fwi_profile(self.ctx, int(on)))
 

@@ -124,6 +124,8 @@ typedef struct {
     int32_t tile_tz;        /* rows per thread in the stencil kernels; 0 = tuned */
     int32_t device;         /* HIP device ordinal */
     int32_t check_finite;   /* scan the wavefield for NaN/Inf after each propagation */
+    int32_t kernel_variant; /* stencil kernels: 0 = tuned, 1 = x taps as shifted loads,
+                               2 = x taps from neighbouring lanes with next-row prefetch */
 } fwi_opts;
 
 /* Per-kernel timing collected while profiling is on (HIP events on the
@@ -211,6 +213,9 @@ int32_t fwi_step_length(fwi_ctx *ctx, double est, double L2_0, double *step_out,
 int32_t fwi_iteration(fwi_ctx *ctx, double *L2, double *step);
 
 /* ---- profiling ------------------------------------------------------------------ */
+/* Rows per thread the stencil kernels use (the autotuner's choice once a
+ * propagation has run) and the tuner's timings as text. */
+int32_t fwi_tuning(const fwi_ctx *ctx, int32_t *tile_tz, char *log, int32_t log_len);
 int32_t fwi_profile(fwi_ctx *ctx, int32_t enable);
 int32_t fwi_kernel_times(fwi_ctx *ctx, fwi_kernel_time times[FWI_K_COUNT]);
 int32_t fwi_sync(fwi_ctx *ctx);

@@ -85,11 +85,12 @@ def test_shot_gradient_matches_reference_golden(psv, name):
 
 
 @pytest.mark.parametrize("surface", [True, False])
-def test_order8_forward_vs_oracle(psv, O, surface):
+@pytest.mark.parametrize("variant", [1, 2])
+def test_order8_forward_vs_oracle(psv, O, surface, variant):
     c = cases.small_case(O, surface=surface, nshot=2)
     med = O.mat_av2(c)
     ref = [O.kernel_psv(c, med, s, accu=False) for s in range(2)]
-    with psv.Simulation(c) as sim:
+    with psv.Simulation(c, kernel_variant=variant) as sim:
         uz, ux = sim.forward()
         for s, (a, b) in enumerate(zip(sim.split(uz), sim.split(ux))):
             ez, ex = cases.rel_l2(a, ref[s]["uz"]), cases.rel_l2(b, ref[s]["ux"])
@@ -99,7 +100,8 @@ def test_order8_forward_vs_oracle(psv, O, surface):
         assert e < 1e-4
 
 
-def test_order8_gradient_vs_oracle(psv, O):
+@pytest.mark.parametrize("variant", [1, 2])
+def test_order8_gradient_vs_oracle(psv, O, variant):
     c = cases.small_case(O, nshot=2, nt=450)
     med_true = O.mat_av2(c)
     obs = cases.per_shot_obs(c, O, med_true)
@@ -112,7 +114,7 @@ def test_order8_gradient_vs_oracle(psv, O):
         L2 += sg["L2"]
         for i, k in enumerate(("g_lam", "g_mu", "g_rho")):
             G[i] += sg[k]
-    with psv.Simulation(c) as sim:
+    with psv.Simulation(c, kernel_variant=variant) as sim:
         sim.set_model(lam0, mu0, rho0)
         sim.set_observed([o[0] for o in obs], [o[1] for o in obs])
         m = sim.gradient()

@@ -134,6 +134,7 @@ def main():
     ap.add_argument("--batch", type=int, default=0)
     ap.add_argument("--tile-tz", type=int, default=0)
     ap.add_argument("--variant", type=int, default=0, help="stencil kernel variant (0 = tuned)")
+    ap.add_argument("--order", type=int, default=0, help="stencil work order 1..3 (0 = tuned)")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -164,7 +165,8 @@ def main():
     N = int(c["fdorder"]) // 2
     log(f"workload {args.config}: {nz}x{nx}, nt {nt}, {nshot} shots, {int(c['rec_off'][-1]) if 'rec_off' in c else ''} "
         "receivers")
-    sim = psv.Simulation(c, batch=args.batch, tile_tz=args.tile_tz, kernel_variant=args.variant)
+    sim = psv.Simulation(c, batch=args.batch, tile_tz=args.tile_tz, kernel_variant=args.variant,
+                         work_order=args.order)
     plan = sim.plan()
     log("plan", plan)
     t0 = time.perf_counter()

@@ -42,7 +42,7 @@ class FwiOpts(C.Structure):
     _fields_ = [("batch", C.c_int32), ("storage", C.c_int32), ("ckpt_interval", C.c_int32),
                 ("mem_budget", C.c_int64), ("form", C.c_int32), ("grad_sign", C.c_int32),
                 ("use_graph", C.c_int32), ("tile_tz", C.c_int32), ("device", C.c_int32),
-                ("check_finite", C.c_int32), ("kernel_variant", C.c_int32)]
+                ("check_finite", C.c_int32), ("kernel_variant", C.c_int32), ("work_order", C.c_int32)]
 
 
 class FwiKernelTime(C.Structure):

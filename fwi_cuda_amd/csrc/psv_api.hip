@@ -156,6 +156,24 @@ hipEvent_t ev_get(fwi_ctx *c) {
     return e;
 }
 
+// Events of the first n pending pairs (all complete) into the per-kernel totals.
+void prof_take(fwi_ctx *c, size_t n) {
+    for (size_t i = 0; i < n; ++i) {
+        const EventPair &p = c->ev_pending[i];
+        float ms = 0;
+        hipEventElapsedTime(&ms, p.a, p.b);
+        c->times[p.kid].launches += 1;
+        c->times[p.kid].total_ms += ms;
+        c->ev_pool.push_back(p.a);
+        c->ev_pool.push_back(p.b);
+    }
+    c->ev_pending.erase(c->ev_pending.begin(), c->ev_pending.begin() + n);
+}
+
+// At most kMaxPending timed launches are outstanding: past that, the host waits
+// for the older half (the stream keeps running the rest) and folds it in.
+constexpr size_t kMaxPending = 2048;
+
 struct Timer {
     fwi_ctx *c;
     int kid;
@@ -171,6 +189,11 @@ struct Timer {
             hipEvent_t b = ev_get(c);
             hipEventRecord(b, c->st);
             c->ev_pending.push_back({kid, a, b});
+            if (c->ev_pending.size() >= kMaxPending) {
+                const size_t half = kMaxPending / 2;
+                hipEventSynchronize(c->ev_pending[half - 1].b);
+                prof_take(c, half);
+            }
         }
     }
 };
@@ -178,15 +201,7 @@ struct Timer {
 void prof_collect(fwi_ctx *c) {
     if (c->ev_pending.empty()) return;
     hipStreamSynchronize(c->st);
-    for (auto &p : c->ev_pending) {
-        float ms = 0;
-        hipEventElapsedTime(&ms, p.a, p.b);
-        c->times[p.kid].launches += 1;
-        c->times[p.kid].total_ms += ms;
-        c->ev_pool.push_back(p.a);
-        c->ev_pool.push_back(p.b);
-    }
-    c->ev_pending.clear();
+    prof_take(c, c->ev_pending.size());
 }
 
 int sync(fwi_ctx *c) {
@@ -413,10 +428,19 @@ int autotune(fwi_ctx *c, int nb) {
     if (c->plain || c->tuned) return FWI_OK;
     const TuneKey key{c->g.nz, c->g.nx, c->N, nb};
     const bool forced = c->o.kernel_variant >= 1 && c->o.kernel_variant <= 3;
+    const bool order_forced = c->o.work_order >= 1 && c->o.work_order <= 3;
+    if (forced && order_forced && c->o.tile_tz > 0) {   // nothing left to choose
+        c->var = c->o.kernel_variant;
+        c->tz = c->o.tile_tz;
+        c->order = c->o.work_order - 1;
+        c->tuned = true;
+        return FWI_OK;
+    }
     {
         std::lock_guard<std::mutex> lk(g_tune_mu);
         auto it = g_tuned.find(key);
         if (it != g_tuned.end() && (!forced || c->o.kernel_variant == it->second.var) &&
+            (!order_forced || c->o.work_order - 1 == it->second.order) &&
             (c->o.tile_tz <= 0 || c->o.tile_tz == it->second.tz)) {
             c->var = it->second.var;
             c->tz = it->second.tz;
@@ -437,8 +461,9 @@ int autotune(fwi_ctx *c, int nb) {
     std::vector<int> vars = {1, 2, 3};
     if (forced) vars = {c->o.kernel_variant};
     float best = 1e30f;
-    TuneVal bv{vars[0], tzs[0], 0};
-    for (int order = 0; order < 3; ++order) {
+    const int o_lo = order_forced ? c->o.work_order - 1 : 0, o_hi = order_forced ? o_lo + 1 : 3;
+    TuneVal bv{vars[0], tzs[0], o_lo};
+    for (int order = o_lo; order < o_hi; ++order) {
         a.order = order;
         for (int var : vars) {
             for (int tz : tzs) {
@@ -893,6 +918,13 @@ int validate(fwi_ctx *c, const fwi_grid *g, const fwi_acq *q) {
     const int N = g->fdorder / 2;
     if (g->nz < 2 * N + 1 || g->nx < 2 * N + 1) return c->fail(FWI_EINVAL, "grid smaller than the stencil");
     if (!(g->dt > 0) || !(g->dz > 0) || !(g->dx > 0)) return c->fail(FWI_EINVAL, "dt, dz, dx must be positive");
+    const int tt = c->o.tile_tz;
+    if (tt != 0 && tt != 8 && tt != 16 && tt != 32 && tt != 64)
+        return c->fail(FWI_EINVAL, "tile_tz %d (0, 8, 16, 32 or 64)", tt);
+    if (c->o.kernel_variant < 0 || c->o.kernel_variant > 3)
+        return c->fail(FWI_EINVAL, "kernel_variant %d (0..3)", c->o.kernel_variant);
+    if (c->o.work_order < 0 || c->o.work_order > 3)
+        return c->fail(FWI_EINVAL, "work_order %d (0..3)", c->o.work_order);
     if (g->snap_dt < 1 || g->snap_dz < 1 || g->snap_dx < 1) return c->fail(FWI_EINVAL, "snap strides < 1");
     if (g->snap_z1 < 0 || g->snap_z2 >= g->nz || g->snap_z1 > g->snap_z2 || g->snap_x1 < 0 ||
         g->snap_x2 >= g->nx || g->snap_x1 > g->snap_x2)

@@ -125,7 +125,11 @@ typedef struct {
     int32_t device;         /* HIP device ordinal */
     int32_t check_finite;   /* scan the wavefield for NaN/Inf after each propagation */
     int32_t kernel_variant; /* stencil kernels: 0 = tuned, 1 = x taps as shifted loads,
-                               2 = x taps from neighbouring lanes with next-row prefetch */
+                               2 = x taps from neighbouring lanes with next-row prefetch,
+                               3 = variant 2 at six waves per SIMD */
+    int32_t work_order;     /* order of the stencil tiles over the grid: 0 = tuned,
+                               1 = shot fastest then z then x, 2 = shot fastest then x then z,
+                               3 = x tiles fastest, shots outermost */
 } fwi_opts;
 
 /* Per-kernel timing collected while profiling is on (HIP events on the

@@ -16,8 +16,9 @@ and adjoint cells, recomputation not counted), with
                 average duration (HIP events on the library's stream), against
                 MI355X's 8 TB/s;
   hbm_frac      the whole step's algorithmic bytes (fwd 68 B, adjoint 84 B,
-                gradient 44 B, recomputed step 60 B per cell-step, CPML strips
-                32 B per cell-step per direction) over wall time and 8 TB/s;
+                gradient 44 B and snapshot 20 B per snapshot step, recomputed
+                step 60 B per cell-step, CPML strips 32 B per cell-step per
+                direction) over wall time and 8 TB/s;
   plain         the reference-structured per-shot form on the same grid
                 (shots/s) and the speed-up in shots/s;
   cpu_baseline  the reference's CPU path (compiled from its sources) on a
@@ -38,12 +39,23 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 HBM_PEAK = 8.0e12
-# per-cell algorithmic bytes of each fused kernel (FP32 fields, read once, write once)
+METRIC = "fwd+adj gradient Gcell-updates/s & % HBM peak; batched-checkpointed vs plain"
+# Per interior cell and launch, algorithmic bytes of each stencil kernel (FP32;
+# every field and medium read once, every updated field written once).
+# One-pass step (k_step):
+B_STEP_FWD = 68        # RW vz vx szz szx sxx, R lam mu mu_zx rho_zp rho_xp, RW We
+B_STEP_RECOMP = 60     # as forward without We
+B_STEP_ADJ = 84        # forward + RW uz ux
+B_SNAP = 20            # W 5 snapshot fields on a snapshot step (written by k_step or k_velocity)
+B_GRAD = 44            # R 5 snapshot fields, RW 3 gradient sums on a snapshot step
+# Two-kernel step (k_stress + k_velocity):
 B_STRESS = 44          # R vz vx, RW szz szx sxx, R lam mu mu_zx
 B_VEL_FWD = 44         # R szz szx sxx, RW vz vx, R rho_zp rho_xp, RW We
 B_VEL_RECOMP = 36      # as forward without We
-B_VEL_ADJ = 60 + 52    # + RW uz ux; gradient: R 5 snapshot fields, RW 3 sums, R lam mu
-B_STORE = 40           # R 5 fields, W 5 snapshot fields
+B_VEL_ADJ = 60         # + RW uz ux
+B_VEL_GRAD = 52        # gradient: R 5 snapshot fields, RW 3 sums, R lam mu
+# CPML memory: 4 psi RW per strip cell and direction per step (half in each kernel of the two-kernel step)
+B_PML = 32
 
 
 def log(*a):
@@ -210,26 +222,36 @@ def main():
     value = 2.0 * cells * nt * nshot * K / wall / 1e9
     shots_per_s = nshot * K / wall
 
-    # executed algorithmic bytes of the whole step (the survey's byte model)
+    # executed algorithmic bytes of the whole gradient over the full grid: forward 68,
+    # adjoint 84, gradient 44 and snapshot store 20 per snapshot step, recomputed
+    # step 60 when checkpointed, CPML 32 per strip cell and direction; checkpoint
+    # copies (<0.3 % at C3) are left out
     ckpt = plan["storage"] == "checkpoint"
+    sdt = int(c["snap"][4])
     zs = int(np.count_nonzero((c["a_z"] != 0) | (c["a_half_z"] != 0))) if c["pml_z"] else 0
     xs = int(np.count_nonzero((c["a_x"] != 0) | (c["a_half_x"] != 0))) if c["pml_x"] else 0
     pml_cells = zs * nx + xs * nz
-    per_shot_step = cells * (68 + 84 + 44 + (60 if ckpt else 20)) + pml_cells * 32 * (3 if ckpt else 2)
-    hbm_frac = per_shot_step * nt * nshot * K / wall / HBM_PEAK
+    model_bytes = (cells * (B_STEP_FWD + B_STEP_ADJ + (B_GRAD + B_SNAP) / sdt + (B_STEP_RECOMP if ckpt else 0))
+                   + pml_cells * B_PML * (3 if ckpt else 2))
+    hbm_frac = model_bytes * nt * nshot * K / wall / HBM_PEAK
 
-    # roofline of the dominant kernel from its live per-launch times
-    batches = plan["batch"]
-    launches_v, ms_v = kt["velocity"]
-    launches_s, ms_s = kt["stress"]
-    # every launch advances one batch; bytes summed over the launch modes the step runs
+    # roofline of the dominant kernel from its live per-launch times.  Every launch
+    # advances one batch by one step; the bytes are summed over the launch modes the
+    # gradient runs (forward, recomputation when checkpointed, adjoint) and divided
+    # by the kernel's summed launch time.
     shot_steps = nshot * nt * K
-    bytes_stress = B_STRESS * interior * shot_steps * (3 if ckpt else 2)
-    bytes_vel = interior * shot_steps * (B_VEL_FWD + B_VEL_ADJ + (B_VEL_RECOMP if ckpt else 0))
-    kern = {
-        "stress": (bytes_stress, ms_s, launches_s),
-        "velocity": (bytes_vel, ms_v, launches_v),
+    passes = 3 if ckpt else 2
+    zs_i = int(np.count_nonzero(((c["a_z"] != 0) | (c["a_half_z"] != 0))[N:nz - N])) if c["pml_z"] else 0
+    xs_i = int(np.count_nonzero(((c["a_x"] != 0) | (c["a_half_x"] != 0))[N:nx - N])) if c["pml_x"] else 0
+    strip = zs_i * (nx - 2 * N) + xs_i * (nz - 2 * N)   # interior cells inside a CPML strip (per direction)
+    per_shot_step = {
+        "step": interior * (B_STEP_FWD + B_STEP_ADJ + B_GRAD / sdt + B_SNAP / sdt
+                            + (B_STEP_RECOMP if ckpt else 0)) + strip * B_PML * passes,
+        "stress": interior * B_STRESS * passes + strip * B_PML / 2 * passes,
+        "velocity": interior * (B_VEL_FWD + B_VEL_ADJ + (B_VEL_GRAD + B_SNAP) / sdt
+                                + (B_VEL_RECOMP if ckpt else 0)) + strip * B_PML / 2 * passes,
     }
+    kern = {k: (per_shot_step[k] * shot_steps, kt[k][1], kt[k][0]) for k in per_shot_step if kt[k][0] > 0}
     dom = max(kern, key=lambda k: kern[k][1])
     b, ms, n = kern[dom]
     achieved = b / (ms / 1e3) / 1e9
@@ -269,7 +291,7 @@ def main():
             cpu = {"error": str(e)}
 
     out = {
-        "metric": "fwd+adj gradient Gcell-updates/s (batched, checkpointed)",
+        "metric": METRIC,
         "value": round(value, 3),
         "unit": "Gcell-updates/s",
         "n_gpus": 1,

@@ -70,6 +70,7 @@ struct fwi_ctx {
     double *tmp = nullptr, *red = nullptr;  // precondition scratch, reduction scratch
     int *zmap = nullptr, *xmap = nullptr, *nonfinite = nullptr;
     float *pf[12] = {};   // b a 1/K  b_half a_half 1/K_half, z then x
+    CpmlPoint *zp = nullptr, *xp = nullptr;   // the same, packed per grid line
     // acquisition
     double *stf_z = nullptr, *stf_x = nullptr;
     int *sg_shot = nullptr, *sg_cell = nullptr, *sg_off = nullptr, *sg_idx = nullptr;   // source groups
@@ -86,8 +87,13 @@ struct fwi_ctx {
     float *gr[3] = {};     // [B][snz][snx]
     float *snap[5] = {};   // FULL: [B][snt][snz][snx]; CHECKPOINT: [B][S/snap_dt][snz][snx]
     size_t snap_stride = 0;
+    // one-pass step: second buffer set of the 5 fields and the stress-side psi
+    // (pvz_z pvx_z pvx_x pvz_x); f / psi always name the current time level
+    bool fusable = false;               // k_step supports this grid, surface and acquisition
+    float *f2[5] = {}, *psi2[4] = {};
     // checkpoint mode
     float *rf[5] = {}, *rpsi[8] = {};   // recompute state
+    float *rf2[5] = {}, *rpsi2[4] = {};
     float *ck = nullptr;                // [nseg][13 arrays of the batch]
     size_t ck_bytes_per_seg = 0;
     int last_s0 = -1, last_nb = 0;
@@ -95,8 +101,9 @@ struct fwi_ctx {
     bool plain = false;
     PlainBufs pb{};
     bool tuned = false;
-    int var = 2;   // stencil kernel variant (autotuned)
+    int var = 2;   // stencil kernel variant (autotuned); 4 = one-pass step
     int order = 0; // stencil work order (autotuned)
+    int depth = 1; // rows the one-pass step loads ahead (autotuned)
     std::string tune_log;
 
     // ---- profiling
@@ -221,9 +228,45 @@ int upload(fwi_ctx *c, T *dst, const T *src, size_t n) {
 }
 
 // ---------------------------------------------------------------------------
+// propagation state
+// ---------------------------------------------------------------------------
+// The current time level (f[0..4], psi[0..7]) and, for the one-pass step, the
+// buffers the next level is written to (f2, psi2 = pvz_z pvx_z pvx_x pvz_x);
+// flip() swaps them in the owner's arrays, so f and psi always name the
+// current level.
+struct Wave {
+    float **f, **psi, **f2, **psi2;
+    void flip() const {
+        for (int i = 0; i < 5; ++i) std::swap(f[i], f2[i]);
+        std::swap(psi[0], psi2[0]);
+        std::swap(psi[1], psi2[1]);
+        std::swap(psi[4], psi2[2]);
+        std::swap(psi[5], psi2[3]);
+    }
+};
+Wave main_wave(fwi_ctx *c) { return {c->f, c->psi, c->f2, c->psi2}; }
+Wave recompute_wave(fwi_ctx *c) { return {c->rf, c->rpsi, c->rf2, c->rpsi2}; }
+
+// ---------------------------------------------------------------------------
 // kernel arguments
 // ---------------------------------------------------------------------------
-KArgs base_args(fwi_ctx *c, int nb, float *const fld[5], float *const ps[8]) {
+void set_state(KArgs &a, const Wave &w) {
+    a.vz = w.f[0];
+    a.vx = w.f[1];
+    a.szz = w.f[2];
+    a.szx = w.f[3];
+    a.sxx = w.f[4];
+    a.pvz_z = w.psi[0]; a.pvx_z = w.psi[1]; a.pszz_z = w.psi[2]; a.pszx_z = w.psi[3];
+    a.pvx_x = w.psi[4]; a.pvz_x = w.psi[5]; a.psxx_x = w.psi[6]; a.pszx_x = w.psi[7];
+    a.o_vz = w.f2[0];
+    a.o_vx = w.f2[1];
+    a.o_szz = w.f2[2];
+    a.o_szx = w.f2[3];
+    a.o_sxx = w.f2[4];
+    a.o_pvz_z = w.psi2[0]; a.o_pvx_z = w.psi2[1]; a.o_pvx_x = w.psi2[2]; a.o_pvz_x = w.psi2[3];
+}
+
+KArgs base_args(fwi_ctx *c, int nb, const Wave &w) {
     KArgs a{};
     a.nz = c->g.nz;
     a.nx = c->g.nx;
@@ -235,11 +278,6 @@ KArgs base_args(fwi_ctx *c, int nb, float *const fld[5], float *const ps[8]) {
         a.cz[k] = k < c->N ? (float)(c->g.hc[k + 1] / c->g.dz) : 0.f;
         a.cx[k] = k < c->N ? (float)(c->g.hc[k + 1] / c->g.dx) : 0.f;
     }
-    a.vz = fld[0];
-    a.vx = fld[1];
-    a.szz = fld[2];
-    a.szx = fld[3];
-    a.sxx = fld[4];
     a.uz = c->f[5];
     a.ux = c->f[6];
     a.We = c->f[7];
@@ -256,8 +294,10 @@ KArgs base_args(fwi_ctx *c, int nb, float *const fld[5], float *const ps[8]) {
     a.bhz = c->pf[3]; a.ahz = c->pf[4]; a.ikhz = c->pf[5];
     a.bx = c->pf[6]; a.ax = c->pf[7]; a.ikx = c->pf[8];
     a.bhx = c->pf[9]; a.ahx = c->pf[10]; a.ikhx = c->pf[11];
-    a.pvz_z = ps[0]; a.pvx_z = ps[1]; a.pszz_z = ps[2]; a.pszx_z = ps[3];
-    a.pvx_x = ps[4]; a.pvz_x = ps[5]; a.psxx_x = ps[6]; a.pszx_x = ps[7];
+    a.zp = c->zp;
+    a.xp = c->xp;
+    set_state(a, w);
+    a.surf_top = c->g.isurf[0] > 0 ? c->g.isurf[0] : -1;   // used by k_step only (fusable: row N)
     a.snap_z1 = c->g.snap_z1;
     a.snap_z2 = c->g.snap_z2;
     a.snap_x1 = c->g.snap_x1;
@@ -358,19 +398,49 @@ void half_steps(fwi_ctx *c, const KArgs &a) {
     }
 }
 
-// one forward-structured time step on the given state
-void fwd_step(fwi_ctx *c, KArgs a, const SrcArgs &sa, int it) {
+// One time step of state w (stress, free surface, velocity, then the sources of
+// time index it).  With dst, the step's state is stored into the snapshot slab
+// dst[5] (stride snap_stride between shots) as gard_fwd_storage2 stores it after
+// the injection.  a follows w: on return it names the new current level.
+void time_step(fwi_ctx *c, KArgs &a, const Wave &w, const SrcArgs &sa, int it, float *const *dst) {
+    set_state(a, w);
+    a.store = 0;
+    if (dst && !c->plain) {   // the velocity (or one-pass) kernel writes the snapshot
+        a.store = 1;
+        a.st_vz = dst[0]; a.st_vx = dst[1]; a.st_szz = dst[2]; a.st_szx = dst[3]; a.st_sxx = dst[4];
+        a.st_stride = c->snap_stride;
+    }
+    if (c->var == 4 && !c->plain) {
+        {
+            Timer t(c, FWI_K_STEP);
+            launch_step(a, c->N, c->tz, c->depth, c->st);
+        }
+        w.flip();
+        set_state(a, w);
+        Timer t(c, FWI_K_SOURCE);
+        launch_inject(a, sa, it, c->st);   // also rewrites the stored velocities of source cells
+        return;
+    }
     half_steps(c, a);
     {
         Timer t(c, FWI_K_SOURCE);
-        launch_inject(a, sa, it, c->st);
+        launch_inject(a, sa, it, c->st);   // with a.store, also rewrites the stored source cells
+    }
+    if (dst && c->plain) {   // the reference-structured form keeps its separate storage pass
+        Timer t(c, FWI_K_STORE);
+        launch_store(a, dst, c->snap_stride, (float)c->g.dt, c->st);
     }
 }
 
-int zero_state(fwi_ctx *c, int nb, float *const fld[5], float *const ps[8], bool u_we) {
-    for (int i = 0; i < 5; ++i) HIPCHK(hipMemsetAsync(fld[i], 0, sizeof(float) * nb * c->fs(), c->st));
-    for (int i = 0; i < 4; ++i) HIPCHK(hipMemsetAsync(ps[i], 0, sizeof(float) * nb * c->psiz(), c->st));
-    for (int i = 4; i < 8; ++i) HIPCHK(hipMemsetAsync(ps[i], 0, sizeof(float) * nb * c->psix(), c->st));
+int zero_state(fwi_ctx *c, int nb, const Wave &w, bool u_we) {
+    for (int i = 0; i < 5; ++i) HIPCHK(hipMemsetAsync(w.f[i], 0, sizeof(float) * nb * c->fs(), c->st));
+    for (int i = 0; i < 4; ++i) HIPCHK(hipMemsetAsync(w.psi[i], 0, sizeof(float) * nb * c->psiz(), c->st));
+    for (int i = 4; i < 8; ++i) HIPCHK(hipMemsetAsync(w.psi[i], 0, sizeof(float) * nb * c->psix(), c->st));
+    if (c->fusable) {   // the one-pass step never writes cells outside the interior: keep them zero
+        for (int i = 0; i < 5; ++i) HIPCHK(hipMemsetAsync(w.f2[i], 0, sizeof(float) * nb * c->fs(), c->st));
+        for (int i = 0; i < 2; ++i) HIPCHK(hipMemsetAsync(w.psi2[i], 0, sizeof(float) * nb * c->psiz(), c->st));
+        for (int i = 2; i < 4; ++i) HIPCHK(hipMemsetAsync(w.psi2[i], 0, sizeof(float) * nb * c->psix(), c->st));
+    }
     if (c->plain) {
         float *m[8] = {c->pb.mvz_z, c->pb.mvx_z, c->pb.mszz_z, c->pb.mszx_z,
                        c->pb.mvz_x, c->pb.mvx_x, c->pb.mszx_x, c->pb.msxx_x};
@@ -382,11 +452,11 @@ int zero_state(fwi_ctx *c, int nb, float *const fld[5], float *const ps[8], bool
 }
 
 // copy the 13-array propagation state of nb shots to / from checkpoint k
-int ckpt_copy(fwi_ctx *c, int k, int nb, float *const fld[5], float *const ps[8], bool save) {
+int ckpt_copy(fwi_ctx *c, int k, int nb, const Wave &w, bool save) {
     char *base = reinterpret_cast<char *>(c->ck) + (size_t)k * c->ck_bytes_per_seg;
     size_t off = 0;
     for (int i = 0; i < 13; ++i) {
-        float *p = i < 5 ? fld[i] : ps[i - 5];
+        float *p = i < 5 ? w.f[i] : w.psi[i - 5];
         const size_t per = i < 5 ? c->fs() : (i < 9 ? c->psiz() : c->psix());
         const size_t nbytes = sizeof(float) * per * nb;
         void *dst = save ? (void *)(base + off) : (void *)p;
@@ -419,83 +489,92 @@ struct TuneKey {
     }
 };
 struct TuneVal {
-    int var, tz, order;
+    int var, tz, depth, order;
 };
 std::map<TuneKey, TuneVal> g_tuned;
 std::mutex g_tune_mu;
 
+void take(fwi_ctx *c, const TuneVal &v) {
+    c->var = v.var;
+    c->tz = v.tz;
+    c->depth = v.depth;
+    c->order = v.order;
+    c->tuned = true;
+}
+
 int autotune(fwi_ctx *c, int nb) {
     if (c->plain || c->tuned) return FWI_OK;
     const TuneKey key{c->g.nz, c->g.nx, c->N, nb};
-    const bool forced = c->o.kernel_variant >= 1 && c->o.kernel_variant <= 3;
+    const int fv = c->o.kernel_variant;   // 0 = any
     const bool order_forced = c->o.work_order >= 1 && c->o.work_order <= 3;
-    if (forced && order_forced && c->o.tile_tz > 0) {   // nothing left to choose
-        c->var = c->o.kernel_variant;
-        c->tz = c->o.tile_tz;
-        c->order = c->o.work_order - 1;
-        c->tuned = true;
+    // candidates (variant, rows per segment, prefetch depth); the one-pass step only where it applies
+    std::vector<TuneVal> cand;
+    for (int var = 1; var <= 4; ++var) {
+        if ((fv && var != fv) || (var == 4 && !c->fusable)) continue;
+        const std::vector<int> tzs = var == 4 ? std::vector<int>{32, 64} : std::vector<int>{8, 16, 32, 64};
+        for (int tz : tzs)
+            if (c->o.tile_tz <= 0 || c->o.tile_tz == tz) cand.push_back({var, tz, 1, 0});
+    }
+    if (cand.empty())
+        return c->fail(FWI_EINVAL, "no stencil kernel for kernel_variant %d, tile_tz %d%s", fv, c->o.tile_tz,
+                       fv == 4 ? " (the one-pass step takes tile_tz 32 or 64, no free surface other than "
+                                 "the top one at row fdorder/2, and sources and receivers inside the interior)"
+                               : "");
+    if (cand.size() == 1 && order_forced) {   // nothing left to choose
+        take(c, {cand[0].var, cand[0].tz, cand[0].depth, c->o.work_order - 1});
         return FWI_OK;
     }
     {
         std::lock_guard<std::mutex> lk(g_tune_mu);
         auto it = g_tuned.find(key);
-        if (it != g_tuned.end() && (!forced || c->o.kernel_variant == it->second.var) &&
+        if (it != g_tuned.end() && (!fv || fv == it->second.var) && (it->second.var != 4 || c->fusable) &&
             (!order_forced || c->o.work_order - 1 == it->second.order) &&
             (c->o.tile_tz <= 0 || c->o.tile_tz == it->second.tz)) {
-            c->var = it->second.var;
-            c->tz = it->second.tz;
-            c->order = it->second.order;
-            c->tuned = true;
+            take(c, it->second);
             return FWI_OK;
         }
     }
-    float *fld[5] = {c->f[0], c->f[1], c->f[2], c->f[3], c->f[4]};
-    RETIF(zero_state(c, nb, fld, c->psi, true));
-    KArgs a = base_args(c, nb, fld, c->psi);
+    const Wave w = main_wave(c);
+    RETIF(zero_state(c, nb, w, true));
+    KArgs a = base_args(c, nb, w);   // the one-pass step reads f and writes f2 on every repetition
     a.energy = 1;
     hipEvent_t e0, e1;
     HIPCHK(hipEventCreate(&e0));
     HIPCHK(hipEventCreate(&e1));
-    std::vector<int> tzs = {8, 16, 32, 64};
-    if (c->o.tile_tz > 0) tzs = {c->o.tile_tz};
-    std::vector<int> vars = {1, 2, 3};
-    if (forced) vars = {c->o.kernel_variant};
+    auto run = [&](const TuneVal &v) {
+        if (v.var == 4) {
+            launch_step(a, c->N, v.tz, v.depth, c->st);
+        } else {
+            launch_stress(a, c->N, v.tz, v.var, c->st);
+            launch_velocity(a, c->N, v.tz, v.var, c->st);
+        }
+    };
     float best = 1e30f;
     const int o_lo = order_forced ? c->o.work_order - 1 : 0, o_hi = order_forced ? o_lo + 1 : 3;
-    TuneVal bv{vars[0], tzs[0], o_lo};
+    TuneVal bv = cand[0];
+    bv.order = o_lo;
     for (int order = o_lo; order < o_hi; ++order) {
         a.order = order;
-        for (int var : vars) {
-            for (int tz : tzs) {
-                for (int w = 0; w < 2; ++w) {
-                    launch_stress(a, c->N, tz, var, c->st);
-                    launch_velocity(a, c->N, tz, var, c->st);
-                }
-                HIPCHK(hipEventRecord(e0, c->st));
-                for (int w = 0; w < 5; ++w) {
-                    launch_stress(a, c->N, tz, var, c->st);
-                    launch_velocity(a, c->N, tz, var, c->st);
-                }
-                HIPCHK(hipEventRecord(e1, c->st));
-                HIPCHK(hipEventSynchronize(e1));
-                float ms = 0;
-                HIPCHK(hipEventElapsedTime(&ms, e0, e1));
-                char buf[64];
-                std::snprintf(buf, sizeof buf, "o%d/v%d/tz%d %.3f; ", order, var, tz, ms / 5);
-                c->tune_log += buf;
-                if (ms < best) {
-                    best = ms;
-                    bv = {var, tz, order};
-                }
+        for (const auto &vt : cand) {
+            for (int rep = 0; rep < 2; ++rep) run(vt);
+            HIPCHK(hipEventRecord(e0, c->st));
+            for (int rep = 0; rep < 5; ++rep) run(vt);
+            HIPCHK(hipEventRecord(e1, c->st));
+            HIPCHK(hipEventSynchronize(e1));
+            float ms = 0;
+            HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+            char buf[64];
+            std::snprintf(buf, sizeof buf, "o%d/v%d/tz%d/d%d %.3f; ", order, vt.var, vt.tz, vt.depth, ms / 5);
+            c->tune_log += buf;
+            if (ms < best) {
+                best = ms;
+                bv = {vt.var, vt.tz, vt.depth, order};
             }
         }
     }
     hipEventDestroy(e0);
     hipEventDestroy(e1);
-    c->var = bv.var;
-    c->tz = bv.tz;
-    c->order = bv.order;
-    c->tuned = true;
+    take(c, bv);
     std::lock_guard<std::mutex> lk(g_tune_mu);
     g_tuned[key] = bv;
     return FWI_OK;
@@ -503,11 +582,11 @@ int autotune(fwi_ctx *c, int nb) {
 
 // Forward pass (kernel_PSV with grad = false) for shots [s0, s0 + nb).
 int run_forward(fwi_ctx *c, int s0, int nb, bool store, bool record, bool energy) {
-    float *fld[5] = {c->f[0], c->f[1], c->f[2], c->f[3], c->f[4]};
     RETIF(autotune(c, nb));
-    RETIF(zero_state(c, nb, fld, c->psi, true));
+    const Wave w = main_wave(c);
+    RETIF(zero_state(c, nb, w, true));
     if (record) HIPCHK(hipMemsetAsync(c->acc, 0, sizeof(double) * 2 * c->nrec_tot, c->st));
-    KArgs a = base_args(c, nb, fld, c->psi);
+    KArgs a = base_args(c, nb, w);
     a.energy = energy;
     const SrcArgs sa = src_args(c, s0, nb);
     const RecArgs ra = rec_args(c, s0, nb);
@@ -515,19 +594,15 @@ int run_forward(fwi_ctx *c, int s0, int nb, bool store, bool record, bool energy
     for (int it = 0; it < c->g.nt; ++it) {
         if (ckpt && it % c->S == 0) {
             Timer t(c, FWI_K_STORE);
-            RETIF(ckpt_copy(c, it / c->S, nb, fld, c->psi, true));
+            RETIF(ckpt_copy(c, it / c->S, nb, w, true));
         }
-        fwd_step(c, a, sa, it);
+        const size_t off = (size_t)(it / c->g.snap_dt) * c->snz * c->snx;
+        float *dst[5] = {c->snap[0] + off, c->snap[1] + off, c->snap[2] + off, c->snap[3] + off, c->snap[4] + off};
+        const bool snap_step = store && !ckpt && it % c->g.snap_dt == 0;
+        time_step(c, a, w, sa, it, snap_step ? dst : nullptr);
         if (record) {
             Timer t(c, FWI_K_RECEIVER);
             launch_record(a, ra, it, c->st);
-        }
-        if (store && !ckpt && it % c->g.snap_dt == 0) {
-            Timer t(c, FWI_K_STORE);
-            const size_t off = (size_t)(it / c->g.snap_dt) * c->snz * c->snx;
-            float *dst[5] = {c->snap[0] + off, c->snap[1] + off, c->snap[2] + off, c->snap[3] + off,
-                             c->snap[4] + off};
-            launch_store(a, dst, c->snap_stride, (float)c->g.dt, c->st);
         }
     }
     c->last_s0 = s0;
@@ -538,7 +613,7 @@ int run_forward(fwi_ctx *c, int s0, int nb, bool store, bool record, bool energy
 // One adjoint step (time index it); on snapshot steps the velocity kernel
 // correlates with the forward slab at slab_off (fwi_grad2 fused), or the plain
 // form runs fwi_grad2 as its own pass after the injection, as the reference does.
-void adj_step(fwi_ctx *c, KArgs a, const SrcArgs &sa, int it, size_t slab_off) {
+void adj_step(fwi_ctx *c, KArgs &a, const Wave &w, const SrcArgs &sa, int it, size_t slab_off) {
     const bool gstep = it % c->g.snap_dt == 0;
     if (gstep) {
         a.sn_vz = c->snap[0] + slab_off;
@@ -549,11 +624,7 @@ void adj_step(fwi_ctx *c, KArgs a, const SrcArgs &sa, int it, size_t slab_off) {
         a.sn_stride = c->snap_stride;
     }
     a.grad = gstep && !c->plain;
-    half_steps(c, a);
-    {
-        Timer t(c, FWI_K_SOURCE);
-        launch_inject(a, sa, it, c->st);
-    }
+    time_step(c, a, w, sa, it, nullptr);
     if (gstep && c->plain) {
         Timer t(c, FWI_K_OTHER);
         plain_grad(a, c->st);
@@ -563,13 +634,13 @@ void adj_step(fwi_ctx *c, KArgs a, const SrcArgs &sa, int it, size_t slab_off) {
 // Adjoint pass (kernel_PSV with grad = true): residuals in res_* are injected at
 // the receivers in reversed time, each shot's scaled by its alpha (a power of two).
 int run_adjoint(fwi_ctx *c, int s0, int nb) {
-    float *fld[5] = {c->f[0], c->f[1], c->f[2], c->f[3], c->f[4]};
-    RETIF(zero_state(c, nb, fld, c->psi, false));
+    const Wave w = main_wave(c);
+    RETIF(zero_state(c, nb, w, false));
     for (int i = 5; i < 7; ++i) HIPCHK(hipMemsetAsync(c->f[i], 0, sizeof(float) * nb * c->fs(), c->st));
     HIPCHK(hipMemsetAsync(c->f[8], 0, sizeof(float) * nb * c->fs(), c->st));
     for (int i = 0; i < 3; ++i)
         HIPCHK(hipMemsetAsync(c->gr[i], 0, sizeof(float) * nb * c->snz * c->snx, c->st));
-    KArgs a = base_args(c, nb, fld, c->psi);
+    KArgs a = base_args(c, nb, w);
     a.We = c->f[8];
     a.energy = 1;
     a.displ = 1;
@@ -579,28 +650,26 @@ int run_adjoint(fwi_ctx *c, int s0, int nb) {
     if (c->storage != FWI_STORE_CHECKPOINT) {
         for (int jt = 0; jt < nt; ++jt) {
             const int it = nt - 1 - jt;
-            adj_step(c, a, sa, it, (size_t)(it / sdt) * slab);
+            adj_step(c, a, w, sa, it, (size_t)(it / sdt) * slab);
         }
     } else {
-        KArgs r = base_args(c, nb, c->rf, c->rpsi);
+        const Wave rw = recompute_wave(c);
+        RETIF(zero_state(c, nb, rw, false));   // cells outside the interior stay zero in both levels
+        KArgs r = base_args(c, nb, rw);
         const SrcArgs fsa = src_args(c, s0, nb);
         for (int k = c->nseg - 1; k >= 0; --k) {
             const int t0 = k * c->S, t1 = std::min(t0 + c->S, nt);
             {
                 Timer t(c, FWI_K_STORE);
-                RETIF(ckpt_copy(c, k, nb, c->rf, c->rpsi, false));
+                RETIF(ckpt_copy(c, k, nb, rw, false));
             }
             for (int it = t0; it < t1; ++it) {
-                fwd_step(c, r, fsa, it);
-                if (it % sdt == 0) {
-                    Timer t(c, FWI_K_STORE);
-                    const size_t off = (size_t)((it - t0) / sdt) * slab;
-                    float *dst[5] = {c->snap[0] + off, c->snap[1] + off, c->snap[2] + off, c->snap[3] + off,
-                                     c->snap[4] + off};
-                    launch_store(r, dst, c->snap_stride, (float)c->g.dt, c->st);
-                }
+                const size_t off = (size_t)((it - t0) / sdt) * slab;
+                float *dst[5] = {c->snap[0] + off, c->snap[1] + off, c->snap[2] + off, c->snap[3] + off,
+                                 c->snap[4] + off};
+                time_step(c, r, rw, fsa, it, it % sdt == 0 ? dst : nullptr);
             }
-            for (int it = t1 - 1; it >= t0; --it) adj_step(c, a, sa, it, (size_t)((it - t0) / sdt) * slab);
+            for (int it = t1 - 1; it >= t0; --it) adj_step(c, a, w, sa, it, (size_t)((it - t0) / sdt) * slab);
         }
     }
     c->last_s0 = s0;
@@ -683,11 +752,14 @@ Need need_bytes(const fwi_ctx *c, int storage, int S) {
     const size_t state_ck = 5 * fs * 4 + 4 * (c->psiz() + c->psix()) * 4;
     n.per_shot = 9 * fs * 4 + 8 * (c->psiz() + c->psix()) * 4 + 3 * snapcell * 4;
     if (c->plain) n.per_shot += 12 * fs * 4;
+    // second time level of the one-pass step (5 fields, 4 stress-side psi)
+    const size_t level2 = c->fusable ? (5 * fs + 2 * (c->psiz() + c->psix())) * 4 : 0;
+    n.per_shot += level2;
     if (storage == FWI_STORE_FULL) {
         n.per_shot += 5 * (size_t)c->snt * snapcell * 4;
     } else {
         const size_t nseg = (nt + S - 1) / S;
-        n.per_shot += state_ck + nseg * state_ck + 5 * (size_t)(S / c->g.snap_dt) * snapcell * 4;
+        n.per_shot += state_ck + level2 + nseg * state_ck + 5 * (size_t)(S / c->g.snap_dt) * snapcell * 4;
     }
     return n;
 }
@@ -762,6 +834,9 @@ int allocate(fwi_ctx *c) {
     c->xmap = I(nx);
     for (int i = 0; i < 6; ++i) c->pf[i] = F(nz);
     for (int i = 6; i < 12; ++i) c->pf[i] = F(nx);
+    c->zp = c->dalloc<CpmlPoint>(nz);
+    c->xp = c->dalloc<CpmlPoint>(nx);
+    ok = ok && c->zp && c->xp;
     const size_t recrows = (size_t)c->nrec_tot * nt;
     c->syn_z = D(recrows); c->syn_x = D(recrows);
     c->obs_z = D(recrows); c->obs_x = D(recrows);
@@ -775,6 +850,12 @@ int allocate(fwi_ctx *c) {
     for (int i = 0; i < 4; ++i) c->psi[i] = F(B * c->psiz());
     for (int i = 4; i < 8; ++i) c->psi[i] = F(B * c->psix());
     for (int i = 0; i < 3; ++i) c->gr[i] = F(B * snapcell);
+    auto level2 = [&](float **f2, float **psi2) {
+        for (int i = 0; i < 5; ++i) f2[i] = F(B * fs);
+        for (int i = 0; i < 2; ++i) psi2[i] = F(B * c->psiz());
+        for (int i = 2; i < 4; ++i) psi2[i] = F(B * c->psix());
+    };
+    if (c->fusable) level2(c->f2, c->psi2);
     if (c->storage == FWI_STORE_FULL) {
         c->snap_stride = (size_t)c->snt * snapcell;
     } else {
@@ -782,6 +863,7 @@ int allocate(fwi_ctx *c) {
         for (int i = 0; i < 5; ++i) c->rf[i] = F(B * fs);
         for (int i = 0; i < 4; ++i) c->rpsi[i] = F(B * c->psiz());
         for (int i = 4; i < 8; ++i) c->rpsi[i] = F(B * c->psix());
+        if (c->fusable) level2(c->rf2, c->rpsi2);
         c->ck_bytes_per_seg = sizeof(float) * B * (5 * fs + 4 * (c->psiz() + c->psix()));
         c->ck = F(c->ck_bytes_per_seg / sizeof(float) * c->nseg);
     }
@@ -848,8 +930,18 @@ int setup_device(fwi_ctx *c, const fwi_acq *q) {
     if (c->g.pml_x)
         for (int i = N; i < nx - N; ++i)
             if (active(c, 6, i)) xmap[i] = c->nxs++;
+    // The one-pass step handles a free surface only at the top row N (no image rows),
+    // and injects only into interior cells (it never writes the others).
+    auto interior = [&](int z, int x) { return z >= N && z < nz - N && x >= N && x < nx - N; };
+    c->fusable = !c->plain && c->g.isurf[1] == 0 && c->g.isurf[2] == 0 && c->g.isurf[3] == 0 &&
+                 (c->g.isurf[0] == 0 || c->g.isurf[0] == N);
+    for (int i = 0; c->fusable && i < c->nsrc_tot; ++i) c->fusable = interior(q->src_z[i], q->src_x[i]);
+    for (int i = 0; c->fusable && i < c->nrec_tot; ++i) c->fusable = interior(q->rec_z[i], q->rec_x[i]);
     RETIF(plan(c));
     RETIF(allocate(c));
+    // snapshot cells outside the interior are never written by the one-pass step: zero once
+    for (int i = 0; i < 5; ++i)
+        HIPCHK(hipMemsetAsync(c->snap[i], 0, sizeof(float) * c->B * c->snap_stride, c->st));
     RETIF(upload(c, c->zmap, zmap.data(), nz));
     RETIF(upload(c, c->xmap, xmap.data(), nx));
     // profiles as b, a, 1/K (integer then half), z then x
@@ -872,6 +964,13 @@ int setup_device(fwi_ctx *c, const fwi_acq *q) {
     }
     for (int k = 0; k < 6; ++k) RETIF(upload(c, c->pf[k], hz.data() + k * nz, nz));
     for (int k = 0; k < 6; ++k) RETIF(upload(c, c->pf[6 + k], hx.data() + k * nx, nx));
+    std::vector<CpmlPoint> pz(nz), px(nx);
+    for (int i = 0; i < nz; ++i)
+        pz[i] = {hz[i], hz[nz + i], hz[2 * nz + i], hz[3 * nz + i], hz[4 * nz + i], hz[5 * nz + i], zmap[i], 0};
+    for (int i = 0; i < nx; ++i)
+        px[i] = {hx[i], hx[nx + i], hx[2 * nx + i], hx[3 * nx + i], hx[4 * nx + i], hx[5 * nx + i], xmap[i], 0};
+    RETIF(upload(c, c->zp, pz.data(), nz));
+    RETIF(upload(c, c->xp, px.data(), nx));
     // sources
     std::vector<int> gs, gc, go, gi;
     build_groups(c->src_off, q->src_z, q->src_x, c->P, gs, gc, go, gi, c->sg_first);
@@ -921,8 +1020,8 @@ int validate(fwi_ctx *c, const fwi_grid *g, const fwi_acq *q) {
     const int tt = c->o.tile_tz;
     if (tt != 0 && tt != 8 && tt != 16 && tt != 32 && tt != 64)
         return c->fail(FWI_EINVAL, "tile_tz %d (0, 8, 16, 32 or 64)", tt);
-    if (c->o.kernel_variant < 0 || c->o.kernel_variant > 3)
-        return c->fail(FWI_EINVAL, "kernel_variant %d (0..3)", c->o.kernel_variant);
+    if (c->o.kernel_variant < 0 || c->o.kernel_variant > 4)
+        return c->fail(FWI_EINVAL, "kernel_variant %d (0..4)", c->o.kernel_variant);
     if (c->o.work_order < 0 || c->o.work_order > 3)
         return c->fail(FWI_EINVAL, "work_order %d (0..3)", c->o.work_order);
     if (g->snap_dt < 1 || g->snap_dz < 1 || g->snap_dx < 1) return c->fail(FWI_EINVAL, "snap strides < 1");
@@ -1394,7 +1493,10 @@ int32_t fwi_tuning(const fwi_ctx *c, int32_t *tile_tz, char *log, int32_t log_le
     if (!c) return FWI_EINVAL;
     if (tile_tz) *tile_tz = c->tz;
     if (log && log_len > 0) {
-        std::strncpy(log, c->tune_log.c_str(), (size_t)log_len - 1);
+        char head[64];
+        std::snprintf(head, sizeof head, "chosen o%d/v%d/tz%d/d%d; ", c->order, c->var, c->tz, c->depth);
+        const std::string s = (c->tuned ? std::string(head) : std::string()) + c->tune_log;
+        std::strncpy(log, s.c_str(), (size_t)log_len - 1);
         log[log_len - 1] = 0;
     }
     return FWI_OK;

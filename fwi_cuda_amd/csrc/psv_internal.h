@@ -26,6 +26,14 @@ constexpr int kPitchAlign = 64;   // floats: rows start on 256-byte boundaries
 constexpr int kBX = 64;           // stencil block: one wavefront across x ...
 constexpr int kBY = 4;            // ... times four z-segments
 
+// CPML profile of one grid line, packed so one pointer serves the one-pass step:
+// b, a, 1/K at the integer point and at the half point, and the strip index
+// (row of psi_z / column of psi_x, -1 outside the strips).
+struct CpmlPoint {
+    float b, a, ik, bh, ah, ikh;
+    int map, pad;
+};
+
 // Everything a stencil / injection kernel reads or writes, passed by value.
 struct KArgs {
     int nz, nx, P, B;
@@ -42,6 +50,7 @@ struct KArgs {
     int nzs, nxs;
     const float *bz, *az, *ikz, *bhz, *ahz, *ikhz;
     const float *bx, *ax, *ikx, *bhx, *ahx, *ikhx;
+    const CpmlPoint *zp, *xp;                 // the same profiles and maps, packed [nz], [nx]
     float *pvz_z, *pvx_z, *pszz_z, *pszx_z;   // [B][nzs][P]
     float *pvx_x, *pvz_x, *psxx_x, *pszx_x;   // [B][nz][nxs]
     // velocity-kernel options
@@ -54,6 +63,15 @@ struct KArgs {
     size_t sn_stride;          // between shots
     float *g_lam, *g_mu, *g_rho;  // [B][snz][snx]
     int snap_z1, snap_z2, snap_x1, snap_x2, snap_dz, snap_dx, snz, snx;
+    // one-pass step (k_step): fields and stress-side psi of the next time level go
+    // to a second buffer set; the velocity-side psi is updated in place
+    float *o_vz, *o_vx, *o_szz, *o_szx, *o_sxx;
+    float *o_pvz_z, *o_pvx_z, *o_pvx_x, *o_pvz_x;
+    int surf_top;              // free-surface row handled inside k_step (== N), or -1
+    // snapshot of this step (gard_fwd_storage2) written by k_step and patched by k_inject
+    int store;
+    float *st_vz, *st_vx, *st_szz, *st_szx, *st_sxx;  // slab of shot 0
+    size_t st_stride;          // between shots
 };
 
 // Point injection / recording (vsrc2, urec2) for the batch.
@@ -85,6 +103,9 @@ struct RecArgs {
 // var 1: x taps as shifted global loads; var 2: x taps from neighbouring lanes, next row prefetched
 void launch_stress(const KArgs &a, int N, int tz, int var, hipStream_t st);
 void launch_velocity(const KArgs &a, int N, int tz, int var, hipStream_t st);
+// stress, top free surface and velocity of one time step in one pass (kernel variant 4);
+// tz = rows per segment (32, 64), depth = rows loaded ahead (1 is the one instantiated)
+void launch_step(const KArgs &a, int N, int tz, int depth, hipStream_t st);
 void launch_surface(const KArgs &a, int N, const int isurf[4], hipStream_t st);
 void launch_inject(const KArgs &a, const SrcArgs &s, int it, hipStream_t st);
 void launch_record(const KArgs &a, const RecArgs &r, int it, hipStream_t st);

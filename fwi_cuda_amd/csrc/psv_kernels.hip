@@ -27,6 +27,14 @@ __device__ __forceinline__ size_t fidx(const KArgs &a, int s, int z, int x) {
     return (size_t)s * a.fs + (size_t)z * a.P + x;
 }
 
+// column / row on the strided snapshot box
+__device__ __forceinline__ bool ix_on_lattice(const KArgs &a, int x) {
+    return x >= a.snap_x1 && x <= a.snap_x2 && (x - a.snap_x1) % a.snap_dx == 0;
+}
+__device__ __forceinline__ bool iz_on_lattice(const KArgs &a, int z) {
+    return z >= a.snap_z1 && z <= a.snap_z2 && (z - a.snap_z1) % a.snap_dz == 0;
+}
+
 // Workgroups are dispatched round-robin over the 8 XCDs (each with its own L2).
 // The 1-D launch index is remapped so that every XCD owns one contiguous range
 // of work, ordered tile-major and shot-minor: the B shots of a tile run back to
@@ -158,7 +166,8 @@ __global__ __launch_bounds__(kBX *kBY) void k_velocity(const KArgs a) {
         bx = a.bx[ix]; ax = a.ax[ix]; ikx = a.ikx[ix];
         bhx = a.bhx[ix]; ahx = a.ahx[ix]; ikhx = a.ikhx[ix];
     }
-    const bool gcol = a.grad && ix >= a.snap_x1 && ix <= a.snap_x2 && (ix - a.snap_x1) % a.snap_dx == 0;
+    // snapshot lattice column: gradient correlation (adjoint) or snapshot store (forward)
+    const bool gcol = (a.grad || a.store) && ix_on_lattice(a, ix);
     const int jx = gcol ? (ix - a.snap_x1) / a.snap_dx : 0;
 
     // qzz[j] = szz[z - N + 1 + j], qzx[j] = szx[z - N + j]
@@ -222,23 +231,33 @@ __global__ __launch_bounds__(kBX *kBY) void k_velocity(const KArgs a) {
         if (a.energy) a.We[fo + r] += vx1 * vx1 + vz1 * vz1;
         if (gcol && z >= a.snap_z1 && z <= a.snap_z2 && (z - a.snap_z1) % a.snap_dz == 0) {
             const int jz = (z - a.snap_z1) / a.snap_dz;
-            const size_t q = (size_t)s * a.snz * a.snx + (size_t)jz * a.snx + jx;
-            const size_t o = (size_t)s * a.sn_stride + (size_t)jz * a.snx + jx;
-            const float Szz = a.sn_szz[o], Sxx = a.sn_sxx[o], Szx = a.sn_szx[o];
-            const float Vz = a.sn_vz[o], Vx = a.sn_vx[o];
-            const float lam = a.lam[r], mu = a.mu[r];
             const float c_zz = qzz[N - 1], c_zx = qzx[N], c_xx = sxx[r];
-            const float lm = lam + mu;
-            // the 1/(lam+mu)^2 and 1/mu^2 factors (~1e-20) carry the gain so the
-            // products stay in the FP32 normal range; the host divides it out in FP64
-            const float il = a.gain_lm / (lm * lm), im = a.gain_lm / (mu * mu);
-            const float s1 = 0.25f * il * (Szz + Sxx) * (c_zz + c_xx);
-            const float s2 = 0.25f * im * (Szz - Sxx) * (c_zz - c_xx);
-            const float s3 = im * (Szx * c_zx);
-            const float s4 = ux * Vx + uz * Vz;
-            a.g_lam[q] += a.gw * s1;
-            a.g_mu[q] += a.gw * (s3 + s1 + s2);
-            a.g_rho[q] += a.gw * s4;
+            if (a.grad) {
+                const size_t q = (size_t)s * a.snz * a.snx + (size_t)jz * a.snx + jx;
+                const size_t o = (size_t)s * a.sn_stride + (size_t)jz * a.snx + jx;
+                const float Szz = a.sn_szz[o], Sxx = a.sn_sxx[o], Szx = a.sn_szx[o];
+                const float Vz = a.sn_vz[o], Vx = a.sn_vx[o];
+                const float lam = a.lam[r], mu = a.mu[r];
+                const float lm = lam + mu;
+                // the 1/(lam+mu)^2 and 1/mu^2 factors (~1e-20) carry the gain so the
+                // products stay in the FP32 normal range; the host divides it out in FP64
+                const float il = a.gain_lm / (lm * lm), im = a.gain_lm / (mu * mu);
+                const float s1 = 0.25f * il * (Szz + Sxx) * (c_zz + c_xx);
+                const float s2 = 0.25f * im * (Szz - Sxx) * (c_zz - c_xx);
+                const float s3 = im * (Szx * c_zx);
+                const float s4 = ux * Vx + uz * Vz;
+                a.g_lam[q] += a.gw * s1;
+                a.g_mu[q] += a.gw * (s3 + s1 + s2);
+                a.g_rho[q] += a.gw * s4;
+            }
+            if (a.store) {   // gard_fwd_storage2; k_inject rewrites source cells after injection
+                const size_t o = (size_t)s * a.st_stride + (size_t)jz * a.snx + jx;
+                a.st_vz[o] = vz1 / dt;
+                a.st_vx[o] = vx1 / dt;
+                a.st_szz[o] = c_zz;
+                a.st_szx[o] = c_zx;
+                a.st_sxx[o] = c_xx;
+            }
         }
     }
 }
@@ -386,7 +405,7 @@ __global__ __launch_bounds__(kBX *kBY, MINW) void k_velocity2(const KArgs a) {
         bx = a.bx[ix]; ax = a.ax[ix]; ikx = a.ikx[ix];
         bhx = a.bhx[ix]; ahx = a.ahx[ix]; ikhx = a.ikhx[ix];
     }
-    const bool gcol = out && a.grad && ix >= a.snap_x1 && ix <= a.snap_x2 && (ix - a.snap_x1) % a.snap_dx == 0;
+    const bool gcol = out && (a.grad || a.store) && ix_on_lattice(a, ix);
     const int jx = gcol ? (ix - a.snap_x1) / a.snap_dx : 0;
 
     // qzz[j] = szz[z - N + 1 + j], qzx[j] = szx[z - N + j]
@@ -467,12 +486,21 @@ __global__ __launch_bounds__(kBX *kBY, MINW) void k_velocity2(const KArgs a) {
         if (a.energy) a.We[fo + r] += vx1 * vx1 + vz1 * vz1;
         if (gcol && z >= a.snap_z1 && z <= a.snap_z2 && (z - a.snap_z1) % a.snap_dz == 0) {
             const int jz = (z - a.snap_z1) / a.snap_dz;
+            const float c_zz = qzz[N - 1], c_xx = sxxc;
+            if (a.store) {   // gard_fwd_storage2; k_inject rewrites source cells after injection
+                const size_t o = (size_t)s * a.st_stride + (size_t)jz * a.snx + jx;
+                a.st_vz[o] = vz1 / dt;
+                a.st_vx[o] = vx1 / dt;
+                a.st_szz[o] = c_zz;
+                a.st_szx[o] = szxc;
+                a.st_sxx[o] = c_xx;
+            }
+            if (!a.grad) continue;
             const size_t q = (size_t)s * a.snz * a.snx + (size_t)jz * a.snx + jx;
             const size_t o = (size_t)s * a.sn_stride + (size_t)jz * a.snx + jx;
             const float Szz = a.sn_szz[o], Sxx = a.sn_sxx[o], Szx = a.sn_szx[o];
             const float Vz = a.sn_vz[o], Vx = a.sn_vx[o];
             const float lam = a.lam[r], mu = a.mu[r];
-            const float c_zz = qzz[N - 1], c_xx = sxxc;
             const float lm = lam + mu;
             const float il = a.gain_lm / (lm * lm), im = a.gain_lm / (mu * mu);
             const float s1 = 0.25f * il * (Szz + Sxx) * (c_zz + c_xx);
@@ -484,6 +512,267 @@ __global__ __launch_bounds__(kBX *kBY, MINW) void k_velocity2(const KArgs a) {
             a.g_rho[q] += a.gw * s4;
         }
     }
+}
+
+// ---------------------------------------------------------------------------
+// Variant 4: the whole time step in one pass (k_step).
+//
+// The wavefront streams down its segment of rows [z0, z1).  At input row j it
+// computes the new stresses of row j (and applies the top free surface when
+// j is that row), pushes them into a stress queue, and then updates the
+// velocities of row z = j - N, whose z taps (szz rows z-N+1 .. z+N, szx rows
+// z-N .. z+N-1) are that queue.  The x taps of both half-steps are
+// neighbouring lanes: a wavefront loads 64 columns, computes stresses on the
+// inner 64 - 2N and writes velocities (and stresses) on the inner 64 - 4N.
+// Stresses the velocity taps need outside the owned block (N columns each
+// side, N rows above and below the segment) are recomputed, so every input is
+// read at the previous time level: fields and the stress-side CPML memory are
+// read from one buffer set and written to the other (the host swaps them each
+// step).  The velocity-side memory is only touched by the owning lane and is
+// updated in place.  Compulsory traffic per cell-step: R+W five fields, R five
+// media, R+W We (68 B), the survey's single-pass byte count.
+// ---------------------------------------------------------------------------
+// MODE bits (compile time, so each launch mode keeps only the pointers it uses):
+// 1 = accumulate We, 2 = adjoint (u += dt v, gradient on snapshot steps), 4 = store
+// snapshots; kModeGeneral reads the three flags from the arguments at run time.
+constexpr int kModeEnergy = 1, kModeAdjoint = 2, kModeStore = 4, kModeGeneral = 8;
+
+// D = prefetch depth: the loads of input row j + D are issued while row j is computed.
+template <int N, int TZ, int MODE, int D>
+__global__ __launch_bounds__(kBX *kBY) void k_step(const KArgs a) {
+    constexpr bool kGeneral = MODE == kModeGeneral;
+    const bool kEnergy = kGeneral ? a.energy != 0 : (MODE & kModeEnergy) != 0;
+    const bool kAdj = kGeneral ? a.displ != 0 : (MODE & kModeAdjoint) != 0;
+    const bool kStore = kGeneral ? a.store != 0 : (MODE & kModeStore) != 0;
+    constexpr int OUTW = kBX - 4 * N;
+    int tbx, tby, s;
+    tile_of<TZ, OUTW>(a, N, tbx, tby, s);
+    const int lane = threadIdx.x;
+    const int x = tbx * OUTW + lane - N;   // lane 2N holds the first owned column N + tbx * OUTW
+    const int z0 = N + (tby * kBY + threadIdx.y) * TZ;
+    const int nzi = a.nz - N;
+    if (z0 >= nzi) return;                 // uniform over the wavefront
+    const int z1 = min(z0 + TZ, nzi);
+    const int cx = min(max(x, 0), a.nx - 1);   // lanes off the grid read a valid column
+    // stresses are computed where the lane has its 2N x taps and the cell is interior;
+    // lanes [2N, 64 - 2N) own their column
+    const bool sint = lane >= N && lane < kBX - N && x >= N && x < a.nx - N;
+    const bool own = lane >= 2 * N && lane < kBX - 2 * N && x < a.nx - N;
+    const int P = a.P;
+    // Per-shot base pointers are wavefront-uniform; element offsets within one
+    // shot's arrays are 32-bit (the host keeps nz * P below 2^29).
+    const size_t fo = (size_t)s * a.fs;
+    const float *__restrict__ vz = a.vz + fo;
+    const float *__restrict__ vx = a.vx + fo;
+    const float *__restrict__ szz = a.szz + fo;
+    const float *__restrict__ szx = a.szx + fo;
+    const float *__restrict__ sxx = a.sxx + fo;
+    float *__restrict__ ovz = a.o_vz + fo;
+    float *__restrict__ ovx = a.o_vx + fo;
+    float *__restrict__ oszz = a.o_szz + fo;
+    float *__restrict__ oszx = a.o_szx + fo;
+    float *__restrict__ osxx = a.o_sxx + fo;
+    const size_t pzo = (size_t)s * a.nzs * P, pxo = (size_t)s * a.nz * a.nxs;
+    const float dt = a.dt;
+
+    CpmlPoint px{1.f, 0.f, 1.f, 1.f, 0.f, 1.f, -1, 0};
+    if (sint) px = a.xp[x];
+    const int cxm = px.map;
+    const float bx = px.b, ax = px.a, ikx = px.ik, bhx = px.bh, ahx = px.ah, ikhx = px.ikh;
+    const bool lcol = own && ix_on_lattice(a, x);
+    const int jx = lcol ? (x - a.snap_x1) / a.snap_dx : 0;
+    // unsigned offsets let the loads use a scalar base plus a 32-bit vector offset
+    auto row = [&](int r) { return (unsigned)(min(max(r, 0), a.nz - 1) * P + cx); };
+
+    // Rotating register queues of Q slots.  The row loop below is unrolled Q
+    // times, so with t the unrolled position of input row j every slot index is
+    // a compile-time constant and the queues advance by renaming registers
+    // instead of moving them:
+    //   vz, vx of row j + d            in slot VQ(d) = (t + N + d) mod Q, d = -N .. N + D
+    //   new szz, szx, sxx of row j + d in slot SQ(d) = (t + 2N + d) mod Q, d = -2N .. 0
+    //   old stresses and media of row j + d in slot (t + d) mod R, d = 0 .. D
+    // Q is a multiple of R = D + 1 and holds the 2N + 1 + D rows of vx in flight.
+    constexpr int R = D + 1;
+    constexpr int Q = ((2 * N + 1 + D + R - 1) / R) * R;
+#define VQ(d) ((t + N + (d) + 2 * Q) % Q)
+#define SQ(d) ((t + 2 * N + (d) + 2 * Q) % Q)
+    float qz[Q], qx[Q], qzz[Q], qzx[Q], qxx[Q];
+#pragma unroll
+    for (int i = 0; i < Q; ++i) {
+        qz[i] = qx[i] = 0.f;
+        qzz[i] = qzx[i] = qxx[i] = 0.f;
+    }
+    const int jb = z0 - N, je = z1 + N;
+    // t = 0 at row jb: vz rows jb-N .. jb+N-2+D in slots 0 .., vx rows jb-N .. jb+N-1+D
+#pragma unroll
+    for (int i = 0; i < 2 * N - 1 + D; ++i) qz[i] = vz[row(jb - N + i)];
+#pragma unroll
+    for (int i = 0; i < 2 * N + D; ++i) qx[i] = vx[row(jb - N + i)];
+    float n_zz[R], n_zx[R], n_xx[R], n_l[R], n_m[R], n_mzx[R];
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+        const unsigned r0 = row(jb + k);
+        n_zz[k] = szz[r0]; n_zx[k] = szx[r0]; n_xx[k] = sxx[r0];
+        n_l[k] = a.lam[r0]; n_m[k] = a.mu[r0]; n_mzx[k] = a.muzx[r0];
+    }
+
+    for (int j0 = jb; j0 < je; j0 += Q) {
+#pragma unroll
+    for (int t = 0; t < Q; ++t) {
+        const int j = j0 + t;
+        if (j >= je) break;                    // uniform over the wavefront
+        const int cur = t % R, ahead = (t + D) % R;
+        const float c_zz = n_zz[cur], c_zx = n_zx[cur], c_xx = n_xx[cur];
+        const float lam = n_l[cur], mu = n_m[cur], muzx = n_mzx[cur];
+        if (j + D < je) {                      // input row j + D, into slots no longer read
+            qz[VQ(N - 1 + D)] = vz[row(j + N - 1 + D)];
+            qx[VQ(N + D)] = vx[row(j + N + D)];
+            const unsigned r1 = row(j + D);
+            n_zz[ahead] = szz[r1]; n_zx[ahead] = szx[r1]; n_xx[ahead] = sxx[r1];
+            n_l[ahead] = a.lam[r1]; n_m[ahead] = a.mu[r1]; n_mzx[ahead] = a.muzx[r1];
+        }
+        // ---- stress half-step on row j
+        const float vzc = qz[VQ(0)], vxc = qx[VQ(0)];   // this lane's vz, vx on row j
+        float vz_z = 0.f, vx_z = 0.f, vz_x = 0.f, vx_x = 0.f;
+#pragma unroll
+        for (int k = 1; k <= N; ++k) {
+            vz_z += a.cz[k - 1] * (qz[VQ(k - 1)] - qz[VQ(-k)]);
+            vx_z += a.cz[k - 1] * (qx[VQ(k)] - qx[VQ(1 - k)]);
+            vz_x += a.cx[k - 1] * (lane_off(vzc, lane, k) - lane_off(vzc, lane, 1 - k));
+            vx_x += a.cx[k - 1] * (lane_off(vxc, lane, k - 1) - lane_off(vxc, lane, -k));
+        }
+        float nzz = c_zz, nzx = c_zx, nxx = c_xx;   // rows and columns outside the interior keep their values
+        if (sint && j >= N && j < nzi) {
+            const bool wr = own && j >= z0 && j < z1;
+            const CpmlPoint pz = a.zp[j];
+            if (pz.map >= 0) {
+                const unsigned pi = pz.map * P + x;
+                const float p1 = pz.b * a.pvz_z[pzo + pi] + pz.a * vz_z;
+                const float p2 = pz.bh * a.pvx_z[pzo + pi] + pz.ah * vx_z;
+                if (wr) {
+                    a.o_pvz_z[pzo + pi] = p1;
+                    a.o_pvx_z[pzo + pi] = p2;
+                }
+                vz_z = vz_z * pz.ik + p1;
+                vx_z = vx_z * pz.ikh + p2;
+            }
+            if (cxm >= 0) {
+                const unsigned pi = j * a.nxs + cxm;
+                const float p1 = bx * a.pvx_x[pxo + pi] + ax * vx_x;
+                const float p2 = bhx * a.pvz_x[pxo + pi] + ahx * vz_x;
+                if (wr) {
+                    a.o_pvx_x[pxo + pi] = p1;
+                    a.o_pvz_x[pxo + pi] = p2;
+                }
+                vx_x = vx_x * ikx + p1;
+                vz_x = vz_x * ikhx + p2;
+            }
+            const float div = lam * (vx_x + vz_z);
+            nzz = c_zz + dt * (div + 2.f * mu * vz_z);
+            nzx = c_zx + dt * muzx * (vz_x + vx_z);
+            nxx = c_xx + dt * (div + 2.f * mu * vx_x);
+            if (j == a.surf_top) {   // surf_mirror at row N: no image rows (src/cpu/n_kernel_lib_PSV.cpp:297-301)
+                nzz = 0.f;
+                nzx = 0.f;
+                nxx = 4.f * dt * vx_x * (lam * mu + mu * mu) / (lam + 2.f * mu);
+            }
+            if (wr) {
+                const unsigned r = j * P + x;
+                oszz[r] = nzz;
+                oszx[r] = nzx;
+                osxx[r] = nxx;
+            }
+        }
+        qzz[SQ(0)] = nzz;
+        qzx[SQ(0)] = nzx;
+        qxx[SQ(0)] = nxx;
+
+        // ---- velocity half-step on row z = j - N
+        const int z = j - N;
+        if (z < z0) continue;                  // the first 2N input rows only fill the queues
+        const float szxc = qzx[SQ(-N)], sxxc = qxx[SQ(-N)];   // this lane's new szx, sxx on row z
+        float szz_z = 0.f, szx_z = 0.f, szx_x = 0.f, sxx_x = 0.f;
+#pragma unroll
+        for (int k = 1; k <= N; ++k) {
+            szz_z += a.cz[k - 1] * (qzz[SQ(k - N)] - qzz[SQ(1 - N - k)]);
+            szx_z += a.cz[k - 1] * (qzx[SQ(k - 1 - N)] - qzx[SQ(-N - k)]);
+            szx_x += a.cx[k - 1] * (lane_off(szxc, lane, k - 1) - lane_off(szxc, lane, -k));
+            sxx_x += a.cx[k - 1] * (lane_off(sxxc, lane, k) - lane_off(sxxc, lane, 1 - k));
+        }
+        if (!own) continue;
+        const unsigned r = z * P + x;
+        // integer-point profiles for szz_z and sxx_x, half-point for szx_z and szx_x,
+        // the reference's slot mapping (src/cpu/n_kernel_PSV.cpp:142-146)
+        const CpmlPoint pz = a.zp[z];
+        if (pz.map >= 0) {
+            const unsigned pi = pz.map * P + x;
+            const float p1 = pz.b * a.pszz_z[pzo + pi] + pz.a * szz_z;
+            const float p2 = pz.bh * a.pszx_z[pzo + pi] + pz.ah * szx_z;
+            a.pszz_z[pzo + pi] = p1;
+            a.pszx_z[pzo + pi] = p2;
+            szz_z = szz_z * pz.ik + p1;
+            szx_z = szx_z * pz.ikh + p2;
+        }
+        if (cxm >= 0) {
+            const unsigned pi = z * a.nxs + cxm;
+            const float p1 = bx * a.psxx_x[pxo + pi] + ax * sxx_x;
+            const float p2 = bhx * a.pszx_x[pxo + pi] + ahx * szx_x;
+            a.psxx_x[pxo + pi] = p1;
+            a.pszx_x[pxo + pi] = p2;
+            sxx_x = sxx_x * ikx + p1;
+            szx_x = szx_x * ikhx + p2;
+        }
+        const float vz0 = qz[VQ(-N)], vx0 = qx[VQ(-N)];   // vz, vx of row z at the previous level
+        const float vz1 = vz0 + dt * a.rhoz[r] * (szx_x + szz_z);
+        const float vx1 = vx0 + dt * a.rhox[r] * (sxx_x + szx_z);
+        ovz[r] = vz1;
+        ovx[r] = vx1;
+        float uz = 0.f, ux = 0.f;
+        if (kAdj) {
+            float *__restrict__ uzs = a.uz + fo;
+            float *__restrict__ uxs = a.ux + fo;
+            uz = uzs[r] + dt * vz0;
+            ux = uxs[r] + dt * vx0;
+            uzs[r] = uz;
+            uxs[r] = ux;
+        }
+        if (kEnergy) {
+            float *__restrict__ we = a.We + fo;
+            we[r] += vx1 * vx1 + vz1 * vz1;
+        }
+        if (!kAdj && !kStore) continue;
+        if (lcol && z >= a.snap_z1 && z <= a.snap_z2 && (z - a.snap_z1) % a.snap_dz == 0) {
+            const int jz = (z - a.snap_z1) / a.snap_dz;
+            const unsigned q = jz * a.snx + jx;   // offset in one shot's snapshot slab
+            const float c_zz2 = qzz[SQ(-N)];   // new szz on row z
+            if (kAdj && a.grad) {
+                const size_t go = (size_t)s * a.snz * a.snx, so = (size_t)s * a.sn_stride;
+                const float Szz = a.sn_szz[so + q], Sxx = a.sn_sxx[so + q], Szx = a.sn_szx[so + q];
+                const float Vz = a.sn_vz[so + q], Vx = a.sn_vx[so + q];
+                const float glam = a.lam[r], gmu = a.mu[r];
+                const float lm = glam + gmu;
+                const float il = a.gain_lm / (lm * lm), im = a.gain_lm / (gmu * gmu);
+                const float s1 = 0.25f * il * (Szz + Sxx) * (c_zz2 + sxxc);
+                const float s2 = 0.25f * im * (Szz - Sxx) * (c_zz2 - sxxc);
+                const float s3 = im * (Szx * szxc);
+                const float s4 = ux * Vx + uz * Vz;
+                a.g_lam[go + q] += a.gw * s1;
+                a.g_mu[go + q] += a.gw * (s3 + s1 + s2);
+                a.g_rho[go + q] += a.gw * s4;
+            }
+            if (kStore) {   // gard_fwd_storage2; k_inject rewrites source cells after injection
+                const size_t o = (size_t)s * a.st_stride + q;
+                a.st_vz[o] = vz1 / dt;
+                a.st_vx[o] = vx1 / dt;
+                a.st_szz[o] = c_zz2;
+                a.st_szx[o] = szxc;
+                a.st_sxx[o] = sxxc;
+            }
+        }
+    }
+    }
+#undef VQ
+#undef SQ
 }
 
 // surf_mirror, one side per launch.  side 0/1: top/bottom row, one thread per
@@ -572,6 +861,15 @@ __global__ void k_inject(const KArgs a, const SrcArgs sa, int it) {
     }
     a.vz[o] = vz;
     a.vx[o] = vx;
+    if (a.store) {   // the one-pass step stored this cell's velocities before the injection
+        const int iz = cell / a.P, ix = cell - iz * a.P;
+        if (iz_on_lattice(a, iz) && ix_on_lattice(a, ix)) {
+            const size_t q = (size_t)s * a.st_stride + (size_t)((iz - a.snap_z1) / a.snap_dz) * a.snx +
+                             (ix - a.snap_x1) / a.snap_dx;
+            a.st_vz[q] = vz / a.dt;
+            a.st_vx[q] = vx / a.dt;
+        }
+    }
 }
 
 // urec2: displacement as the FP64 running sum of dt v / (dz dx).
@@ -635,7 +933,11 @@ static dim3 stencil_grid(const KArgs &a, int N, int tz, int var) {
     return dim3(((nxi + outw - 1) / outw) * ((segs + kBY - 1) / kBY) * a.B);
 }
 
+// the two-kernel form is instantiated for TZ 8, 16, 32, 64 (16 for anything else)
+static int two_kernel_tz(int tz) { return (tz == 8 || tz == 32 || tz == 64) ? tz : 16; }
+
 void launch_stress(const KArgs &a, int N, int tz, int var, hipStream_t st) {
+    tz = two_kernel_tz(tz);
     const dim3 grid = stencil_grid(a, N, tz, var), block(kBX, kBY);
     if (var == 3) {
         PSV_N_CASES(k_stress2, , 6)
@@ -647,6 +949,7 @@ void launch_stress(const KArgs &a, int N, int tz, int var, hipStream_t st) {
 }
 
 void launch_velocity(const KArgs &a, int N, int tz, int var, hipStream_t st) {
+    tz = two_kernel_tz(tz);
     const dim3 grid = stencil_grid(a, N, tz, var), block(kBX, kBY);
     if (var == 3) {
         PSV_N_CASES(k_velocity2, , 6)
@@ -654,6 +957,57 @@ void launch_velocity(const KArgs &a, int N, int tz, int var, hipStream_t st) {
         PSV_N_CASES(k_velocity2, )
     } else {
         PSV_N_CASES(k_velocity, )
+    }
+}
+
+// the launch modes the host runs get their own code; anything else takes the general one
+template <int N, int TZ, int D>
+static void step_modes(const KArgs &a, dim3 grid, dim3 block, hipStream_t st) {
+    const int mode = (a.energy ? kModeEnergy : 0) | (a.displ ? kModeAdjoint : 0) | (a.store ? kModeStore : 0);
+    switch (mode) {
+        case 0: hipLaunchKernelGGL((k_step<N, TZ, 0, D>), grid, block, 0, st, a); break;            // recompute
+        case kModeEnergy: hipLaunchKernelGGL((k_step<N, TZ, kModeEnergy, D>), grid, block, 0, st, a); break;
+        case kModeStore: hipLaunchKernelGGL((k_step<N, TZ, kModeStore, D>), grid, block, 0, st, a); break;
+        case kModeEnergy | kModeStore:
+            hipLaunchKernelGGL((k_step<N, TZ, kModeEnergy | kModeStore, D>), grid, block, 0, st, a);
+            break;
+        case kModeEnergy | kModeAdjoint:
+            hipLaunchKernelGGL((k_step<N, TZ, kModeEnergy | kModeAdjoint, D>), grid, block, 0, st, a);
+            break;
+        default: hipLaunchKernelGGL((k_step<N, TZ, kModeGeneral, D>), grid, block, 0, st, a); break;
+    }
+}
+
+// Only depth 1 is instantiated: on gfx950 at order 8, depths 2 and 3 cost a wave per
+// SIMD (136-146 VGPRs) and measured 13-15 % slower at 1001 x 2001 x 32 shots.
+template <int N, int TZ>
+static void step_depth(const KArgs &a, int depth, dim3 grid, dim3 block, hipStream_t st) {
+    (void)depth;
+    step_modes<N, TZ, 1>(a, grid, block, st);
+}
+
+template <int N>
+static void step_tz(const KArgs &a, int tz, int depth, dim3 grid, dim3 block, hipStream_t st) {
+    if (tz == 32)
+        step_depth<N, 32>(a, depth, grid, block, st);
+    else
+        step_depth<N, 64>(a, depth, grid, block, st);
+}
+
+// k_step writes 64 - 4N columns per wavefront; tz is 32 or 64, depth 1 .. 3
+void launch_step(const KArgs &a, int N, int tz, int depth, hipStream_t st) {
+    if (tz != 32) tz = 64;   // the grid must match the instantiated TZ
+    const int nxi = a.nx - 2 * N, nzi = a.nz - 2 * N;
+    const int outw = kBX - 4 * N;
+    const int segs = (nzi + tz - 1) / tz;
+    const dim3 grid(((nxi + outw - 1) / outw) * ((segs + kBY - 1) / kBY) * a.B), block(kBX, kBY);
+    switch (N) {
+        case 1: step_tz<1>(a, tz, depth, grid, block, st); break;
+        case 2: step_tz<2>(a, tz, depth, grid, block, st); break;
+        case 3: step_tz<3>(a, tz, depth, grid, block, st); break;
+        case 4: step_tz<4>(a, tz, depth, grid, block, st); break;
+        case 5: step_tz<5>(a, tz, depth, grid, block, st); break;
+        default: step_tz<6>(a, tz, depth, grid, block, st); break;
     }
 }
 

@@ -338,8 +338,8 @@ class Simulation:
     # -- profiling --------------------------------------------------------------
     def tuning(self):
         tz = C.c_int32()
-        buf = C.create_string_buffer(512)
-        self._chk(self.lib.fwi_tuning(self.ctx, C.byref(tz), buf, 512))
+        buf = C.create_string_buffer(8192)
+        self._chk(self.lib.fwi_tuning(self.ctx, C.byref(tz), buf, 8192))
         return tz.value, buf.value.decode()
 
     def forward_snapshots(self, shot):
@@ -356,7 +356,7 @@ class Simulation:
         self._chk(self.lib.fwi_profile(self.ctx, int(on)))
 
     def kernel_times(self) -> dict:
-        t = (FwiKernelTime * 7)()
+        t = (FwiKernelTime * len(_lib.KERNELS))()   # FWI_K_COUNT entries
         self._chk(self.lib.fwi_kernel_times(self.ctx, t))
         return {n: (t[i].launches, t[i].total_ms) for i, n in enumerate(_lib.KERNELS)}
 

@@ -121,12 +121,16 @@ typedef struct {
     int32_t grad_sign;      /* -1 (default): the CPU path's "grad -= ..." (n_kernel_lib_PSV.cpp:477-479);
                                +1: the GPU path's "+=" (d_kernel_lib_PSV.cu:838-840) */
     int32_t use_graph;      /* capture time-step chunks in hipGraphs and replay them */
-    int32_t tile_tz;        /* rows per thread in the stencil kernels; 0 = tuned */
+    int32_t tile_tz;        /* rows per thread in the stencil kernels; 0 = tuned;
+                               8, 16, 32, 64 (variants 1-3), 32, 64 (variant 4) */
     int32_t device;         /* HIP device ordinal */
     int32_t check_finite;   /* scan the wavefield for NaN/Inf after each propagation */
     int32_t kernel_variant; /* stencil kernels: 0 = tuned, 1 = x taps as shifted loads,
                                2 = x taps from neighbouring lanes with next-row prefetch,
-                               3 = variant 2 at six waves per SIMD */
+                               3 = variant 2 at six waves per SIMD,
+                               4 = the whole step in one pass (needs no free surface other
+                                   than the top one at row fdorder/2, and sources and
+                                   receivers inside the interior) */
     int32_t work_order;     /* order of the stencil tiles over the grid: 0 = tuned,
                                1 = shot fastest then z then x, 2 = shot fastest then x then z,
                                3 = x tiles fastest, shots outermost */
@@ -147,7 +151,8 @@ enum fwi_kernel_id {
     FWI_K_RECEIVER = 4,   /* urec2 */
     FWI_K_STORE = 5,      /* gard_fwd_storage2 / checkpoint copies */
     FWI_K_OTHER = 6,
-    FWI_K_COUNT = 7
+    FWI_K_STEP = 7,       /* one-pass time step: stress, top free surface and velocity (kernel variant 4) */
+    FWI_K_COUNT = 8
 };
 
 /* ---- library ------------------------------------------------------------- */

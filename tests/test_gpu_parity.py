@@ -85,7 +85,7 @@ def test_shot_gradient_matches_reference_golden(psv, name):
 
 
 @pytest.mark.parametrize("surface", [True, False])
-@pytest.mark.parametrize("variant", [1, 2])
+@pytest.mark.parametrize("variant", [1, 2, 4])
 def test_order8_forward_vs_oracle(psv, O, surface, variant):
     c = cases.small_case(O, surface=surface, nshot=2)
     med = O.mat_av2(c)
@@ -100,7 +100,7 @@ def test_order8_forward_vs_oracle(psv, O, surface, variant):
         assert e < 1e-4
 
 
-@pytest.mark.parametrize("variant", [1, 2])
+@pytest.mark.parametrize("variant", [1, 2, 4])
 def test_order8_gradient_vs_oracle(psv, O, variant):
     c = cases.small_case(O, nshot=2, nt=450)
     med_true = O.mat_av2(c)
@@ -126,7 +126,8 @@ def test_order8_gradient_vs_oracle(psv, O, variant):
             assert e < 2e-3
 
 
-def test_checkpointed_equals_full_storage(psv, O):
+@pytest.mark.parametrize("variant", [1, 4])
+def test_checkpointed_equals_full_storage(psv, O, variant):
     c = cases.small_case(O, nshot=2, nt=300, snap_dt=2)
     med_true = O.mat_av2(c)
     obs = cases.per_shot_obs(c, O, med_true)
@@ -134,7 +135,7 @@ def test_checkpointed_equals_full_storage(psv, O):
     res = {}
     for storage, interval in (("full", 0), ("ckpt", 14), ("ckpt", 300)):
         st = psv._lib.STORE_FULL if storage == "full" else psv._lib.STORE_CHECKPOINT
-        with psv.Simulation(c, storage=st, ckpt_interval=interval) as sim:
+        with psv.Simulation(c, storage=st, ckpt_interval=interval, kernel_variant=variant) as sim:
             assert sim.plan()["storage"] == ("full" if storage == "full" else "checkpoint")
             sim.set_model(lam0, mu0, rho0)
             sim.set_observed([o[0] for o in obs], [o[1] for o in obs])
@@ -147,14 +148,15 @@ def test_checkpointed_equals_full_storage(psv, O):
             np.testing.assert_array_equal(G[i], ref[1][i], err_msg=str(key))
 
 
-def test_batched_uneven_equals_per_shot(psv, O):
+@pytest.mark.parametrize("variant", [1, 4])
+def test_batched_uneven_equals_per_shot(psv, O, variant):
     c = cases.small_case(O, nshot=4, nt=260, uneven=True)
     med_true = O.mat_av2(c)
     obs = cases.per_shot_obs(c, O, med_true)
     lam0, mu0, rho0 = cases.start_model(c)
     out = {}
     for batch in (4, 1, 3):
-        with psv.Simulation(c, batch=batch) as sim:
+        with psv.Simulation(c, batch=batch, kernel_variant=variant) as sim:
             assert sim.plan()["batch"] <= batch
             seis = sim.forward()
             sim.set_model(lam0, mu0, rho0)
@@ -255,3 +257,58 @@ def test_shared_cells_and_empty_shot(psv, O):
         assert uz.shape == (3, int(c["nt"]))
         assert cases.rel_l2(uz, ref["uz"]) < 1e-4
         np.testing.assert_array_equal(uz[0], uz[1])
+
+
+def test_one_pass_step_matches_two_kernel_step(psv, O):
+    """Variant 4 (stress, surface and velocity in one pass, two time levels)
+    against variant 1: seismograms, stored snapshots (the source cells are
+    rewritten by the injection) and the shot gradient.  Same arithmetic, so the
+    differences are FP32 contraction choices only."""
+    c = cases.small_case(O, nshot=2, nt=300)
+    med_true = O.mat_av2(c)
+    obs = cases.per_shot_obs(c, O, med_true)
+    lam0, mu0, rho0 = cases.start_model(c)
+    out = {}
+    for v in (1, 4):
+        with psv.Simulation(c, kernel_variant=v, storage=psv._lib.STORE_FULL) as sim:
+            seis = sim.forward()
+            _, snaps = sim.forward_snapshots(1)
+            sim.set_model(lam0, mu0, rho0)
+            sim.set_observed([o[0] for o in obs], [o[1] for o in obs])
+            m = sim.gradient()
+            out[v] = (seis, snaps, m, sim.get_gradient())
+    for i in range(2):
+        e = cases.rel_l2(out[4][0][i], out[1][0][i])
+        _print(f"one-pass vs two-kernel seismogram {i}: rel-L2 {e:.2e}")
+        assert e < 1e-6
+    for k in out[1][1]:
+        e = cases.rel_l2(out[4][1][k], out[1][1][k])
+        assert e < 1e-6, (k, e)
+    np.testing.assert_allclose(out[4][2], out[1][2], rtol=1e-5)
+    for i in range(3):
+        e = cases.rel_l2(out[4][3][i], out[1][3][i])
+        _print(f"one-pass vs two-kernel gradient {i}: rel-L2 {e:.2e}")
+        assert e < 1e-4
+
+
+def test_work_orders_agree_bitwise(psv, O):
+    """The work order only changes which workgroup computes which tile."""
+    c = cases.small_case(O, nshot=3, nt=120)
+    ref = None
+    for v, tz in ((4, 32), (1, 8)):
+        for order in (1, 2, 3):
+            with psv.Simulation(c, kernel_variant=v, tile_tz=tz, work_order=order) as sim:
+                uz, ux = sim.forward()
+            if ref is None or ref[0] != v:
+                ref = (v, uz, ux)
+            np.testing.assert_array_equal(uz, ref[1])
+            np.testing.assert_array_equal(ux, ref[2])
+
+
+def test_one_pass_step_refused_where_it_does_not_apply(psv, O):
+    c = cases.small_case(O, nshot=1, nt=20)
+    c["isurf"] = np.array([int(c["fdorder"]) // 2, int(c["nz"]) - 20, 0, 0], np.int32)   # a bottom surface too
+    with psv.Simulation(c, kernel_variant=4) as sim:
+        with pytest.raises(psv._lib.FwiError) as e:
+            sim.forward()
+        assert e.value.status == -1

@@ -88,6 +88,7 @@ def load() -> C.CDLL:
         "fwi_forward_snapshots": (C.c_int32, [vp, C.c_int32] + [_dp] * 7),
         "fwi_gradient": (C.c_int32, [vp, C.c_int32, C.c_int32, _dp]),
         "fwi_gradient_raw": (C.c_int32, [vp, C.c_int32, C.c_int32, _dp, _dp, _dp, _dp, _dp]),
+        "fwi_misfit_gradient": (C.c_int32, [vp, C.c_int32, C.c_int32, _dp, _dp, _dp, _dp, _dp]),
         "fwi_zero_gradient": (C.c_int32, [vp]),
         "fwi_get_gradient": (C.c_int32, [vp, _dp, _dp, _dp]),
         "fwi_set_gradient": (C.c_int32, [vp, _dp, _dp, _dp]),
@@ -136,6 +137,7 @@ def check(status: int, ctx=None):
 EXPORTED = ("fwi_abi_version", "fwi_strerror", "fwi_last_error", "fwi_default_opts", "fwi_holbergcoeff",
             "fwi_cpml", "fwi_wavelet", "fwi_dtstab", "fwi_create", "fwi_destroy", "fwi_plan", "fwi_set_model",
             "fwi_get_model", "fwi_get_media", "fwi_set_observed", "fwi_forward", "fwi_get_wavefield",
-            "fwi_gradient", "fwi_gradient_raw", "fwi_zero_gradient", "fwi_get_gradient", "fwi_set_gradient",
+            "fwi_gradient", "fwi_gradient_raw", "fwi_misfit_gradient", "fwi_zero_gradient", "fwi_get_gradient",
+            "fwi_set_gradient",
             "fwi_taper_gradient", "fwi_misfit", "fwi_save_model", "fwi_update_model", "fwi_step_length",
             "fwi_iteration", "fwi_tuning", "fwi_profile", "fwi_kernel_times", "fwi_sync", "fwi_forward_snapshots")

@@ -85,6 +85,8 @@ struct fwi_ctx {
     float *f[9] = {};      // vz vx szz szx sxx uz ux We Wa   [B][nz][P]
     float *psi[8] = {};    // pvz_z pvx_z pszz_z pszx_z [B][nzs][P]; pvx_x pvz_x psxx_x pszx_x [B][nz][nxs]
     float *gr[3] = {};     // [B][snz][snx]
+    float *gx = nullptr;   // [2][B][snz][snx]: the extra sums of the misfit derivative (gform 1)
+    int gform = 0;         // 1 while fwi_misfit_gradient runs
     float *snap[5] = {};   // FULL: [B][snt][snz][snx]; CHECKPOINT: [B][S/snap_dt][snz][snx]
     size_t snap_stride = 0;
     // one-pass step: second buffer set of the 5 fields and the stress-side psi
@@ -309,6 +311,9 @@ KArgs base_args(fwi_ctx *c, int nb, const Wave &w) {
     a.g_lam = c->gr[0];
     a.g_mu = c->gr[1];
     a.g_rho = c->gr[2];
+    a.gform = c->gform;
+    a.g_ext = c->gx;
+    a.g_ext_stride = (size_t)c->B * c->snz * c->snx;
     a.gw = (float)(c->o.grad_sign * (double)c->g.snap_dt * c->g.dt);
     a.gain_lm = (float)kGainLM;
     a.order = c->order;
@@ -640,6 +645,7 @@ int run_adjoint(fwi_ctx *c, int s0, int nb) {
     HIPCHK(hipMemsetAsync(c->f[8], 0, sizeof(float) * nb * c->fs(), c->st));
     for (int i = 0; i < 3; ++i)
         HIPCHK(hipMemsetAsync(c->gr[i], 0, sizeof(float) * nb * c->snz * c->snx, c->st));
+    if (c->gform) HIPCHK(hipMemsetAsync(c->gx, 0, sizeof(float) * 2 * c->B * c->snz * c->snx, c->st));
     KArgs a = base_args(c, nb, w);
     a.We = c->f[8];
     a.energy = 1;
@@ -750,7 +756,7 @@ Need need_bytes(const fwi_ctx *c, int storage, int S) {
     n.shared = 5 * fs * 4 + 9 * nz * nx * 8 + 2 * nz * nx * 8 + (size_t)c->nrec_tot * nt * 8 * 6 +
                (size_t)c->nsrc_tot * nt * 8 * 2 + (64 << 20);
     const size_t state_ck = 5 * fs * 4 + 4 * (c->psiz() + c->psix()) * 4;
-    n.per_shot = 9 * fs * 4 + 8 * (c->psiz() + c->psix()) * 4 + 3 * snapcell * 4;
+    n.per_shot = 9 * fs * 4 + 8 * (c->psiz() + c->psix()) * 4 + 5 * snapcell * 4;
     if (c->plain) n.per_shot += 12 * fs * 4;
     // second time level of the one-pass step (5 fields, 4 stress-side psi)
     const size_t level2 = c->fusable ? (5 * fs + 2 * (c->psiz() + c->psix())) * 4 : 0;
@@ -850,6 +856,7 @@ int allocate(fwi_ctx *c) {
     for (int i = 0; i < 4; ++i) c->psi[i] = F(B * c->psiz());
     for (int i = 4; i < 8; ++i) c->psi[i] = F(B * c->psix());
     for (int i = 0; i < 3; ++i) c->gr[i] = F(B * snapcell);
+    c->gx = F(2 * B * snapcell);
     auto level2 = [&](float **f2, float **psi2) {
         for (int i = 0; i < 5; ++i) f2[i] = F(B * fs);
         for (int i = 0; i < 2; ++i) psi2[i] = F(B * c->psiz());
@@ -1330,6 +1337,80 @@ int32_t fwi_gradient_raw(fwi_ctx *c, int32_t shot0, int32_t nshots, const double
             }
         }
     }
+    return FWI_OK;
+}
+
+int32_t fwi_misfit_gradient(fwi_ctx *c, int32_t shot0, int32_t nshots, const double *adj_z, const double *adj_x,
+                            double *d_lam, double *d_mu, double *d_rho) {
+    if (!c || !adj_z || !adj_x || !d_lam || !d_mu || !d_rho) return FWI_EINVAL;
+    if (!c->have_model) return c->fail(FWI_ESTATE, "fwi_misfit_gradient before fwi_set_model");
+    if (c->plain) return c->fail(FWI_EINVAL, "the plain form computes the reference's sums only");
+    if (c->g.snap_dz != 1 || c->g.snap_dx != 1)
+        return c->fail(FWI_EINVAL, "the misfit derivative needs snap_dz = snap_dx = 1 (staggered chain rule)");
+    std::vector<std::pair<int, int>> bs;
+    if (batches(c, shot0, nshots, bs) != FWI_OK) return c->fail(FWI_EINVAL, "shot range");
+    const int snz = c->snz, snx = c->snx, z1 = c->g.snap_z1, x1 = c->g.snap_x1, nx = c->g.nx;
+    const size_t snapcell = (size_t)snz * snx;
+    // raw sums over the shots, FP64: s1, s1 + s2, rz, s3, rx
+    std::vector<double> raw(5 * snapcell, 0.0);
+    std::vector<float> h(snapcell * c->B);
+    const int nt = c->g.nt, rb = c->rec_off[shot0];
+    c->gform = 1;
+    for (auto [s0, nb] : bs) {
+        const size_t off = (size_t)(c->rec_off[s0] - rb) * nt;
+        const int r = batch_gradient(c, s0, nb, adj_z + off, adj_x + off, nullptr);
+        if (r != FWI_OK) {
+            c->gform = 0;
+            return r;
+        }
+        for (int k = 0; k < 5; ++k) {
+            const float *src = k < 3 ? c->gr[k] : c->gx + (size_t)(k - 3) * c->B * snapcell;
+            HIPCHK(hipMemcpyAsync(h.data(), src, snapcell * nb * 4, hipMemcpyDeviceToHost, c->st));
+            RETIF(sync(c));
+            const bool gained = k == 0 || k == 1 || k == 3;   // the lam / mu terms carry kGainLM
+            for (int s = 0; s < nb; ++s) {
+                const double den = c->alpha[s0 + s] * (gained ? kGainLM : 1.0);
+                for (size_t i = 0; i < snapcell; ++i) raw[k * snapcell + i] += (double)h[s * snapcell + i] / den;
+            }
+        }
+    }
+    c->gform = 0;
+    // The sums were taken with weight grad_sign * snap_dt * dt; with the CPU sign (-1)
+    // they are +dJ/dm of J = sum 0.5 dt r^2 for adjoint sources r.
+    const double sg = -1.0 / c->o.grad_sign;
+    std::vector<double> mu((size_t)c->g.nz * nx);
+    HIPCHK(hipMemcpyAsync(mu.data(), c->m_mu, mu.size() * 8, hipMemcpyDeviceToHost, c->st));
+    RETIF(sync(c));
+    const size_t n = (size_t)c->g.nz * nx;
+    std::memset(d_lam, 0, n * 8);
+    std::memset(d_mu, 0, n * 8);
+    std::memset(d_rho, 0, n * 8);
+    auto R = [&](int k, int jz, int jx) {   // raw sum k at box cell (jz, jx), 0 outside the box
+        return (jz < 0 || jx < 0) ? 0.0 : raw[k * snapcell + (size_t)jz * snx + jx];
+    };
+    for (int jz = 0; jz < snz; ++jz)
+        for (int jx = 0; jx < snx; ++jx) {
+            const int iz = z1 + jz, ix = x1 + jx;
+            const size_t o = (size_t)iz * nx + ix;
+            d_lam[o] = sg * R(0, jz, jx);
+            // mu_zx at (i, j) is the harmonic mean of mu at (i..i+1, j..j+1)
+            // (mat_av2, src/cpu/n_kernel_lib_PSV.cpp:875-880); the shear term of mu at
+            // (i, j) gathers the four mu_zx points it enters: d mu_zx / d mu = mu_zx^2 / (4 mu^2),
+            // with the s3 sum's own 1/mu^2 undone at each point
+            double shear = 0.0;
+            for (int dz = 0; dz <= 1; ++dz)
+                for (int dx = 0; dx <= 1; ++dx) {
+                    const int pz = iz - dz, px = ix - dx;
+                    if (jz - dz < 0 || jx - dx < 0) continue;
+                    const double mp = mu[(size_t)pz * nx + px];
+                    shear += R(3, jz - dz, jx - dx) * mp * mp;
+                }
+            const double mk = mu[o];
+            d_mu[o] = sg * (R(1, jz, jx) + (mk != 0.0 ? shear / (4.0 * mk * mk) : 0.0));
+            // rho_zp at (i, j) averages rho at (i, j) and (i+1, j), rho_xp at (i, j) and (i, j+1)
+            // (mat_av2, :886-887): each density gathers half of the two points of each kind
+            d_rho[o] = sg * 0.5 * (R(2, jz, jx) + R(2, jz - 1, jx) + R(4, jz, jx) + R(4, jz, jx - 1));
+        }
     return FWI_OK;
 }
 

@@ -62,6 +62,12 @@ struct KArgs {
     const float *sn_vz, *sn_vx, *sn_szz, *sn_szx, *sn_sxx;  // slab of shot 0 at this step
     size_t sn_stride;          // between shots
     float *g_lam, *g_mu, *g_rho;  // [B][snz][snx]
+    // gform 0: the reference's sums (fwi_grad2).  gform 1: the terms of the misfit
+    // derivative kept apart for the staggered-grid chain rule on the host:
+    // g_lam = s1, g_mu = s1 + s2, g_rho = dt vz'.Vz, g_ext = s3, g_ext + g_ext_stride = dt vx'.Vx
+    int gform;
+    float *g_ext;
+    size_t g_ext_stride;
     int snap_z1, snap_z2, snap_x1, snap_x2, snap_dz, snap_dx, snz, snx;
     // one-pass step (k_step): fields and stress-side psi of the next time level go
     // to a second buffer set; the velocity-side psi is updated in place

@@ -35,6 +35,26 @@ __device__ __forceinline__ bool iz_on_lattice(const KArgs &a, int z) {
     return z >= a.snap_z1 && z <= a.snap_z2 && (z - a.snap_z1) % a.snap_dz == 0;
 }
 
+// Gradient sums at snapshot cell q (offset in the batch's [B][snz][snx] arrays).
+// s1, s2, s3: the lam+mu, mu and shear correlations of fwi_grad2
+// (src/cpu/n_kernel_lib_PSV.cpp:468-479); s4 = u'.V its density term.  With
+// gform 1 the density terms are the adjoint velocity times the forward velocity
+// per component (rz, rx) and s3 is kept apart, for the host's chain rule.
+__device__ __forceinline__ void grad_accumulate(const KArgs &a, size_t q, float s1, float s2, float s3, float s4,
+                                                float rz, float rx) {
+    if (!a.gform) {
+        a.g_lam[q] += a.gw * s1;
+        a.g_mu[q] += a.gw * (s3 + s1 + s2);
+        a.g_rho[q] += a.gw * s4;
+    } else {
+        a.g_lam[q] += a.gw * s1;
+        a.g_mu[q] += a.gw * (s1 + s2);
+        a.g_rho[q] += a.gw * rz;
+        a.g_ext[q] += a.gw * s3;
+        a.g_ext[a.g_ext_stride + q] += a.gw * rx;
+    }
+}
+
 // Workgroups are dispatched round-robin over the 8 XCDs (each with its own L2).
 // The 1-D launch index is remapped so that every XCD owns one contiguous range
 // of work, ordered tile-major and shot-minor: the B shots of a tile run back to
@@ -245,10 +265,7 @@ __global__ __launch_bounds__(kBX *kBY) void k_velocity(const KArgs a) {
                 const float s1 = 0.25f * il * (Szz + Sxx) * (c_zz + c_xx);
                 const float s2 = 0.25f * im * (Szz - Sxx) * (c_zz - c_xx);
                 const float s3 = im * (Szx * c_zx);
-                const float s4 = ux * Vx + uz * Vz;
-                a.g_lam[q] += a.gw * s1;
-                a.g_mu[q] += a.gw * (s3 + s1 + s2);
-                a.g_rho[q] += a.gw * s4;
+                grad_accumulate(a, q, s1, s2, s3, ux * Vx + uz * Vz, dt * (vz1 * Vz), dt * (vx1 * Vx));
             }
             if (a.store) {   // gard_fwd_storage2; k_inject rewrites source cells after injection
                 const size_t o = (size_t)s * a.st_stride + (size_t)jz * a.snx + jx;
@@ -506,10 +523,7 @@ __global__ __launch_bounds__(kBX *kBY, MINW) void k_velocity2(const KArgs a) {
             const float s1 = 0.25f * il * (Szz + Sxx) * (c_zz + c_xx);
             const float s2 = 0.25f * im * (Szz - Sxx) * (c_zz - c_xx);
             const float s3 = im * (Szx * szxc);
-            const float s4 = ux * Vx + uz * Vz;
-            a.g_lam[q] += a.gw * s1;
-            a.g_mu[q] += a.gw * (s3 + s1 + s2);
-            a.g_rho[q] += a.gw * s4;
+            grad_accumulate(a, q, s1, s2, s3, ux * Vx + uz * Vz, dt * (vz1 * Vz), dt * (vx1 * Vx));
         }
     }
 }
@@ -755,10 +769,7 @@ __global__ __launch_bounds__(kBX *kBY) void k_step(const KArgs a) {
                 const float s1 = 0.25f * il * (Szz + Sxx) * (c_zz2 + sxxc);
                 const float s2 = 0.25f * im * (Szz - Sxx) * (c_zz2 - sxxc);
                 const float s3 = im * (Szx * szxc);
-                const float s4 = ux * Vx + uz * Vz;
-                a.g_lam[go + q] += a.gw * s1;
-                a.g_mu[go + q] += a.gw * (s3 + s1 + s2);
-                a.g_rho[go + q] += a.gw * s4;
+                grad_accumulate(a, go + q, s1, s2, s3, ux * Vx + uz * Vz, dt * (vz1 * Vz), dt * (vx1 * Vx));
             }
             if (kStore) {   // gard_fwd_storage2; k_inject rewrites source cells after injection
                 const size_t o = (size_t)s * a.st_stride + q;

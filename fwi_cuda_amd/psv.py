@@ -279,6 +279,16 @@ class Simulation:
                                             *[dptr(a) for a in out]))
         return tuple(out)
 
+    def misfit_gradient(self, adj_z, adj_x, shot0=0, nshots=None):
+        """dJ/d(lam, mu, rho) [nz][nx] of J = sum 0.5 dt r^2 for adjoint sources r
+        (CSR rows like the seismograms): the backward pass of forward()."""
+        nshots = self.nshot - shot0 if nshots is None else nshots
+        adj_z, adj_x = (np.ascontiguousarray(a, np.float64) for a in (adj_z, adj_x))
+        out = [np.zeros((self.nz, self.nx)) for _ in range(3)]
+        self._chk(self.lib.fwi_misfit_gradient(self.ctx, shot0, nshots, dptr(adj_z), dptr(adj_x),
+                                               *[dptr(a) for a in out]))
+        return tuple(out)
+
     def get_gradient(self):
         out = [np.zeros((self.nz, self.nx)) for _ in range(3)]
         self._chk(self.lib.fwi_get_gradient(self.ctx, *[dptr(a) for a in out]))

@@ -207,6 +207,16 @@ int32_t fwi_gradient(fwi_ctx *ctx, int32_t shot0, int32_t nshots, double *misfit
  * (CSR like the seismograms, injected as type-0 sources in reversed time). */
 int32_t fwi_gradient_raw(fwi_ctx *ctx, int32_t shot0, int32_t nshots, const double *adj_z,
                          const double *adj_x, double *g_lam, double *g_mu, double *g_rho);
+/* Derivative of J = sum over samples of 0.5 dt r^2 with respect to lam, mu and rho
+ * ([nz][nx], zero outside the snapshot box) for adjoint sources r = adj_z / adj_x
+ * (CSR like the seismograms): the backward pass of the modelling operator.  No
+ * single reference function: it is fwi_grad2's correlation
+ * (src/cpu/n_kernel_lib_PSV.cpp:439-484) with the density term taken as the adjoint
+ * velocity times the forward velocity, and the shear and density terms carried
+ * through mat_av2's staggered averages (:853-922) by the chain rule, so that it
+ * matches finite differences of J.  Needs snap_dz = snap_dx = 1 and the fused form. */
+int32_t fwi_misfit_gradient(fwi_ctx *ctx, int32_t shot0, int32_t nshots, const double *adj_z,
+                            const double *adj_x, double *d_lam, double *d_mu, double *d_rho);
 int32_t fwi_zero_gradient(fwi_ctx *ctx);
 int32_t fwi_get_gradient(fwi_ctx *ctx, double *g_lam, double *g_mu, double *g_rho);
 int32_t fwi_set_gradient(fwi_ctx *ctx, const double *g_lam, const double *g_mu, const double *g_rho);

@@ -133,3 +133,40 @@ def test_cpml_absorbs(oracle_mod):
         st = f["state"]
         e[key] = float((st["vx"] ** 2 + st["vz"] ** 2).sum())
     assert e["pml"] < 0.05 * e["rigid"], e
+
+
+def test_reference_correlation_against_misfit_derivative(oracle_mod):
+    """What the reference's gradient (fwi_grad2 with the CPU sign, restated by the
+    oracle) is: for lam it is +dJ/dlam of J = sum 0.5 dt r^2 (directional
+    derivatives along smooth bumps agree with central differences to a few
+    per cent, so the CPU path's update m + eps f G climbs the misfit); its density
+    term is not dJ/drho (1e-2 .. 1e-1 of it, either sign)."""
+    import sys
+    from pathlib import Path
+    sys.path.insert(0, str(Path(__file__).parent))
+    import cases
+    O = oracle_mod
+    c = cases.small_case(O, nshot=1, nt=450, snap_dt=1)
+    obs = cases.per_shot_obs(c, O, O.mat_av2(c))
+    lam0, mu0, rho0 = cases.start_model(c)
+    sg = O.shot_gradient(c, O.mat_av2(c, lam0, mu0, rho0), 0, *obs[0])
+    z1, z2, x1, x2 = (int(v) for v in c["snap"][:4])
+    nz, nx = int(c["nz"]), int(c["nx"])
+    zz, xx = np.mgrid[0:nz, 0:nx]
+    ratios = {}
+    for p, name in ((0, "lam"), (2, "rho")):
+        G = np.zeros((nz, nx))
+        G[z1:z2 + 1, x1:x2 + 1] = sg["raw"][p]
+        r = []
+        for zc, xc in ((42, 26), (44, 52), (11, 26)):
+            base = [lam0, mu0, rho0]
+            dm = 1e-3 * base[p] * np.exp(-((zz - zc) ** 2 + (xx - xc) ** 2) / 8.0)
+            mp = [m.copy() for m in base]
+            mm = [m.copy() for m in base]
+            mp[p] += dm
+            mm[p] -= dm
+            fd = (O.misfit_all(c, *mp, obs) - O.misfit_all(c, *mm, obs)) / 2
+            r.append(fd / float(np.sum(G * dm)))
+        ratios[name] = np.array(r)
+    assert np.all(np.abs(ratios["lam"] - 1) < 0.06), ratios
+    assert np.all(np.abs(ratios["rho"]) < 0.2), ratios

@@ -142,17 +142,16 @@ def c5(nt=4000, seed=5, nshot=32, true_model=True, nz=1001, nx=2001):
     from scipy.ndimage import gaussian_filter
     cp = layers(nz, nx, 8, 2000.0, 4500.0, seed)
     cs = cp / np.sqrt(3.0)
-    if true_model:
-        rng = np.random.default_rng(seed)
-        zz, xx = np.mgrid[0:nz, 0:nx]
-        for k in range(3):
-            zc, xc = rng.integers(nz // 5, nz - nz // 4), rng.integers(nx // 5, nx - nx // 5)
-            a, b = rng.integers(30, 80), rng.integers(40, 120)
-            m = ((zz - zc) / a) ** 2 + ((xx - xc) / b) ** 2 < 1
-            f = 1.08 if k % 2 == 0 else 0.92
-            cp[m] *= f
-            cs[m] *= f
-    else:
+    rng = np.random.default_rng(seed)
+    zz, xx = np.mgrid[0:nz, 0:nx]
+    for k in range(3):   # three elliptical anomalies, +8 % / -8 % / +8 % in cp and cs
+        zc, xc = rng.integers(nz // 5, nz - nz // 4), rng.integers(nx // 5, nx - nx // 5)
+        a, b = rng.integers(30, 80), rng.integers(40, 120)
+        m = ((zz - zc) / a) ** 2 + ((xx - xc) / b) ** 2 < 1
+        f = 1.08 if k % 2 == 0 else 0.92
+        cp[m] *= f
+        cs[m] *= f
+    if not true_model:   # the start model: the true model smoothed with a 40-cell Gaussian
         cp = gaussian_filter(cp, 40.0, mode="nearest")
         cs = gaussian_filter(cs, 40.0, mode="nearest")
     src = np.linspace(NPML + 10, nx - NPML - 11, nshot).astype(np.int32)

@@ -1,0 +1,159 @@
+"""The C-ABI library without a GPU: it loads, exports every entry point the
+header declares with the layouts the Python binding assumes, its host-side
+preprocessing reproduces the compiled reference bit for bit (golden
+``pieces.npz``), and the device entry points refuse to run without a device
+instead of falling back to the CPU."""
+import ctypes as C
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN, ROOT, gpu_available
+
+HEADER = ROOT / "include" / "fwi_psv.h"
+
+
+@pytest.fixture(scope="module")
+def lib():
+    from fwi_cuda_amd import _lib
+    return _lib.load()
+
+
+def _declared():
+    text = HEADER.read_text()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(fwi_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_every_declared_symbol_is_exported(lib):
+    from fwi_cuda_amd import _lib
+    names = _declared()
+    assert len(names) >= 30
+    for n in names:
+        assert hasattr(lib, n), n
+    assert set(names) == set(_lib.EXPORTED)
+    out = subprocess.run(["nm", "-D", "--defined-only", str(_lib.LIB_PATH)], capture_output=True, text=True).stdout
+    exported = set(re.findall(r"\bT (fwi_[a-z0-9_]+)", out))
+    assert set(names) <= exported, set(names) - exported
+
+
+def test_struct_layouts_match_the_header(tmp_path):
+    """sizeof / offsetof of the ABI structs from the C compiler against ctypes."""
+    from fwi_cuda_amd import _lib
+    src = tmp_path / "layout.c"
+    src.write_text(
+        '#include <stdio.h>\n#include <stddef.h>\n#include "fwi_psv.h"\n'
+        "int main(void) {\n"
+        '  printf("%zu %zu %zu %zu %d\\n", sizeof(fwi_grid), sizeof(fwi_acq), sizeof(fwi_opts),\n'
+        "         sizeof(fwi_kernel_time), (int)FWI_K_COUNT);\n"
+        '  printf("%zu %zu %zu\\n", offsetof(fwi_grid, taper), offsetof(fwi_opts, work_order),\n'
+        "         offsetof(fwi_acq, rec_x));\n"
+        "  return 0;\n}\n")
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-I", str(ROOT / "include"), str(src), "-o", str(exe)], check=True)
+    lines = subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split("\n")
+    sizes = [int(v) for v in lines[0].split()]
+    offs = [int(v) for v in lines[1].split()]
+    assert sizes[:4] == [C.sizeof(_lib.FwiGrid), C.sizeof(_lib.FwiAcq), C.sizeof(_lib.FwiOpts),
+                         C.sizeof(_lib.FwiKernelTime)]
+    assert sizes[4] == len(_lib.KERNELS)
+    assert offs == [_lib.FwiGrid.taper.offset, _lib.FwiOpts.work_order.offset, _lib.FwiAcq.rec_x.offset]
+
+
+def test_default_options(lib):
+    from fwi_cuda_amd import _lib
+    o = _lib.FwiOpts()
+    lib.fwi_default_opts(C.byref(o))
+    assert (o.storage, o.form, o.grad_sign, o.check_finite, o.kernel_variant, o.work_order) == \
+        (_lib.STORE_AUTO, _lib.FORM_FUSED, -1, 1, 0, 0)
+    assert lib.fwi_abi_version() == 1
+    assert lib.fwi_strerror(-8)
+
+
+def test_host_preprocessing_matches_reference():
+    """holbergcoeff, cpml_PSV and wavelet of the library against the values the
+    reference's own CPU sources computed (bit for bit)."""
+    from fwi_cuda_amd import psv
+    z = np.load(GOLDEN / "pieces.npz", allow_pickle=False)
+    table = z["holberg"]
+    for e in range(table.shape[0]):
+        for n in range(table.shape[1]):
+            np.testing.assert_array_equal(psv.holbergcoeff(2 * (n + 1), e), table[e, n])
+    for i in range(4):
+        args = z[f"cpml.{i}.args"]
+        out = psv.cpml_PSV(*[float(a) if k not in (5, 6, 7, 8) else int(a) for k, a in enumerate(args)])
+        for key in ("a", "b", "K", "a_half", "b_half", "K_half"):
+            np.testing.assert_array_equal(out[key], z[f"cpml.{i}.{key}"], err_msg=f"cpml {i} {key}")
+    nt, dt, amp, fc, ts = z["wavelet.args"]
+    np.testing.assert_array_equal(psv.wavelet(int(nt), dt, amp, fc, ts), z["wavelet.s"])
+
+
+def test_stability_limit_closed_form():
+    from fwi_cuda_amd import psv
+    nz, nx = 20, 30
+    rho = np.full((nz, nx), 2000.0)
+    mu = rho * 1500.0 ** 2
+    lam = rho * 3000.0 ** 2 - 2 * mu
+    lam[5, 7] = rho[5, 7] * 4000.0 ** 2 - 2 * mu[5, 7]    # the fastest cell sets the limit
+    hc = psv.holbergcoeff(8)
+    dts = psv.checkfd_dtstab(lam, mu, rho, hc, 1.0, 0)
+    assert dts == pytest.approx(1.0 / (np.sqrt(2.0) * np.abs(hc[1:5]).sum() * 4000.0), rel=1e-12)
+
+
+@pytest.mark.skipif(gpu_available(), reason="checks the behaviour without a device")
+def test_no_device_no_fallback():
+    """Without a HIP device the context is refused (FWI_ENODEV): the HIP path has no CPU fallback."""
+    import sys
+    sys.path.insert(0, str(ROOT / "tests"))
+    from oracle import oracle as O
+    import cases
+    from fwi_cuda_amd import _lib, psv
+    O.load("oracle")
+    c = cases.small_case(O, nshot=1, nt=10)
+    with pytest.raises(_lib.FwiError) as e:
+        psv.Simulation(c)
+    assert e.value.status == -8
+
+
+def test_invalid_options_are_rejected_before_any_device_work():
+    """Bad tile rows / variants fail as FWI_EINVAL (validation precedes the device check)."""
+    import sys
+    sys.path.insert(0, str(ROOT / "tests"))
+    from oracle import oracle as O
+    import cases
+    from fwi_cuda_amd import _lib, psv
+    O.load("oracle")
+    c = cases.small_case(O, nshot=1, nt=10)
+    for kw in ({"tile_tz": 5}, {"kernel_variant": 9}, {"work_order": 4}):
+        with pytest.raises(_lib.FwiError) as e:
+            psv.Simulation(c, **kw)
+        assert e.value.status == -1, kw
+
+
+@pytest.mark.parametrize("name,kw,shape,nshot", [
+    ("C1", dict(nt=20), (201, 401), 1),
+    ("C2", dict(nt=20), (301, 601), 8),
+    ("C3", dict(nt=20, nshot=6, nz=201, nx=401), (201, 401), 6),
+    ("C4", dict(nt=20, nshot=3, nz=301, nx=601), (301, 601), 3),
+    ("C5", dict(nt=20, nshot=4, nz=201, nx=401), (201, 401), 4),
+])
+def test_workload_builders(name, kw, shape, nshot):
+    """The seeded workloads: shapes, shot counts, receivers inside the grid, dt at
+    half the stencil's limit, a free surface at the top row of the interior."""
+    from fwi_cuda_amd import configs, psv
+    c = configs.BUILDERS[name](**kw)
+    assert (int(c["nz"]), int(c["nx"])) == shape and int(c["nshot"]) == nshot
+    hc = psv.holbergcoeff(8)
+    dts = psv.checkfd_dtstab(c["lam"], c["mu"], c["rho"], hc, float(c["dx"]), configs.NPML)
+    assert float(c["dt"]) == pytest.approx(0.5 * dts, rel=1e-12)
+    assert list(c["isurf"]) == [4, 0, 0, 0]
+    if "rec_off" in c:
+        counts = np.diff(c["rec_off"])
+        assert len(counts) == nshot and counts.min() >= 1
+    zr, xr = np.asarray(c["z_rec"]), np.asarray(c["x_rec"])
+    assert (zr >= 4).all() and (xr >= configs.NPML).all() and (xr < shape[1] - configs.NPML).all()
+    # identical inputs from the same seed
+    c2 = configs.BUILDERS[name](**kw)
+    np.testing.assert_array_equal(c["lam"], c2["lam"])

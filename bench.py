@@ -149,6 +149,7 @@ def main():
     ap.add_argument("--order", type=int, default=0, help="stencil work order 1..3 (0 = tuned)")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-graph", action="store_true", help="skip the hipGraph-replay gradient")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a device copy of MIB MiB five times (calibrates rocprof byte counters)")
     args = ap.parse_args()
@@ -262,6 +263,26 @@ def main():
                       "launches": v[2], "total_ms": round(v[1], 1)} for k, v in kern.items()}
     sim.close()
 
+    # the same gradient with chunks of time steps captured in hipGraphs and replayed
+    # (per-kernel events do not exist inside a graph: wall clock only)
+    graph = None
+    if not args.no_graph:
+        with psv.Simulation(c, batch=args.batch, tile_tz=args.tile_tz, kernel_variant=args.variant,
+                            work_order=args.order, use_graph=1) as gs:
+            gs.set_observed(*obs)
+            gs.set_model(*start)
+            gs.zero_gradient()
+            gs.sync()
+            sync_torch()
+            t0 = time.perf_counter()
+            gs.gradient()
+            gs.sync()
+            sync_torch()
+            tg = time.perf_counter() - t0
+        graph = {"ms_per_step": round(tg * 1e3, 2), "Gcell_per_s": round(2.0 * cells * nt * nshot / tg / 1e9, 3),
+                 "speedup_vs_eager": round(wall / K / tg, 3)}
+        log("graph replay:", graph)
+
     plain = None
     if not args.no_plain:
         try:
@@ -312,6 +333,7 @@ def main():
         "hbm_frac": round(hbm_frac, 4),
         "roofline": roofline,
         "kernels": per_kernel,
+        "graph": graph,
         "plain": plain,
         "cpu_baseline": cpu,
     }

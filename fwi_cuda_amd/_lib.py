@@ -19,7 +19,7 @@ STATUS = {0: "FWI_OK", -1: "FWI_EINVAL", -2: "FWI_ENOMEM", -3: "FWI_EHIP", -4: "
           -5: "FWI_ECFL", -6: "FWI_ESTEP", -7: "FWI_ESTATE", -8: "FWI_ENODEV"}
 STORE_AUTO, STORE_FULL, STORE_CHECKPOINT = 0, 1, 2
 FORM_FUSED, FORM_PLAIN = 0, 1
-KERNELS = ("stress", "velocity", "surface", "source", "receiver", "store", "other", "step")
+KERNELS = ("stress", "velocity", "surface", "source", "receiver", "store", "other", "step", "graph")
 
 
 class FwiGrid(C.Structure):

@@ -87,6 +87,10 @@ struct fwi_ctx {
     float *gr[3] = {};     // [B][snz][snx]
     float *gx = nullptr;   // [2][B][snz][snx]: the extra sums of the misfit derivative (gform 1)
     int gform = 0;         // 1 while fwi_misfit_gradient runs
+    // graph replay: per-step tables of the forward, recomputation and adjoint passes [nt],
+    // and the base time index of the chunk being replayed
+    StepInfo *si_fwd = nullptr, *si_rec = nullptr, *si_adj = nullptr;
+    int *si_base = nullptr;
     float *snap[5] = {};   // FULL: [B][snt][snz][snx]; CHECKPOINT: [B][S/snap_dt][snz][snx]
     size_t snap_stride = 0;
     // one-pass step: second buffer set of the 5 fields and the stress-side psi
@@ -437,6 +441,58 @@ void time_step(fwi_ctx *c, KArgs &a, const Wave &w, const SrcArgs &sa, int it, f
     }
 }
 
+// ---------------------------------------------------------------------------
+// hipGraph replay of chunks of whole time steps
+// ---------------------------------------------------------------------------
+constexpr int kGraphChunk = 64;   // steps per replayed graph outside checkpoint segments
+
+bool use_graphs(const fwi_ctx *c) { return c->o.use_graph && !c->plain; }
+
+// The launches of one chunk (its steps built with si set, so that every per-step
+// quantity comes from the step table and the base counter) are captured the first
+// time and replayed for every later chunk of the same pass; only the base time
+// index is written before each replay.  The chunk must leave the two time levels
+// of the one-pass step where it found them (an even number of steps).
+class ChunkGraph {
+  public:
+    explicit ChunkGraph(fwi_ctx *c) : c_(c) {}
+    ~ChunkGraph() {
+        if (exec_) hipGraphExecDestroy(exec_);
+    }
+    ChunkGraph(const ChunkGraph &) = delete;
+    ChunkGraph &operator=(const ChunkGraph &) = delete;
+
+    template <class F>
+    int run(int base, F &&launch_chunk) {
+        fwi_ctx *c = c_;
+        launch_set_int(c->si_base, base, c->st);
+        if (!exec_) {
+            const int prof = c->prof_on;
+            c->prof_on = 0;   // no event records inside the capture
+            hipError_t e = hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal);
+            if (e == hipSuccess) {
+                launch_chunk();
+                hipGraph_t g = nullptr;
+                e = hipStreamEndCapture(c->st, &g);
+                if (e == hipSuccess) e = hipGraphInstantiate(&exec_, g, nullptr, nullptr, 0);
+                if (g) hipGraphDestroy(g);
+            }
+            c->prof_on = prof;
+            if (e != hipSuccess) {
+                exec_ = nullptr;
+                return c->fail(FWI_EHIP, "graph capture: %s", hipGetErrorString(e));
+            }
+        }
+        Timer t(c, FWI_K_GRAPH);
+        HIPCHK(hipGraphLaunch(exec_, c->st));
+        return FWI_OK;
+    }
+
+  private:
+    fwi_ctx *c_;
+    hipGraphExec_t exec_ = nullptr;
+};
+
 int zero_state(fwi_ctx *c, int nb, const Wave &w, bool u_we) {
     for (int i = 0; i < 5; ++i) HIPCHK(hipMemsetAsync(w.f[i], 0, sizeof(float) * nb * c->fs(), c->st));
     for (int i = 0; i < 4; ++i) HIPCHK(hipMemsetAsync(w.psi[i], 0, sizeof(float) * nb * c->psiz(), c->st));
@@ -596,10 +652,32 @@ int run_forward(fwi_ctx *c, int s0, int nb, bool store, bool record, bool energy
     const SrcArgs sa = src_args(c, s0, nb);
     const RecArgs ra = rec_args(c, s0, nb);
     const bool ckpt = store && c->storage == FWI_STORE_CHECKPOINT;
-    for (int it = 0; it < c->g.nt; ++it) {
+    const int nt = c->g.nt;
+    // graph chunks: a checkpoint segment, else kGraphChunk steps (even: the one-pass
+    // step's two time levels are back in place after each chunk)
+    const int K = ckpt ? c->S : kGraphChunk;
+    const bool graph = use_graphs(c) && K % 2 == 0;
+    ChunkGraph cg(c);
+    float *snap0[5] = {c->snap[0], c->snap[1], c->snap[2], c->snap[3], c->snap[4]};
+    for (int it = 0; it < nt;) {
         if (ckpt && it % c->S == 0) {
             Timer t(c, FWI_K_STORE);
             RETIF(ckpt_copy(c, it / c->S, nb, w, true));
+        }
+        if (graph && it + K <= nt) {
+            KArgs ag = a;
+            ag.si = c->si_fwd;
+            ag.si_base = c->si_base;
+            ag.si_dir = 1;
+            RETIF(cg.run(it, [&]() {
+                for (int k = 0; k < K; ++k) {
+                    ag.si_k = k;
+                    time_step(c, ag, w, sa, it + k, store && !ckpt ? snap0 : nullptr);
+                    if (record) launch_record(ag, ra, it + k, c->st);
+                }
+            }));
+            it += K;
+            continue;
         }
         const size_t off = (size_t)(it / c->g.snap_dt) * c->snz * c->snx;
         float *dst[5] = {c->snap[0] + off, c->snap[1] + off, c->snap[2] + off, c->snap[3] + off, c->snap[4] + off};
@@ -609,6 +687,7 @@ int run_forward(fwi_ctx *c, int s0, int nb, bool store, bool record, bool energy
             Timer t(c, FWI_K_RECEIVER);
             launch_record(a, ra, it, c->st);
         }
+        ++it;
     }
     c->last_s0 = s0;
     c->last_nb = nb;
@@ -619,6 +698,17 @@ int run_forward(fwi_ctx *c, int s0, int nb, bool store, bool record, bool energy
 // correlates with the forward slab at slab_off (fwi_grad2 fused), or the plain
 // form runs fwi_grad2 as its own pass after the injection, as the reference does.
 void adj_step(fwi_ctx *c, KArgs &a, const Wave &w, const SrcArgs &sa, int it, size_t slab_off) {
+    if (a.si) {   // replayed graph: the step table gates the correlation and places the slab
+        a.sn_vz = c->snap[0];
+        a.sn_vx = c->snap[1];
+        a.sn_szz = c->snap[2];
+        a.sn_szx = c->snap[3];
+        a.sn_sxx = c->snap[4];
+        a.sn_stride = c->snap_stride;
+        a.grad = 1;
+        time_step(c, a, w, sa, it, nullptr);
+        return;
+    }
     const bool gstep = it % c->g.snap_dt == 0;
     if (gstep) {
         a.sn_vz = c->snap[0] + slab_off;
@@ -653,21 +743,61 @@ int run_adjoint(fwi_ctx *c, int s0, int nb) {
     const SrcArgs sa = adj_args(c, s0, nb);
     const int nt = c->g.nt, sdt = c->g.snap_dt;
     const size_t slab = (size_t)c->snz * c->snx;
+    KArgs ag = a;   // adjoint steps of a replayed graph: time index base - k
+    ag.si = c->si_adj;
+    ag.si_base = c->si_base;
+    ag.si_dir = -1;
+    ChunkGraph cga(c);
     if (c->storage != FWI_STORE_CHECKPOINT) {
-        for (int jt = 0; jt < nt; ++jt) {
+        const int K = kGraphChunk;
+        const bool graph = use_graphs(c);
+        for (int jt = 0; jt < nt;) {
             const int it = nt - 1 - jt;
+            if (graph && jt + K <= nt) {
+                RETIF(cga.run(it, [&]() {
+                    for (int k = 0; k < K; ++k) {
+                        ag.si_k = k;
+                        adj_step(c, ag, w, sa, it - k, 0);
+                    }
+                }));
+                jt += K;
+                continue;
+            }
             adj_step(c, a, w, sa, it, (size_t)(it / sdt) * slab);
+            ++jt;
         }
     } else {
         const Wave rw = recompute_wave(c);
         RETIF(zero_state(c, nb, rw, false));   // cells outside the interior stay zero in both levels
         KArgs r = base_args(c, nb, rw);
         const SrcArgs fsa = src_args(c, s0, nb);
+        KArgs rg = r;   // recomputed steps of a replayed graph: time index base + k
+        rg.si = c->si_rec;
+        rg.si_base = c->si_base;
+        rg.si_dir = 1;
+        ChunkGraph cgr(c);
+        const bool graph = use_graphs(c) && c->S % 2 == 0;
+        float *snap0[5] = {c->snap[0], c->snap[1], c->snap[2], c->snap[3], c->snap[4]};
         for (int k = c->nseg - 1; k >= 0; --k) {
             const int t0 = k * c->S, t1 = std::min(t0 + c->S, nt);
             {
                 Timer t(c, FWI_K_STORE);
                 RETIF(ckpt_copy(c, k, nb, rw, false));
+            }
+            if (graph && t1 - t0 == c->S) {
+                RETIF(cgr.run(t0, [&]() {
+                    for (int j = 0; j < c->S; ++j) {
+                        rg.si_k = j;
+                        time_step(c, rg, rw, fsa, t0 + j, snap0);
+                    }
+                }));
+                RETIF(cga.run(t1 - 1, [&]() {
+                    for (int j = 0; j < c->S; ++j) {
+                        ag.si_k = j;
+                        adj_step(c, ag, w, sa, t1 - 1 - j, 0);
+                    }
+                }));
+                continue;
             }
             for (int it = t0; it < t1; ++it) {
                 const size_t off = (size_t)((it - t0) / sdt) * slab;
@@ -784,6 +914,7 @@ int plan(fwi_ctx *c) {
     }
     S = std::max(sdt, (S + sdt - 1) / sdt * sdt);
     S = std::min(S, (c->g.nt + sdt - 1) / sdt * sdt);
+    if (c->o.use_graph && S % 2) S += sdt;   // replayed segments need an even step count (sdt is then odd)
     int storage = c->o.storage;
     auto fits = [&](int st_, int b) {
         Need n = need_bytes(c, st_, S);
@@ -842,7 +973,11 @@ int allocate(fwi_ctx *c) {
     for (int i = 6; i < 12; ++i) c->pf[i] = F(nx);
     c->zp = c->dalloc<CpmlPoint>(nz);
     c->xp = c->dalloc<CpmlPoint>(nx);
-    ok = ok && c->zp && c->xp;
+    c->si_fwd = c->dalloc<StepInfo>(nt);
+    c->si_rec = c->dalloc<StepInfo>(nt);
+    c->si_adj = c->dalloc<StepInfo>(nt);
+    c->si_base = c->dalloc<int>(1);
+    ok = ok && c->zp && c->xp && c->si_fwd && c->si_rec && c->si_adj && c->si_base;
     const size_t recrows = (size_t)c->nrec_tot * nt;
     c->syn_z = D(recrows); c->syn_x = D(recrows);
     c->obs_z = D(recrows); c->obs_x = D(recrows);
@@ -949,6 +1084,23 @@ int setup_device(fwi_ctx *c, const fwi_acq *q) {
     // snapshot cells outside the interior are never written by the one-pass step: zero once
     for (int i = 0; i < 5; ++i)
         HIPCHK(hipMemsetAsync(c->snap[i], 0, sizeof(float) * c->B * c->snap_stride, c->st));
+    {   // per-step tables for graph replay (what time_step / adj_step derive from it)
+        const int nt = c->g.nt, sdt = c->g.snap_dt;
+        const long long slab = (long long)c->snz * c->snx;
+        const bool full = c->storage != FWI_STORE_CHECKPOINT;
+        std::vector<StepInfo> f(nt), r(nt), a(nt);
+        for (int it = 0; it < nt; ++it) {
+            const int t0 = full ? 0 : (it / c->S) * c->S;
+            const long long off = (long long)((it - t0) / sdt) * slab;
+            const int snap = it % sdt == 0;
+            f[it] = {it, 0, full && snap, 0, full ? off : 0};
+            r[it] = {it, 0, snap, 0, off};
+            a[it] = {it, snap, 0, 0, off};
+        }
+        RETIF(upload(c, c->si_fwd, f.data(), nt));
+        RETIF(upload(c, c->si_rec, r.data(), nt));
+        RETIF(upload(c, c->si_adj, a.data(), nt));
+    }
     RETIF(upload(c, c->zmap, zmap.data(), nz));
     RETIF(upload(c, c->xmap, xmap.data(), nx));
     // profiles as b, a, 1/K (integer then half), z then x

@@ -34,6 +34,16 @@ struct CpmlPoint {
     int map, pad;
 };
 
+// What changes from one time step to the next, for launches captured in a hipGraph
+// and replayed: indexed by the time index it, built once per context and pass.
+struct StepInfo {
+    int it;          // time index
+    int grad;        // correlate on this step (it % snap_dt == 0, adjoint pass)
+    int store;       // store a snapshot on this step (forward / recomputation passes)
+    int pad;
+    long long off;   // offset of this step's slab in the snapshot arrays
+};
+
 // Everything a stencil / injection kernel reads or writes, passed by value.
 struct KArgs {
     int nz, nx, P, B;
@@ -78,6 +88,12 @@ struct KArgs {
     int store;
     float *st_vz, *st_vx, *st_szz, *st_szx, *st_sxx;  // slab of shot 0
     size_t st_stride;          // between shots
+    // Graph replay: with si set, the time index is *si_base + si_dir * si_k (si_base is
+    // advanced between replays), and grad / store / the slab offsets come from si[it]
+    // (the sn_* and st_* pointers then name the start of the snapshot arrays).
+    const StepInfo *si;
+    const int *si_base;
+    int si_k, si_dir;
 };
 
 // Point injection / recording (vsrc2, urec2) for the batch.
@@ -115,6 +131,7 @@ void launch_step(const KArgs &a, int N, int tz, int depth, hipStream_t st);
 void launch_surface(const KArgs &a, int N, const int isurf[4], hipStream_t st);
 void launch_inject(const KArgs &a, const SrcArgs &s, int it, hipStream_t st);
 void launch_record(const KArgs &a, const RecArgs &r, int it, hipStream_t st);
+void launch_set_int(int *p, int v, hipStream_t st);   // *p = v, as a kernel
 void launch_store(const KArgs &a, float *const dst[5], size_t dst_stride, float dt, hipStream_t st);
 
 // plain reference-structured path (psv_plain.hip)

@@ -35,6 +35,19 @@ __device__ __forceinline__ bool iz_on_lattice(const KArgs &a, int z) {
     return z >= a.snap_z1 && z <= a.snap_z2 && (z - a.snap_z1) % a.snap_dz == 0;
 }
 
+// Per-step switches of a launch: from the arguments, or, in a replayed graph,
+// from the step table at the time index the graph's base counter gives.
+struct StepView {
+    int it, grad, store;
+    long long off;   // slab offset in the snapshot arrays
+};
+__device__ __forceinline__ StepView step_view(const KArgs &a, int it) {
+    if (!a.si) return {it, a.grad, a.store, 0};
+    const int t = *a.si_base + a.si_dir * a.si_k;
+    const StepInfo d = a.si[t];
+    return {t, a.grad && d.grad, a.store && d.store, d.off};
+}
+
 // Gradient sums at snapshot cell q (offset in the batch's [B][snz][snx] arrays).
 // s1, s2, s3: the lam+mu, mu and shear correlations of fwi_grad2
 // (src/cpu/n_kernel_lib_PSV.cpp:468-479); s4 = u'.V its density term.  With
@@ -187,7 +200,8 @@ __global__ __launch_bounds__(kBX *kBY) void k_velocity(const KArgs a) {
         bhx = a.bhx[ix]; ahx = a.ahx[ix]; ikhx = a.ikhx[ix];
     }
     // snapshot lattice column: gradient correlation (adjoint) or snapshot store (forward)
-    const bool gcol = (a.grad || a.store) && ix_on_lattice(a, ix);
+    const StepView sv = step_view(a, 0);
+    const bool gcol = (sv.grad || sv.store) && ix_on_lattice(a, ix);
     const int jx = gcol ? (ix - a.snap_x1) / a.snap_dx : 0;
 
     // qzz[j] = szz[z - N + 1 + j], qzx[j] = szx[z - N + j]
@@ -252,9 +266,9 @@ __global__ __launch_bounds__(kBX *kBY) void k_velocity(const KArgs a) {
         if (gcol && z >= a.snap_z1 && z <= a.snap_z2 && (z - a.snap_z1) % a.snap_dz == 0) {
             const int jz = (z - a.snap_z1) / a.snap_dz;
             const float c_zz = qzz[N - 1], c_zx = qzx[N], c_xx = sxx[r];
-            if (a.grad) {
+            if (sv.grad) {
                 const size_t q = (size_t)s * a.snz * a.snx + (size_t)jz * a.snx + jx;
-                const size_t o = (size_t)s * a.sn_stride + (size_t)jz * a.snx + jx;
+                const size_t o = sv.off + (size_t)s * a.sn_stride + (size_t)jz * a.snx + jx;
                 const float Szz = a.sn_szz[o], Sxx = a.sn_sxx[o], Szx = a.sn_szx[o];
                 const float Vz = a.sn_vz[o], Vx = a.sn_vx[o];
                 const float lam = a.lam[r], mu = a.mu[r];
@@ -267,8 +281,8 @@ __global__ __launch_bounds__(kBX *kBY) void k_velocity(const KArgs a) {
                 const float s3 = im * (Szx * c_zx);
                 grad_accumulate(a, q, s1, s2, s3, ux * Vx + uz * Vz, dt * (vz1 * Vz), dt * (vx1 * Vx));
             }
-            if (a.store) {   // gard_fwd_storage2; k_inject rewrites source cells after injection
-                const size_t o = (size_t)s * a.st_stride + (size_t)jz * a.snx + jx;
+            if (sv.store) {   // gard_fwd_storage2; k_inject rewrites source cells after injection
+                const size_t o = sv.off + (size_t)s * a.st_stride + (size_t)jz * a.snx + jx;
                 a.st_vz[o] = vz1 / dt;
                 a.st_vx[o] = vx1 / dt;
                 a.st_szz[o] = c_zz;
@@ -422,7 +436,8 @@ __global__ __launch_bounds__(kBX *kBY, MINW) void k_velocity2(const KArgs a) {
         bx = a.bx[ix]; ax = a.ax[ix]; ikx = a.ikx[ix];
         bhx = a.bhx[ix]; ahx = a.ahx[ix]; ikhx = a.ikhx[ix];
     }
-    const bool gcol = out && (a.grad || a.store) && ix_on_lattice(a, ix);
+    const StepView sv = step_view(a, 0);
+    const bool gcol = out && (sv.grad || sv.store) && ix_on_lattice(a, ix);
     const int jx = gcol ? (ix - a.snap_x1) / a.snap_dx : 0;
 
     // qzz[j] = szz[z - N + 1 + j], qzx[j] = szx[z - N + j]
@@ -504,17 +519,17 @@ __global__ __launch_bounds__(kBX *kBY, MINW) void k_velocity2(const KArgs a) {
         if (gcol && z >= a.snap_z1 && z <= a.snap_z2 && (z - a.snap_z1) % a.snap_dz == 0) {
             const int jz = (z - a.snap_z1) / a.snap_dz;
             const float c_zz = qzz[N - 1], c_xx = sxxc;
-            if (a.store) {   // gard_fwd_storage2; k_inject rewrites source cells after injection
-                const size_t o = (size_t)s * a.st_stride + (size_t)jz * a.snx + jx;
+            if (sv.store) {   // gard_fwd_storage2; k_inject rewrites source cells after injection
+                const size_t o = sv.off + (size_t)s * a.st_stride + (size_t)jz * a.snx + jx;
                 a.st_vz[o] = vz1 / dt;
                 a.st_vx[o] = vx1 / dt;
                 a.st_szz[o] = c_zz;
                 a.st_szx[o] = szxc;
                 a.st_sxx[o] = c_xx;
             }
-            if (!a.grad) continue;
+            if (!sv.grad) continue;
             const size_t q = (size_t)s * a.snz * a.snx + (size_t)jz * a.snx + jx;
-            const size_t o = (size_t)s * a.sn_stride + (size_t)jz * a.snx + jx;
+            const size_t o = sv.off + (size_t)s * a.sn_stride + (size_t)jz * a.snx + jx;
             const float Szz = a.sn_szz[o], Sxx = a.sn_sxx[o], Szx = a.sn_szx[o];
             const float Vz = a.sn_vz[o], Vx = a.sn_vx[o];
             const float lam = a.lam[r], mu = a.mu[r];
@@ -558,6 +573,8 @@ __global__ __launch_bounds__(kBX *kBY) void k_step(const KArgs a) {
     const bool kEnergy = kGeneral ? a.energy != 0 : (MODE & kModeEnergy) != 0;
     const bool kAdj = kGeneral ? a.displ != 0 : (MODE & kModeAdjoint) != 0;
     const bool kStore = kGeneral ? a.store != 0 : (MODE & kModeStore) != 0;
+    const StepView sv = step_view(a, 0);
+    const bool doStore = kStore && sv.store, doGrad = kAdj && sv.grad;
     constexpr int OUTW = kBX - 4 * N;
     int tbx, tby, s;
     tile_of<TZ, OUTW>(a, N, tbx, tby, s);
@@ -759,8 +776,8 @@ __global__ __launch_bounds__(kBX *kBY) void k_step(const KArgs a) {
             const int jz = (z - a.snap_z1) / a.snap_dz;
             const unsigned q = jz * a.snx + jx;   // offset in one shot's snapshot slab
             const float c_zz2 = qzz[SQ(-N)];   // new szz on row z
-            if (kAdj && a.grad) {
-                const size_t go = (size_t)s * a.snz * a.snx, so = (size_t)s * a.sn_stride;
+            if (doGrad) {
+                const size_t go = (size_t)s * a.snz * a.snx, so = sv.off + (size_t)s * a.sn_stride;
                 const float Szz = a.sn_szz[so + q], Sxx = a.sn_sxx[so + q], Szx = a.sn_szx[so + q];
                 const float Vz = a.sn_vz[so + q], Vx = a.sn_vx[so + q];
                 const float glam = a.lam[r], gmu = a.mu[r];
@@ -771,8 +788,8 @@ __global__ __launch_bounds__(kBX *kBY) void k_step(const KArgs a) {
                 const float s3 = im * (Szx * szxc);
                 grad_accumulate(a, go + q, s1, s2, s3, ux * Vx + uz * Vz, dt * (vz1 * Vz), dt * (vx1 * Vx));
             }
-            if (kStore) {   // gard_fwd_storage2; k_inject rewrites source cells after injection
-                const size_t o = (size_t)s * a.st_stride + q;
+            if (doStore) {   // gard_fwd_storage2; k_inject rewrites source cells after injection
+                const size_t o = sv.off + (size_t)s * a.st_stride + q;
                 a.st_vz[o] = vz1 / dt;
                 a.st_vx[o] = vx1 / dt;
                 a.st_szz[o] = c_zz2;
@@ -851,9 +868,11 @@ __global__ void k_surface(const KArgs a, int side, int is) {
 
 // vsrc2: one thread per (shot, cell) group; a group's sources are added in
 // source order, as the reference's serial loop does for sources sharing a cell.
-__global__ void k_inject(const KArgs a, const SrcArgs sa, int it) {
+__global__ void k_inject(const KArgs a, const SrcArgs sa, int it_arg) {
     const int g = blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= sa.ngroups) return;
+    const StepView sv = step_view(a, it_arg);
+    const int it = sv.it;
     const int s = sa.grp_shot[g] - sa.shot0;
     const int cell = sa.grp_cell[g];
     const size_t o = (size_t)s * a.fs + cell;
@@ -872,10 +891,10 @@ __global__ void k_inject(const KArgs a, const SrcArgs sa, int it) {
     }
     a.vz[o] = vz;
     a.vx[o] = vx;
-    if (a.store) {   // the one-pass step stored this cell's velocities before the injection
+    if (sv.store) {   // the stencil kernel stored this cell's velocities before the injection
         const int iz = cell / a.P, ix = cell - iz * a.P;
         if (iz_on_lattice(a, iz) && ix_on_lattice(a, ix)) {
-            const size_t q = (size_t)s * a.st_stride + (size_t)((iz - a.snap_z1) / a.snap_dz) * a.snx +
+            const size_t q = sv.off + (size_t)s * a.st_stride + (size_t)((iz - a.snap_z1) / a.snap_dz) * a.snx +
                              (ix - a.snap_x1) / a.snap_dx;
             a.st_vz[q] = vz / a.dt;
             a.st_vx[q] = vx / a.dt;
@@ -883,10 +902,16 @@ __global__ void k_inject(const KArgs a, const SrcArgs sa, int it) {
     }
 }
 
+// the base time index of a replayed graph (a kernel, not a memset: it may sit in a graph)
+__global__ void k_set_int(int *p, int v) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) *p = v;
+}
+
 // urec2: displacement as the FP64 running sum of dt v / (dz dx).
-__global__ void k_record(const KArgs a, const RecArgs ra, int it) {
+__global__ void k_record(const KArgs a, const RecArgs ra, int it_arg) {
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= ra.nrec) return;
+    const int it = step_view(a, it_arg).it;
     const size_t o = (size_t)(ra.rec_shot[r] - ra.shot0) * a.fs + ra.rec_cell[r];
     const double uz = ra.acc_z[r] + ra.dt * (double)a.vz[o] / ra.dzdx;
     const double ux = ra.acc_x[r] + ra.dt * (double)a.vx[o] / ra.dzdx;
@@ -1042,6 +1067,8 @@ void launch_inject(const KArgs &a, const SrcArgs &s, int it, hipStream_t st) {
     if (s.ngroups <= 0) return;
     hipLaunchKernelGGL(k_inject, dim3((s.ngroups + 127) / 128), dim3(128), 0, st, a, s, it);
 }
+
+void launch_set_int(int *p, int v, hipStream_t st) { hipLaunchKernelGGL(k_set_int, dim3(1), dim3(64), 0, st, p, v); }
 
 void launch_record(const KArgs &a, const RecArgs &r, int it, hipStream_t st) {
     if (r.nrec <= 0) return;

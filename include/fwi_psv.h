@@ -120,7 +120,9 @@ typedef struct {
     int32_t form;           /* enum fwi_form */
     int32_t grad_sign;      /* -1 (default): the CPU path's "grad -= ..." (n_kernel_lib_PSV.cpp:477-479);
                                +1: the GPU path's "+=" (d_kernel_lib_PSV.cu:838-840) */
-    int32_t use_graph;      /* capture time-step chunks in hipGraphs and replay them */
+    int32_t use_graph;      /* capture chunks of time steps (a checkpoint segment, or 64 steps)
+                               in a hipGraph once per propagation and replay it; per-kernel
+                               times then only see the replays (FWI_K_GRAPH) */
     int32_t tile_tz;        /* rows per thread in the stencil kernels; 0 = tuned;
                                8, 16, 32, 64 (variants 1-3), 32, 64 (variant 4) */
     int32_t device;         /* HIP device ordinal */
@@ -152,7 +154,8 @@ enum fwi_kernel_id {
     FWI_K_STORE = 5,      /* gard_fwd_storage2 / checkpoint copies */
     FWI_K_OTHER = 6,
     FWI_K_STEP = 7,       /* one-pass time step: stress, top free surface and velocity (kernel variant 4) */
-    FWI_K_COUNT = 8
+    FWI_K_GRAPH = 8,      /* a replayed hipGraph of several whole time steps (use_graph) */
+    FWI_K_COUNT = 9
 };
 
 /* ---- library ------------------------------------------------------------- */

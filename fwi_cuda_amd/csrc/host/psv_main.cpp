@@ -165,13 +165,37 @@ int main(int argc, char **argv) {
                 return 1;
             }
         if ((st = fwi_set_observed(ctx, oz.data(), ox.data()))) return die("set_observed", st, ctx);
-        std::vector<double> L2s;
+        std::vector<double> L2s, mis(nshot);
+        double est = 0.01;   // first step-length estimate (src/cpu/n_simulate_PSV.cpp:243)
         for (int it = 0; it < niter; ++it) {
-            double L2 = 0, step = 0;
-            st = fwi_iteration(ctx, &L2, &step);
-            if (st) return die("fwi iteration", st, ctx);
+            // one simulate_fwi_PSV iteration (src/cpu/n_simulate_PSV.cpp:249-582) through the ABI:
+            // copy_mat, mat_av2 of the current model, the batched shot gradients, taper2,
+            // step_length_PSV and update_mat2
+            std::printf("FWI: Iteration %d\n", it);
+            if ((st = fwi_save_model(ctx))) return die("copy_mat", st, ctx);
+            if ((st = fwi_get_model(ctx, lam.data(), mu.data(), rho.data())) ||
+                (st = fwi_set_model(ctx, lam.data(), mu.data(), rho.data())))
+                return die("mat_av2", st, ctx);
+            if ((st = fwi_gradient(ctx, 0, nshot, mis.data()))) return die("gradient", st, ctx);
+            double L2 = 0;
+            for (int s = 0; s < nshot; ++s) {
+                L2 += mis[s];
+                std::printf("L2 NORM: %d, %.9g\n", s, L2);
+            }
+            if ((st = fwi_taper_gradient(ctx))) return die("taper", st, ctx);
+            double step = 0;
+            int32_t ntrials = 0;
+            st = fwi_step_length(ctx, est, L2, &step, &ntrials);
+            if (st == FWI_ESTEP) {   // the reference prints this and exit(0)s (n_step_length_PSV.cpp:173-178)
+                std::printf("Steplength estimation failed! (%d trials)\n", ntrials);
+                fwi_destroy(ctx);
+                return 2;
+            }
+            if (st) return die("step length", st, ctx);
+            if ((st = fwi_update_model(ctx, step, 0.5, 1.25))) return die("update_mat2", st, ctx);
+            est = step;
             L2s.push_back(L2);
-            std::printf("FWI: Iteration %d  L2 = %.9g  step = %.6g\n", it, L2, step);
+            std::printf("FWI: Iteration %d  L2 = %.9g  step = %.6g  (%d trials)\n", it, L2, step, ntrials);
             if (m.mat_save_interval > 0 && it % m.mat_save_interval == 0) {
                 fwi_get_model(ctx, lam.data(), mu.data(), rho.data());
                 psvio::write_mat(dir, it, n, lam.data(), mu.data(), rho.data());

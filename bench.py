@@ -76,6 +76,36 @@ def build_workload(name, nt, nshot):
     return c, (lam0, mu0, rho0)
 
 
+def kernel_instantiation(kind, tune_log, N):
+    """The template instance the autotuner picked for a stencil kernel kind, as the
+    profiler names it (without the argument list)."""
+    import re
+    m = re.search(r"chosen o\d/v(\d)/tz(\d+)", tune_log or "")
+    if not m:
+        return None
+    var, tz = int(m.group(1)), int(m.group(2))
+    if kind == "step":
+        return f"k_step<{N}, {tz}, "   # several launch modes share the prefix
+    base = {"stress": "k_stress", "velocity": "k_velocity"}[kind]
+    if var == 1:
+        return f"{base}<{N}, {tz}>"
+    return f"{base}2<{N}, {tz}, {1 if var == 2 else 6}>"
+
+
+def measured_traffic(inst):
+    """HBM bytes per launch of a kernel instance from the committed counter summary
+    (lines 'traffic <instance> ... bytes_per_launch=<B>'), or None."""
+    path = ROOT / "profiles" / "roofline_traffic.txt"
+    if not inst or not path.exists():
+        return None
+    for line in path.read_text().splitlines():
+        if line.startswith("traffic ") and inst in line:
+            for tok in line.split():
+                if tok.startswith("bytes_per_launch="):
+                    return float(tok.split("=", 1)[1])
+    return None
+
+
 class quiet_stdout:
     """Send file descriptor 1 to /dev/null (the reference prints progress from C++)."""
 
@@ -139,11 +169,15 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1)
-    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--warmup", type=int, default=0,
+                    help="untimed gradients before the timed ones (the observed-data forward, which "
+                         "runs the autotuner, already warms every kernel)")
     ap.add_argument("--config", default="C3")
     ap.add_argument("--nt", type=int, default=0, help="shorter time axis (profiling runs)")
     ap.add_argument("--nshot", type=int, default=0)
     ap.add_argument("--batch", type=int, default=0)
+    ap.add_argument("--storage", choices=("auto", "full", "checkpoint"), default="auto")
+    ap.add_argument("--ckpt-interval", type=int, default=0)
     ap.add_argument("--tile-tz", type=int, default=0)
     ap.add_argument("--variant", type=int, default=0, help="stencil kernel variant (0 = tuned)")
     ap.add_argument("--order", type=int, default=0, help="stencil work order 1..3 (0 = tuned)")
@@ -178,13 +212,20 @@ def main():
     N = int(c["fdorder"]) // 2
     log(f"workload {args.config}: {nz}x{nx}, nt {nt}, {nshot} shots, {int(c['rec_off'][-1]) if 'rec_off' in c else ''} "
         "receivers")
-    sim = psv.Simulation(c, batch=args.batch, tile_tz=args.tile_tz, kernel_variant=args.variant,
-                         work_order=args.order)
+    storage = {"auto": _lib.STORE_AUTO, "full": _lib.STORE_FULL, "checkpoint": _lib.STORE_CHECKPOINT}[args.storage]
+    kw = dict(batch=args.batch, tile_tz=args.tile_tz, kernel_variant=args.variant, work_order=args.order,
+              storage=storage, ckpt_interval=args.ckpt_interval)
+    sim = psv.Simulation(c, **kw)
     plan = sim.plan()
     log("plan", plan)
+    # the observed-data forward (it also runs the autotuner) is timed per launch too, so
+    # that launch counts and averages can be set beside a profiler's whole-run summary
+    sim.profile(True)
     t0 = time.perf_counter()
     obs = sim.forward()
     log(f"observed data generated in {time.perf_counter() - t0:.1f} s")
+    kt_obs = sim.kernel_times()
+    sim.profile(False)
     sim.set_observed(*obs)
     sim.set_model(*start)
 
@@ -259,9 +300,24 @@ def main():
     dom = max(kern, key=lambda k: kern[k][1])
     b, ms, n = kern[dom]
     achieved = b / (ms / 1e3) / 1e9
-    roofline = {"kernel": dom, "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK / 1e9,
-                "unit": "GB/s", "frac": round(achieved * 1e9 / HBM_PEAK, 4), "traffic": None,
-                "avg_launch_ms": round(ms / max(n, 1), 4), "algorithmic_bytes_per_launch": int(b / max(n, 1))}
+    # every launch of the run that events timed (the observed-data forward and the timed
+    # gradients): the launch set of a whole-run profiler summary of this command
+    obs_shot_step = {"step": interior * B_STEP_FWD + strip * B_PML,
+                     "stress": interior * B_STRESS + strip * B_PML / 2,
+                     "velocity": interior * B_VEL_FWD + strip * B_PML / 2}
+    b_all = b + obs_shot_step[dom] * nshot * nt
+    ms_all, n_all = ms + kt_obs[dom][1], n + kt_obs[dom][0]
+    inst = kernel_instantiation(dom, tune_log, N)
+    traffic = measured_traffic(inst)
+    roofline = {"kernel": dom, "instantiation": inst, "bound": "hbm", "achieved": round(achieved, 1),
+                "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": round(achieved * 1e9 / HBM_PEAK, 4),
+                "traffic": traffic, "avg_launch_ms": round(ms / max(n, 1), 4),
+                "algorithmic_bytes_per_launch": int(b / max(n, 1)),
+                "all_launches": {"launches": n_all, "avg_launch_ms": round(ms_all / max(n_all, 1), 4),
+                                 "algorithmic_bytes_per_launch": int(b_all / max(n_all, 1))}}
+    if traffic:
+        roofline["traffic_source"] = ("profiles/roofline_traffic.txt: rocprofv3 FETCH_SIZE x 2 + WRITE_SIZE per "
+                                      "launch over the same launch mix (all_launches)")
     per_kernel = {k: {"achieved_GBps": round(v[0] / (v[1] / 1e3) / 1e9, 1) if v[1] else None,
                       "launches": v[2], "total_ms": round(v[1], 1)} for k, v in kern.items()}
     sim.close()
@@ -270,8 +326,7 @@ def main():
     # (per-kernel events do not exist inside a graph: wall clock only)
     graph = None
     if not args.no_graph:
-        with psv.Simulation(c, batch=args.batch, tile_tz=args.tile_tz, kernel_variant=args.variant,
-                            work_order=args.order, use_graph=1) as gs:
+        with psv.Simulation(c, use_graph=1, **kw) as gs:
             gs.set_observed(*obs)
             gs.set_model(*start)
             gs.zero_gradient()

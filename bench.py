@@ -318,6 +318,7 @@ def main():
     if traffic:
         roofline["traffic_source"] = ("profiles/roofline_traffic.txt: rocprofv3 FETCH_SIZE x 2 + WRITE_SIZE per "
                                       "launch over the same launch mix (all_launches)")
+        roofline["traffic_over_algorithmic"] = round(traffic / max(b_all / max(n_all, 1), 1.0), 3)
     per_kernel = {k: {"achieved_GBps": round(v[0] / (v[1] / 1e3) / 1e9, 1) if v[1] else None,
                       "launches": v[2], "total_ms": round(v[1], 1)} for k, v in kern.items()}
     sim.close()

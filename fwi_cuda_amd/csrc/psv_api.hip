@@ -96,6 +96,8 @@ struct fwi_ctx {
     // one-pass step: second buffer set of the 5 fields and the stress-side psi
     // (pvz_z pvx_z pvx_x pvz_x); f / psi always name the current time level
     bool fusable = false;               // k_step supports this grid, surface and acquisition
+    ZTab ztab{};                        // z profiles of the strip rows for variant 5
+    bool ztab_ok = false;               // the z strips are [N, lo) and [hi, nz - N), at most kZTabRows rows
     float *f2[5] = {}, *psi2[4] = {};
     // checkpoint mode
     float *rf[5] = {}, *rpsi[8] = {};   // recompute state
@@ -393,9 +395,10 @@ void half_steps(fwi_ctx *c, const KArgs &a) {
         plain_step(a, c->pb, c->N, c->g.isurf, c->g.pml_z, c->g.pml_x, c->st);
         return;
     }
+    const ZTab *zt = c->ztab_ok ? &c->ztab : nullptr;
     {
         Timer t(c, FWI_K_STRESS);
-        launch_stress(a, c->N, c->tz, c->var, c->st);
+        launch_stress(a, c->N, c->tz, c->var, c->st, zt);
     }
     if (has_surface(c)) {
         Timer t(c, FWI_K_SURFACE);
@@ -403,7 +406,7 @@ void half_steps(fwi_ctx *c, const KArgs &a) {
     }
     {
         Timer t(c, FWI_K_VELOCITY);
-        launch_velocity(a, c->N, c->tz, c->var, c->st);
+        launch_velocity(a, c->N, c->tz, c->var, c->st, zt);
     }
 }
 
@@ -570,17 +573,21 @@ int autotune(fwi_ctx *c, int nb) {
     const bool order_forced = c->o.work_order >= 1 && c->o.work_order <= 3;
     // candidates (variant, rows per segment, prefetch depth); the one-pass step only where it applies
     std::vector<TuneVal> cand;
-    for (int var = 1; var <= 4; ++var) {
-        if ((fv && var != fv) || (var == 4 && !c->fusable)) continue;
-        const std::vector<int> tzs = var == 4 ? std::vector<int>{32, 64} : std::vector<int>{8, 16, 32, 64};
+    for (int var = 1; var <= 5; ++var) {
+        if ((fv && var != fv) || (var == 4 && !c->fusable) || (var == 5 && !c->ztab_ok)) continue;
+        const std::vector<int> tzs = var == 4   ? std::vector<int>{32, 64}
+                                     : var == 5 ? std::vector<int>{16, 32}
+                                                : std::vector<int>{8, 16, 32, 64};
         for (int tz : tzs)
             if (c->o.tile_tz <= 0 || c->o.tile_tz == tz) cand.push_back({var, tz, 1, 0});
     }
     if (cand.empty())
         return c->fail(FWI_EINVAL, "no stencil kernel for kernel_variant %d, tile_tz %d%s", fv, c->o.tile_tz,
-                       fv == 4 ? " (the one-pass step takes tile_tz 32 or 64, no free surface other than "
-                                 "the top one at row fdorder/2, and sources and receivers inside the interior)"
-                               : "");
+                       fv == 4   ? " (the one-pass step takes tile_tz 32 or 64, no free surface other than "
+                                   "the top one at row fdorder/2, and sources and receivers inside the interior)"
+                       : fv == 5 ? " (variant 5 takes tile_tz 16 or 32, and z CPML strips of at most 64 rows "
+                                   "at the top and bottom of the grid)"
+                                 : "");
     if (cand.size() == 1 && order_forced) {   // nothing left to choose
         take(c, {cand[0].var, cand[0].tz, cand[0].depth, c->o.work_order - 1});
         return FWI_OK;
@@ -606,8 +613,8 @@ int autotune(fwi_ctx *c, int nb) {
         if (v.var == 4) {
             launch_step(a, c->N, v.tz, v.depth, c->st);
         } else {
-            launch_stress(a, c->N, v.tz, v.var, c->st);
-            launch_velocity(a, c->N, v.tz, v.var, c->st);
+            launch_stress(a, c->N, v.tz, v.var, c->st, &c->ztab);
+            launch_velocity(a, c->N, v.tz, v.var, c->st, &c->ztab);
         }
     };
     float best = 1e30f;
@@ -1072,6 +1079,17 @@ int setup_device(fwi_ctx *c, const fwi_acq *q) {
     if (c->g.pml_x)
         for (int i = N; i < nx - N; ++i)
             if (active(c, 6, i)) xmap[i] = c->nxs++;
+    // variant 5 takes the z strips as the row ranges [N, lo) and [hi, nz - N)
+    {
+        int lo = N, hi = nz - N;
+        while (lo < nz - N && zmap[lo] >= 0) ++lo;
+        while (hi > lo && zmap[hi - 1] >= 0) --hi;
+        bool ok = c->nzs <= kZTabRows;
+        for (int i = lo; ok && i < hi; ++i) ok = zmap[i] < 0;
+        c->ztab_ok = ok;
+        c->ztab.lo = lo;
+        c->ztab.hi = hi;
+    }
     // The one-pass step handles a free surface only at the top row N (no image rows),
     // and injects only into interior cells (it never writes the others).
     auto interior = [&](int z, int x) { return z >= N && z < nz - N && x >= N && x < nx - N; };
@@ -1130,6 +1148,10 @@ int setup_device(fwi_ctx *c, const fwi_acq *q) {
         px[i] = {hx[i], hx[nx + i], hx[2 * nx + i], hx[3 * nx + i], hx[4 * nx + i], hx[5 * nx + i], xmap[i], 0};
     RETIF(upload(c, c->zp, pz.data(), nz));
     RETIF(upload(c, c->xp, px.data(), nx));
+    if (c->ztab_ok)
+        for (int i = 0; i < nz; ++i)
+            if (zmap[i] >= 0)
+                for (int k = 0; k < 6; ++k) c->ztab.p[zmap[i]][k] = hz[k * nz + i];
     // sources
     std::vector<int> gs, gc, go, gi;
     build_groups(c->src_off, q->src_z, q->src_x, c->P, gs, gc, go, gi, c->sg_first);
@@ -1179,8 +1201,8 @@ int validate(fwi_ctx *c, const fwi_grid *g, const fwi_acq *q) {
     const int tt = c->o.tile_tz;
     if (tt != 0 && tt != 8 && tt != 16 && tt != 32 && tt != 64)
         return c->fail(FWI_EINVAL, "tile_tz %d (0, 8, 16, 32 or 64)", tt);
-    if (c->o.kernel_variant < 0 || c->o.kernel_variant > 4)
-        return c->fail(FWI_EINVAL, "kernel_variant %d (0..4)", c->o.kernel_variant);
+    if (c->o.kernel_variant < 0 || c->o.kernel_variant > 5)
+        return c->fail(FWI_EINVAL, "kernel_variant %d (0..5)", c->o.kernel_variant);
     if (c->o.work_order < 0 || c->o.work_order > 3)
         return c->fail(FWI_EINVAL, "work_order %d (0..3)", c->o.work_order);
     if (g->snap_dt < 1 || g->snap_dz < 1 || g->snap_dx < 1) return c->fail(FWI_EINVAL, "snap strides < 1");
@@ -1278,7 +1300,8 @@ int32_t fwi_create(const fwi_grid *g, const fwi_acq *q, const fwi_opts *opts, fw
     if (r == FWI_OK) {
         c->g = *g;
         c->N = g->fdorder / 2;
-        c->P = (int)round_up(g->nx, kPitchAlign);
+        // at least one pad column: variant 5's halo lanes store into column P - 1
+        c->P = (int)round_up(g->nx + 1, kPitchAlign);
         c->snt = 1 + (g->nt - 1) / g->snap_dt;
         c->snz = 1 + (g->snap_z2 - g->snap_z1) / g->snap_dz;
         c->snx = 1 + (g->snap_x2 - g->snap_x1) / g->snap_dx;

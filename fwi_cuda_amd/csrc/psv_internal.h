@@ -1,7 +1,8 @@
 // psv_internal.h -- shared declarations of the gfx950 P/SV library.
 //
 // Device data layout (one context, a batch of B shots advanced together):
-//   float fields [B][nz][P]      vz vx szz szx sxx uz ux We We_adj; P = nx rounded up to 64
+//   float fields [B][nz][P]      vz vx szz szx sxx uz ux We We_adj; P = nx + 1 rounded up to 64
+//                                (column P - 1 is never read: stores of halo lanes land there)
 //   float psi_z  [B][nzs][P]     CPML memory of the z-derivatives, only the nzs rows
 //                                where the z profile is active (zmap[iz] >= 0)
 //   float psi_x  [B][nz][nxs]    CPML memory of the x-derivatives, only the nxs active columns
@@ -32,6 +33,17 @@ constexpr int kBY = 4;            // ... times four z-segments
 struct CpmlPoint {
     float b, a, ik, bh, ah, ikh;
     int map, pad;
+};
+
+// CPML z profiles of the strip rows, passed by value to the variant-5 stencil
+// kernels so that a wave-uniform row reads them as scalar loads.  Interior rows
+// [N, nz - N) are outside the z strips for lo <= z < hi and inside them
+// otherwise; strip row r (psi_z row) holds b, a, 1/K at the integer point, then
+// at the half point.
+constexpr int kZTabRows = 64;
+struct ZTab {
+    int lo, hi;
+    float p[kZTabRows][6];
 };
 
 // What changes from one time step to the next, for launches captured in a hipGraph
@@ -122,9 +134,11 @@ struct RecArgs {
 };
 
 // ---- launchers (psv_kernels.hip) ------------------------------------------------
-// var 1: x taps as shifted global loads; var 2: x taps from neighbouring lanes, next row prefetched
-void launch_stress(const KArgs &a, int N, int tz, int var, hipStream_t st);
-void launch_velocity(const KArgs &a, int N, int tz, int var, hipStream_t st);
+// var 1: x taps as shifted global loads; var 2: x taps from neighbouring lanes, next row
+// prefetched; var 3: var 2 at six waves per SIMD; var 5: var 2 with wave-uniform addressing
+// and ordered loads (needs zt)
+void launch_stress(const KArgs &a, int N, int tz, int var, hipStream_t st, const ZTab *zt = nullptr);
+void launch_velocity(const KArgs &a, int N, int tz, int var, hipStream_t st, const ZTab *zt = nullptr);
 // stress, top free surface and velocity of one time step in one pass (kernel variant 4);
 // tz = rows per segment (32, 64), depth = rows loaded ahead (1 is the one instantiated)
 void launch_step(const KArgs &a, int N, int tz, int depth, hipStream_t st);

@@ -544,6 +544,352 @@ __global__ __launch_bounds__(kBX *kBY, MINW) void k_velocity2(const KArgs a) {
 }
 
 // ---------------------------------------------------------------------------
+// Variant 5: variant 2's stencil with wave-uniform addressing and ordered loads.
+//
+// The wavefront's row segment is made wave-uniform (readfirstlane), so the row
+// loop, the z-strip tests and the snapshot-lattice tests run on the scalar unit,
+// and the z profiles of the strip rows come from the kernel arguments (ZTab) as
+// scalar loads.  Every array is addressed as a wave-uniform base pointer (per
+// shot and row offset, in SGPRs) plus one 32-bit byte offset per lane, the
+// saddr + voffset form of the global loads: a lane keeps a few offset registers
+// instead of a 64-bit address per field.
+//
+// Vector-memory loads retire in order (one vmcnt counter), so waiting for a load
+// also waits for every load issued before it.  Each row therefore issues, in
+// this order: the loads row z uses only after its derivatives (fields to update,
+// CPML memory, displacement, energy, snapshot and gradient sums), then the loads
+// of row z + 1's derivative inputs (the queue heads, and sxx for the velocity
+// kernel).  The derivatives of row z wait only for what row z - 1 prefetched,
+// the update waits only for row z's own loads, and the next row's loads stay in
+// flight across both.  All lanes load (halo lanes read clamped, valid cells) and
+// only owned lanes store.  The velocity kernel's launch modes are template flags.
+// ---------------------------------------------------------------------------
+constexpr int kVelEnergy = 1, kVelAdjoint = 2, kVelStore = 4;
+
+template <class T>
+__device__ __forceinline__ T ld_at(const T *base, uint32_t off) {
+    return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(base) + off);
+}
+template <class T>
+__device__ __forceinline__ void st_at(T *base, uint32_t off, T v) {
+    *reinterpret_cast<T *>(reinterpret_cast<char *>(base) + off) = v;
+}
+__device__ __forceinline__ int wave_uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ __forceinline__ float wave_uniform(float v) {
+    return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
+}
+
+// z-strip test and strip row (the psi_z row) of a wave-uniform row
+__device__ __forceinline__ bool in_zstrip(const ZTab &zt, int z) { return z < zt.lo || z >= zt.hi; }
+__device__ __forceinline__ int strip_row(const ZTab &zt, int N, int z) {
+    return z < zt.lo ? z - N : zt.lo - N + (z - zt.hi);
+}
+
+template <int N, int TZ>
+__global__ __launch_bounds__(kBX *kBY) void k_stress5(const KArgs a, const ZTab zt) {
+    constexpr int OUTW = kBX - 2 * N;
+    int tbx, tby, s;
+    tile_of<TZ, OUTW>(a, N, tbx, tby, s);
+    const int lane = threadIdx.x;
+    const int z0 = N + (tby * kBY + wave_uniform((int)threadIdx.y)) * TZ;
+    const int nzi = a.nz - N;
+    if (z0 >= nzi) return;
+    const int zend = min(z0 + TZ, nzi);
+    const int ix = tbx * OUTW + lane;
+    const bool out = lane >= N && lane < kBX - N && ix < a.nx - N;
+    const int cx = min(ix, a.nx - 1);   // halo lanes past the grid read a valid column
+    const uint32_t c4 = (uint32_t)cx * 4u;
+    const uint32_t P4 = (uint32_t)a.P * 4u;
+    const size_t P = a.P, fs = (size_t)s * a.fs;
+    // base of row z0 + d of a field of this shot, of a media array
+    auto fld = [&](float *f, int d) { return f + fs + (size_t)(z0 + d) * P; };
+    auto med = [&](const float *f, int d) { return f + (size_t)(z0 + d) * P; };
+    const float dt = a.dt;
+
+    // CPML: the lane's x profile; psi_z of this shot (strip row r at r * P);
+    // psi_x of this shot from row z0 (row z at (z - z0) * nxs)
+    const CpmlPoint xq = a.xp[cx];
+    const int cxm = out ? xq.map : -1;
+    const uint32_t xm4 = (uint32_t)max(cxm, 0) * 4u, nxs4 = (uint32_t)a.nxs * 4u;
+    float *pz1 = a.pvz_z + (size_t)s * a.nzs * P, *pz2 = a.pvx_z + (size_t)s * a.nzs * P;
+    float *px1 = a.pvx_x + ((size_t)s * a.nz + z0) * a.nxs, *px2 = a.pvz_x + ((size_t)s * a.nz + z0) * a.nxs;
+    // stores of [..][P] rows: owned lanes at their column, halo lanes at pad column P - 1
+    const uint32_t dtr = out ? 0u : (uint32_t)(a.P - 1) * 4u - c4;
+
+    // qz[j] = vz[z - N + j], qx[j] = vx[z - N + 1 + j]
+    float qz[2 * N], qx[2 * N];
+#pragma unroll
+    for (int j = 0; j < 2 * N - 1; ++j) {
+        qz[j + 1] = ld_at(fld(a.vz, j - N), c4);
+        qx[j + 1] = ld_at(fld(a.vx, j - N + 1), c4);
+    }
+    const float *vzq = fld(a.vz, N - 1), *vxq = fld(a.vx, N);   // queue heads vz[z + N - 1], vx[z + N]
+    float *szz = fld(a.szz, 0), *szx = fld(a.szx, 0), *sxx = fld(a.sxx, 0);
+    const float *lam = med(a.lam, 0), *mu = med(a.mu, 0), *muzx = med(a.muzx, 0);
+    // Queue heads two rows ahead, in two register pairs that alternate between even
+    // and odd rows (the loop is unrolled by two, so no in-flight load is copied);
+    // the last rows reload row zend - 1.
+    const uint32_t vlast = (uint32_t)(zend - 1 - z0) * P4 + c4;
+    float hzA = ld_at(vzq, c4), hxA = ld_at(vxq, c4);
+    float hzB = ld_at(vzq, min(c4 + P4, vlast)), hxB = ld_at(vxq, min(c4 + P4, vlast));
+    // row z (byte offset vo of (z, cx) from row z0); hz, hx hold its heads on entry and
+    // row z + 2's on return
+    auto row = [&](int z, uint32_t vo, float &hz, float &hx) {
+#pragma unroll
+        for (int j = 0; j < 2 * N - 1; ++j) {
+            qz[j] = qz[j + 1];
+            qx[j] = qx[j + 1];
+        }
+        qz[2 * N - 1] = hz;
+        qx[2 * N - 1] = hx;
+        // row z's update inputs, then row z + 2's queue heads; nothing else is loaded
+        // outside the rare CPML strips, so the waits below count exactly these
+        const float c_zz = ld_at(szz, vo), c_zx = ld_at(szx, vo), c_xx = ld_at(sxx, vo);
+        const float l = ld_at(lam, vo), m = ld_at(mu, vo), mzx = ld_at(muzx, vo);
+        const uint32_t v2 = min(vo + 2 * P4, vlast);
+        hz = ld_at(vzq, v2);
+        hx = ld_at(vxq, v2);
+        const float vzc = qz[N], vxc = qx[N - 1];
+        float vz_z = 0.f, vx_z = 0.f, vz_x = 0.f, vx_x = 0.f;
+#pragma unroll
+        for (int k = 1; k <= N; ++k) {
+            vz_z += a.cz[k - 1] * (qz[N + k - 1] - qz[N - k]);
+            vx_z += a.cz[k - 1] * (qx[N + k - 1] - qx[N - k]);
+            vz_x += a.cx[k - 1] * (lane_off(vzc, lane, k) - lane_off(vzc, lane, 1 - k));
+            vx_x += a.cx[k - 1] * (lane_off(vxc, lane, k - 1) - lane_off(vxc, lane, -k));
+        }
+        if (in_zstrip(zt, z)) {   // wave-uniform; b, a, 1/K at the integer point, then at the half point
+            const int zr = strip_row(zt, N, z);
+            const float *pf = zt.p[zr];
+            const uint32_t zo = (uint32_t)zr * P4 + c4;
+            const float p1 = pf[0] * ld_at(pz1, zo) + pf[1] * vz_z;
+            const float p2 = pf[3] * ld_at(pz2, zo) + pf[4] * vx_z;
+            st_at(pz1, zo + dtr, p1);
+            st_at(pz2, zo + dtr, p2);
+            vz_z = vz_z * pf[2] + p1;
+            vx_z = vx_z * pf[5] + p2;
+        }
+        if (cxm >= 0) {
+            const uint32_t xo = (uint32_t)(z - z0) * nxs4 + xm4;
+            const float p1 = xq.b * ld_at(px1, xo) + xq.a * vx_x;
+            const float p2 = xq.bh * ld_at(px2, xo) + xq.ah * vz_x;
+            st_at(px1, xo, p1);
+            st_at(px2, xo, p2);
+            vx_x = vx_x * xq.ik + p1;
+            vz_x = vz_x * xq.ikh + p2;
+        }
+        const float div = l * (vx_x + vz_z);
+        const uint32_t so = vo + dtr;
+        st_at(szz, so, c_zz + dt * (div + 2.f * m * vz_z));
+        st_at(szx, so, c_zx + dt * mzx * (vz_x + vx_z));
+        st_at(sxx, so, c_xx + dt * (div + 2.f * m * vx_x));
+    };
+    int z = z0;
+    uint32_t vo = c4;
+    for (; z + 1 < zend; z += 2, vo += 2 * P4) {
+        row(z, vo, hzA, hxA);
+        row(z + 1, vo + P4, hzB, hxB);
+    }
+    if (z < zend) row(z, vo, hzA, hxA);
+}
+
+// The velocity rows of one wavefront; GRAD: this step correlates with the snapshot slab.
+template <int N, int TZ, int MODE, bool GRAD>
+__device__ __forceinline__ void velocity5_rows(const KArgs &a, const ZTab &zt, const StepView &sv) {
+    constexpr int OUTW = kBX - 2 * N;
+    constexpr bool displ = (MODE & kVelAdjoint) != 0;   // u += dt v
+    int tbx, tby, s;
+    tile_of<TZ, OUTW>(a, N, tbx, tby, s);
+    const int lane = threadIdx.x;
+    const int z0 = N + (tby * kBY + wave_uniform((int)threadIdx.y)) * TZ;
+    const int nzi = a.nz - N;
+    if (z0 >= nzi) return;
+    const int zend = min(z0 + TZ, nzi);
+    const int ix = tbx * OUTW + lane;
+    const bool out = lane >= N && lane < kBX - N && ix < a.nx - N;
+    const int cx = min(ix, a.nx - 1);
+    const uint32_t c4 = (uint32_t)cx * 4u;
+    const uint32_t P4 = (uint32_t)a.P * 4u;
+    const size_t P = a.P, fs = (size_t)s * a.fs;
+    auto fld = [&](float *f, int d) { return f + fs + (size_t)(z0 + d) * P; };
+    auto med = [&](const float *f, int d) { return f + (size_t)(z0 + d) * P; };
+    const float dt = a.dt;
+
+    const CpmlPoint xq = a.xp[cx];
+    const int cxm = out ? xq.map : -1;
+    const uint32_t xm4 = (uint32_t)max(cxm, 0) * 4u, nxs4 = (uint32_t)a.nxs * 4u;
+    float *pz1 = a.pszz_z + (size_t)s * a.nzs * P, *pz2 = a.pszx_z + (size_t)s * a.nzs * P;
+    float *px1 = a.psxx_x + ((size_t)s * a.nz + z0) * a.nxs, *px2 = a.pszx_x + ((size_t)s * a.nz + z0) * a.nxs;
+    // stores of [..][P] rows: owned lanes at their column, halo lanes at pad column P - 1
+    const uint32_t dtr = out ? 0u : (uint32_t)(a.P - 1) * 4u - c4;
+
+    // snapshot lattice: this step's switches, the lane's column, the next lattice row
+    const bool store = (MODE & kVelStore) && sv.store;
+    const bool gcol = out && (GRAD || store) && ix_on_lattice(a, ix);
+    const uint32_t j4 = gcol ? (uint32_t)((ix - a.snap_x1) / a.snap_dx) * 4u : 0u;
+    int zl = a.snap_z1, jz = 0;
+    if (z0 > zl) {
+        jz = (z0 - zl + a.snap_dz - 1) / a.snap_dz;
+        zl += jz * a.snap_dz;
+    }
+    const size_t slab = (size_t)a.snz * a.snx;
+    const size_t soff = (size_t)sv.off;
+
+    // qzz[j] = szz[z - N + 1 + j], qzx[j] = szx[z - N + j]
+    float qzz[2 * N], qzx[2 * N];
+#pragma unroll
+    for (int j = 0; j < 2 * N - 1; ++j) {
+        qzz[j + 1] = ld_at(fld(a.szz, j - N + 1), c4);
+        qzx[j + 1] = ld_at(fld(a.szx, j - N), c4);
+    }
+    const float *szzq = fld(a.szz, N), *szxq = fld(a.szx, N - 1);   // queue heads szz[z + N], szx[z + N - 1]
+    const float *sxx = fld(a.sxx, 0);
+    float *vz = fld(a.vz, 0), *vx = fld(a.vx, 0);
+    const float *rhoz = med(a.rhoz, 0), *rhox = med(a.rhox, 0);
+    // Derivative inputs two rows ahead, in register sets that alternate between even
+    // and odd rows (the loop is unrolled by two); the last rows reload row zend - 1.
+    const uint32_t vlast = (uint32_t)(zend - 1 - z0) * P4 + c4;
+    float hzzA = ld_at(szzq, c4), hzxA = ld_at(szxq, c4), hxxA = ld_at(sxx, c4);
+    const uint32_t v1 = min(c4 + P4, vlast);
+    float hzzB = ld_at(szzq, v1), hzxB = ld_at(szxq, v1), hxxB = ld_at(sxx, v1);
+    // row z (byte offset vo of (z, cx) from row z0); hzz, hzx, hxx hold its derivative
+    // inputs on entry and row z + 2's on return
+    auto row = [&](int z, uint32_t vo, float &hzz, float &hzx, float &hxx) {
+#pragma unroll
+        for (int j = 0; j < 2 * N - 1; ++j) {
+            qzz[j] = qzz[j + 1];
+            qzx[j] = qzx[j + 1];
+        }
+        qzz[2 * N - 1] = hzz;
+        qzx[2 * N - 1] = hzx;
+        const float sxxc = hxx;
+        // row z's update inputs (with the snapshot and gradient sums of a correlating
+        // step: read on every row, a clamped lattice row off the lattice), then row z + 2's
+        // derivative inputs; nothing else is loaded outside the rare CPML strips, so the
+        // waits below count exactly these
+        const bool lrow = (GRAD || store) && z == zl && zl <= a.snap_z2;   // wave-uniform
+        const size_t qrow = (size_t)s * slab + (size_t)min(jz, a.snz - 1) * a.snx;
+        const float vz0 = ld_at(vz, vo), vx0 = ld_at(vx, vo), rz_ = ld_at(rhoz, vo), rx_ = ld_at(rhox, vo);
+        float uz0 = 0.f, ux0 = 0.f, we0 = 0.f;
+        if constexpr (displ) {
+            uz0 = ld_at(fld(a.uz, 0), vo);
+            ux0 = ld_at(fld(a.ux, 0), vo);
+        }
+        if constexpr ((MODE & kVelEnergy) != 0) we0 = ld_at(fld(a.We, 0), vo);
+        float Szz = 0.f, Sxx = 0.f, Szx = 0.f, Vz = 0.f, Vx = 0.f, gl = 0.f, gm = 0.f;
+        float g0 = 0.f, g1 = 0.f, g2 = 0.f, g3 = 0.f, g4 = 0.f;
+        if constexpr (GRAD) {
+            const size_t so = soff + (size_t)s * a.sn_stride + (size_t)min(jz, a.snz - 1) * a.snx;
+            Szz = ld_at(a.sn_szz + so, j4); Sxx = ld_at(a.sn_sxx + so, j4); Szx = ld_at(a.sn_szx + so, j4);
+            Vz = ld_at(a.sn_vz + so, j4); Vx = ld_at(a.sn_vx + so, j4);
+            gl = ld_at(med(a.lam, 0), vo); gm = ld_at(med(a.mu, 0), vo);
+            g0 = ld_at(a.g_lam + qrow, j4); g1 = ld_at(a.g_mu + qrow, j4); g2 = ld_at(a.g_rho + qrow, j4);
+            g3 = ld_at(a.g_ext + qrow, j4);   // the misfit-derivative sums (gform 1) are always allocated
+            g4 = ld_at(a.g_ext + a.g_ext_stride + qrow, j4);
+        }
+        const uint32_t v2 = min(vo + 2 * P4, vlast);
+        hzz = ld_at(szzq, v2);
+        hzx = ld_at(szxq, v2);
+        hxx = ld_at(sxx, v2);
+        const float szxc = qzx[N];
+        float szz_z = 0.f, szx_z = 0.f, szx_x = 0.f, sxx_x = 0.f;
+#pragma unroll
+        for (int k = 1; k <= N; ++k) {
+            szz_z += a.cz[k - 1] * (qzz[N + k - 1] - qzz[N - k]);
+            szx_z += a.cz[k - 1] * (qzx[N + k - 1] - qzx[N - k]);
+            szx_x += a.cx[k - 1] * (lane_off(szxc, lane, k - 1) - lane_off(szxc, lane, -k));
+            sxx_x += a.cx[k - 1] * (lane_off(sxxc, lane, k) - lane_off(sxxc, lane, 1 - k));
+        }
+        // integer-point profiles for szz_z and sxx_x, half-point for szx_z and szx_x,
+        // the reference's slot mapping (src/cpu/n_kernel_PSV.cpp:142-146)
+        if (in_zstrip(zt, z)) {   // wave-uniform
+            const int zr = strip_row(zt, N, z);
+            const float *pf = zt.p[zr];
+            const uint32_t zo = (uint32_t)zr * P4 + c4;
+            const float p1 = pf[0] * ld_at(pz1, zo) + pf[1] * szz_z;
+            const float p2 = pf[3] * ld_at(pz2, zo) + pf[4] * szx_z;
+            st_at(pz1, zo + dtr, p1);
+            st_at(pz2, zo + dtr, p2);
+            szz_z = szz_z * pf[2] + p1;
+            szx_z = szx_z * pf[5] + p2;
+        }
+        if (cxm >= 0) {
+            const uint32_t xo = (uint32_t)(z - z0) * nxs4 + xm4;
+            const float p1 = xq.b * ld_at(px1, xo) + xq.a * sxx_x;
+            const float p2 = xq.bh * ld_at(px2, xo) + xq.ah * szx_x;
+            st_at(px1, xo, p1);
+            st_at(px2, xo, p2);
+            sxx_x = sxx_x * xq.ik + p1;
+            szx_x = szx_x * xq.ikh + p2;
+        }
+        const float vz1 = vz0 + dt * rz_ * (szx_x + szz_z);
+        const float vx1 = vx0 + dt * rx_ * (sxx_x + szx_z);
+        const uint32_t so = vo + dtr;
+        st_at(vz, so, vz1);
+        st_at(vx, so, vx1);
+        float uz = 0.f, ux = 0.f;
+        if constexpr (displ) {
+            uz = uz0 + dt * vz0;
+            ux = ux0 + dt * vx0;
+            st_at(fld(a.uz, 0), so, uz);
+            st_at(fld(a.ux, 0), so, ux);
+        }
+        if constexpr ((MODE & kVelEnergy) != 0) st_at(fld(a.We, 0), so, we0 + (vx1 * vx1 + vz1 * vz1));
+        if (lrow) {
+            const float c_zz = qzz[N - 1], c_xx = sxxc;
+            if (store && gcol) {   // gard_fwd_storage2; k_inject rewrites source cells after injection
+                const size_t o = soff + (size_t)s * a.st_stride + (size_t)jz * a.snx;
+                st_at(a.st_vz + o, j4, vz1 / dt);
+                st_at(a.st_vx + o, j4, vx1 / dt);
+                st_at(a.st_szz + o, j4, c_zz);
+                st_at(a.st_szx + o, j4, szxc);
+                st_at(a.st_sxx + o, j4, c_xx);
+            }
+            if (GRAD && gcol) {
+                const float lm = gl + gm;
+                const float il = a.gain_lm / (lm * lm), im = a.gain_lm / (gm * gm);
+                const float s1 = 0.25f * il * (Szz + Sxx) * (c_zz + c_xx);
+                const float s2 = 0.25f * im * (Szz - Sxx) * (c_zz - c_xx);
+                const float s3 = im * (Szx * szxc);
+                st_at(a.g_lam + qrow, j4, g0 + a.gw * s1);
+                if (!a.gform) {
+                    st_at(a.g_mu + qrow, j4, g1 + a.gw * (s3 + s1 + s2));
+                    st_at(a.g_rho + qrow, j4, g2 + a.gw * (ux * Vx + uz * Vz));
+                } else {
+                    st_at(a.g_mu + qrow, j4, g1 + a.gw * (s1 + s2));
+                    st_at(a.g_rho + qrow, j4, g2 + a.gw * (dt * (vz1 * Vz)));
+                    st_at(a.g_ext + qrow, j4, g3 + a.gw * s3);
+                    st_at(a.g_ext + a.g_ext_stride + qrow, j4, g4 + a.gw * (dt * (vx1 * Vx)));
+                }
+            }
+            zl += a.snap_dz;
+            ++jz;
+        }
+    };
+    int z = z0;
+    uint32_t vo = c4;
+    for (; z + 1 < zend; z += 2, vo += 2 * P4) {
+        row(z, vo, hzzA, hzxA, hxxA);
+        row(z + 1, vo + P4, hzzB, hzxB, hxxB);
+    }
+    if (z < zend) row(z, vo, hzzA, hzxA, hxxA);
+}
+
+// kVelAdjoint launches branch once, on the step's correlation switch, to a loop
+// whose loads are all unconditional
+template <int N, int TZ, int MODE>
+__global__ __launch_bounds__(kBX *kBY) void k_velocity5(const KArgs a, const ZTab zt) {
+    const StepView sv = step_view(a, 0);
+    if constexpr ((MODE & kVelAdjoint) != 0) {
+        if (sv.grad) {
+            velocity5_rows<N, TZ, MODE, true>(a, zt, sv);
+            return;
+        }
+    }
+    velocity5_rows<N, TZ, MODE, false>(a, zt, sv);
+}
+
+// ---------------------------------------------------------------------------
 // Variant 4: the whole time step in one pass (k_step).
 //
 // The wavefront streams down its segment of rows [z0, z1).  At input row j it
@@ -969,13 +1315,50 @@ static dim3 stencil_grid(const KArgs &a, int N, int tz, int var) {
     return dim3(((nxi + outw - 1) / outw) * ((segs + kBY - 1) / kBY) * a.B);
 }
 
-// the two-kernel form is instantiated for TZ 8, 16, 32, 64 (16 for anything else)
-static int two_kernel_tz(int tz) { return (tz == 8 || tz == 32 || tz == 64) ? tz : 16; }
+// the two-kernel form is instantiated for TZ 8, 16, 32, 64 (16 for anything else);
+// variant 5 for TZ 16 and 32
+static int two_kernel_tz(int tz, int var) {
+    if (var == 5) return tz == 32 ? 32 : 16;
+    return (tz == 8 || tz == 32 || tz == 64) ? tz : 16;
+}
 
-void launch_stress(const KArgs &a, int N, int tz, int var, hipStream_t st) {
-    tz = two_kernel_tz(tz);
+#define PSV_TZ5_CASES(KER, N, ...)                                                            \
+    if (tz == 32)                                                                             \
+        hipLaunchKernelGGL((KER<N, 32 __VA_ARGS__>), grid, block, 0, st, a, *zt);             \
+    else                                                                                      \
+        hipLaunchKernelGGL((KER<N, 16 __VA_ARGS__>), grid, block, 0, st, a, *zt);
+
+#define PSV_N5_CASES(KER, ...)                             \
+    switch (N) {                                         \
+        case 1: PSV_TZ5_CASES(KER, 1, __VA_ARGS__) break;  \
+        case 2: PSV_TZ5_CASES(KER, 2, __VA_ARGS__) break;  \
+        case 3: PSV_TZ5_CASES(KER, 3, __VA_ARGS__) break;  \
+        case 4: PSV_TZ5_CASES(KER, 4, __VA_ARGS__) break;  \
+        case 5: PSV_TZ5_CASES(KER, 5, __VA_ARGS__) break;  \
+        default: PSV_TZ5_CASES(KER, 6, __VA_ARGS__) break; \
+    }
+
+// variant 5's velocity launch modes: the ones the passes run get their own code
+static void launch_velocity5(const KArgs &a, const ZTab *zt, int N, int tz, dim3 grid, dim3 block,
+                             hipStream_t st) {
+    const int mode = (a.energy ? kVelEnergy : 0) | (a.displ ? kVelAdjoint : 0) | (a.store ? kVelStore : 0);
+    switch (mode) {
+        case 0: PSV_N5_CASES(k_velocity5, , 0) break;                                   // recomputation
+        case kVelEnergy: PSV_N5_CASES(k_velocity5, , kVelEnergy) break;                 // forward
+        case kVelStore: PSV_N5_CASES(k_velocity5, , kVelStore) break;                   // recomputation, stored
+        case kVelEnergy | kVelStore: PSV_N5_CASES(k_velocity5, , kVelEnergy | kVelStore) break;
+        case kVelEnergy | kVelAdjoint: PSV_N5_CASES(k_velocity5, , kVelEnergy | kVelAdjoint) break;  // adjoint
+        default: PSV_N5_CASES(k_velocity5, , kVelEnergy | kVelAdjoint | kVelStore) break;
+    }
+}
+
+void launch_stress(const KArgs &a, int N, int tz, int var, hipStream_t st, const ZTab *zt) {
+    if (var == 5 && !zt) var = 2;   // variant 5 needs the strip-row table
+    tz = two_kernel_tz(tz, var);
     const dim3 grid = stencil_grid(a, N, tz, var), block(kBX, kBY);
-    if (var == 3) {
+    if (var == 5) {
+        PSV_N5_CASES(k_stress5, )
+    } else if (var == 3) {
         PSV_N_CASES(k_stress2, , 6)
     } else if (var == 2) {
         PSV_N_CASES(k_stress2, )
@@ -984,10 +1367,13 @@ void launch_stress(const KArgs &a, int N, int tz, int var, hipStream_t st) {
     }
 }
 
-void launch_velocity(const KArgs &a, int N, int tz, int var, hipStream_t st) {
-    tz = two_kernel_tz(tz);
+void launch_velocity(const KArgs &a, int N, int tz, int var, hipStream_t st, const ZTab *zt) {
+    if (var == 5 && (!zt || (a.grad && !a.displ))) var = 2;   // variant 5 correlates in adjoint launches only
+    tz = two_kernel_tz(tz, var);
     const dim3 grid = stencil_grid(a, N, tz, var), block(kBX, kBY);
-    if (var == 3) {
+    if (var == 5) {
+        launch_velocity5(a, zt, N, tz, grid, block, st);
+    } else if (var == 3) {
         PSV_N_CASES(k_velocity2, , 6)
     } else if (var == 2) {
         PSV_N_CASES(k_velocity2, )

@@ -85,7 +85,7 @@ def test_shot_gradient_matches_reference_golden(psv, name):
 
 
 @pytest.mark.parametrize("surface", [True, False])
-@pytest.mark.parametrize("variant", [1, 2, 4])
+@pytest.mark.parametrize("variant", [1, 2, 4, 5])
 def test_order8_forward_vs_oracle(psv, O, surface, variant):
     c = cases.small_case(O, surface=surface, nshot=2)
     med = O.mat_av2(c)
@@ -100,7 +100,7 @@ def test_order8_forward_vs_oracle(psv, O, surface, variant):
         assert e < 1e-4
 
 
-@pytest.mark.parametrize("variant", [1, 2, 4])
+@pytest.mark.parametrize("variant", [1, 2, 4, 5])
 def test_order8_gradient_vs_oracle(psv, O, variant):
     c = cases.small_case(O, nshot=2, nt=450)
     med_true = O.mat_av2(c)
@@ -126,7 +126,7 @@ def test_order8_gradient_vs_oracle(psv, O, variant):
             assert e < 2e-3
 
 
-@pytest.mark.parametrize("variant", [1, 4])
+@pytest.mark.parametrize("variant", [1, 4, 5])
 def test_checkpointed_equals_full_storage(psv, O, variant):
     c = cases.small_case(O, nshot=2, nt=300, snap_dt=2)
     med_true = O.mat_av2(c)
@@ -148,7 +148,7 @@ def test_checkpointed_equals_full_storage(psv, O, variant):
             np.testing.assert_array_equal(G[i], ref[1][i], err_msg=str(key))
 
 
-@pytest.mark.parametrize("variant", [1, 4])
+@pytest.mark.parametrize("variant", [1, 4, 5])
 def test_batched_uneven_equals_per_shot(psv, O, variant):
     c = cases.small_case(O, nshot=4, nt=260, uneven=True)
     med_true = O.mat_av2(c)
@@ -259,6 +259,45 @@ def test_shared_cells_and_empty_shot(psv, O):
         np.testing.assert_array_equal(uz[0], uz[1])
 
 
+@pytest.mark.parametrize("fdorder", [2, 8])
+@pytest.mark.parametrize("tz", [16, 32])
+def test_uniform_addressing_variant_matches_variant2(psv, O, fdorder, tz):
+    """Variant 5 (wave-uniform addressing, ordered loads, strip profiles from the
+    kernel arguments, halo stores into the pad column) against variant 2, the
+    same stencil: seismograms, stored snapshots, the wavefield, the shot
+    gradient and the misfit-derivative sums, over uneven shots in one batch."""
+    c = cases.small_case(O, nshot=3, nt=240, fdorder=fdorder, uneven=True)
+    med_true = O.mat_av2(c)
+    obs = cases.per_shot_obs(c, O, med_true)
+    lam0, mu0, rho0 = cases.start_model(c)
+    out = {}
+    for v in (2, 5):
+        with psv.Simulation(c, kernel_variant=v, tile_tz=tz, storage=psv._lib.STORE_FULL) as sim:
+            seis = sim.forward()
+            wf = sim.wavefield(1, "szx")
+            _, snaps = sim.forward_snapshots(1)
+            sim.set_model(lam0, mu0, rho0)
+            sim.set_observed([o[0] for o in obs], [o[1] for o in obs])
+            m = sim.gradient()
+            G = sim.get_gradient()
+            adj = [np.asarray(o, np.float64) * 1e-3 for o in obs[0]]
+            mg = sim.misfit_gradient(*adj, shot0=0, nshots=1)
+            out[v] = (seis, wf, snaps, m, G, mg)
+    for i in range(2):
+        e = cases.rel_l2(out[5][0][i], out[2][0][i])
+        _print(f"order {fdorder} tz {tz}: variant 5 vs 2 seismogram {i} rel-L2 {e:.2e}")
+        assert e < 1e-6
+    assert cases.rel_l2(out[5][1], out[2][1]) < 1e-6
+    for k in out[2][2]:
+        assert cases.rel_l2(out[5][2][k], out[2][2][k]) < 1e-6, k
+    np.testing.assert_allclose(out[5][3], out[2][3], rtol=1e-5)
+    for i in range(3):
+        e = cases.rel_l2(out[5][4][i], out[2][4][i])
+        e2 = cases.rel_l2(out[5][5][i], out[2][5][i])
+        _print(f"order {fdorder} tz {tz}: variant 5 vs 2 gradient {i} rel-L2 {e:.2e}, misfit derivative {e2:.2e}")
+        assert e < 1e-4 and e2 < 1e-4
+
+
 def test_one_pass_step_matches_two_kernel_step(psv, O):
     """Variant 4 (stress, surface and velocity in one pass, two time levels)
     against variant 1: seismograms, stored snapshots (the source cells are
@@ -305,7 +344,7 @@ def test_work_orders_agree_bitwise(psv, O):
             np.testing.assert_array_equal(ux, ref[2])
 
 
-@pytest.mark.parametrize("variant", [1, 4])
+@pytest.mark.parametrize("variant", [1, 4, 5])
 @pytest.mark.parametrize("storage", ["full", "ckpt"])
 def test_graph_replay_equals_eager(psv, O, variant, storage):
     """use_graph: chunks of time steps captured once per propagation and replayed

@@ -398,7 +398,7 @@ void half_steps(fwi_ctx *c, const KArgs &a) {
     const ZTab *zt = c->ztab_ok ? &c->ztab : nullptr;
     {
         Timer t(c, FWI_K_STRESS);
-        launch_stress(a, c->N, c->tz, c->var, c->st, zt);
+        launch_stress(a, c->N, c->tz, c->var, c->st, zt, c->depth);
     }
     if (has_surface(c)) {
         Timer t(c, FWI_K_SURFACE);
@@ -406,7 +406,7 @@ void half_steps(fwi_ctx *c, const KArgs &a) {
     }
     {
         Timer t(c, FWI_K_VELOCITY);
-        launch_velocity(a, c->N, c->tz, c->var, c->st, zt);
+        launch_velocity(a, c->N, c->tz, c->var, c->st, zt, c->depth);
     }
 }
 
@@ -573,8 +573,15 @@ int autotune(fwi_ctx *c, int nb) {
     const bool order_forced = c->o.work_order >= 1 && c->o.work_order <= 3;
     // candidates (variant, rows per segment, prefetch depth); the one-pass step only where it applies
     std::vector<TuneVal> cand;
-    for (int var = 1; var <= 5; ++var) {
-        if ((fv && var != fv) || (var == 4 && !c->fusable) || (var == 5 && !c->ztab_ok)) continue;
+    for (int var = 1; var <= 6; ++var) {
+        if ((fv && var != fv) || (var == 4 && !c->fusable) || (var >= 5 && !c->ztab_ok)) continue;
+        if (var == 6) {   // loads one row ahead, segments of one or two row blocks
+            for (int m = 1; m <= 2; ++m) {
+                const int tz = v6_block(c->N, 1) * m;
+                if (c->o.tile_tz <= 0 || c->o.tile_tz == tz) cand.push_back({var, tz, 1, 0});
+            }
+            continue;
+        }
         const std::vector<int> tzs = var == 4   ? std::vector<int>{32, 64}
                                      : var == 5 ? std::vector<int>{16, 32}
                                                 : std::vector<int>{8, 16, 32, 64};
@@ -587,6 +594,8 @@ int autotune(fwi_ctx *c, int nb) {
                                    "the top one at row fdorder/2, and sources and receivers inside the interior)"
                        : fv == 5 ? " (variant 5 takes tile_tz 16 or 32, and z CPML strips of at most 64 rows "
                                    "at the top and bottom of the grid)"
+                       : fv == 6 ? " (variant 6 takes tile_tz 0, fdorder + 2 or 2 fdorder + 4, and z CPML "
+                                   "strips of at most 64 rows at the top and bottom of the grid)"
                                  : "");
     if (cand.size() == 1 && order_forced) {   // nothing left to choose
         take(c, {cand[0].var, cand[0].tz, cand[0].depth, c->o.work_order - 1});
@@ -613,8 +622,8 @@ int autotune(fwi_ctx *c, int nb) {
         if (v.var == 4) {
             launch_step(a, c->N, v.tz, v.depth, c->st);
         } else {
-            launch_stress(a, c->N, v.tz, v.var, c->st, &c->ztab);
-            launch_velocity(a, c->N, v.tz, v.var, c->st, &c->ztab);
+            launch_stress(a, c->N, v.tz, v.var, c->st, &c->ztab, v.depth);
+            launch_velocity(a, c->N, v.tz, v.var, c->st, &c->ztab, v.depth);
         }
     };
     float best = 1e30f;
@@ -1199,10 +1208,11 @@ int validate(fwi_ctx *c, const fwi_grid *g, const fwi_acq *q) {
     if (g->nz < 2 * N + 1 || g->nx < 2 * N + 1) return c->fail(FWI_EINVAL, "grid smaller than the stencil");
     if (!(g->dt > 0) || !(g->dz > 0) || !(g->dx > 0)) return c->fail(FWI_EINVAL, "dt, dz, dx must be positive");
     const int tt = c->o.tile_tz;
-    if (tt != 0 && tt != 8 && tt != 16 && tt != 32 && tt != 64)
-        return c->fail(FWI_EINVAL, "tile_tz %d (0, 8, 16, 32 or 64)", tt);
-    if (c->o.kernel_variant < 0 || c->o.kernel_variant > 5)
-        return c->fail(FWI_EINVAL, "kernel_variant %d (0..5)", c->o.kernel_variant);
+    if (tt < 0 || (c->o.kernel_variant != 6 && tt != 0 && tt != 8 && tt != 16 && tt != 32 && tt != 64))
+        return c->fail(FWI_EINVAL, "tile_tz %d (0, 8, 16, 32 or 64; variant 6: 0 or its row block times 1 or 2)",
+                       tt);
+    if (c->o.kernel_variant < 0 || c->o.kernel_variant > 6)
+        return c->fail(FWI_EINVAL, "kernel_variant %d (0..6)", c->o.kernel_variant);
     if (c->o.work_order < 0 || c->o.work_order > 3)
         return c->fail(FWI_EINVAL, "work_order %d (0..3)", c->o.work_order);
     if (g->snap_dt < 1 || g->snap_dz < 1 || g->snap_dx < 1) return c->fail(FWI_EINVAL, "snap strides < 1");

@@ -1,0 +1,97 @@
+// psv_device.h -- device helpers shared by the stencil translation units.
+#pragma once
+
+#include "psv_internal.h"
+
+namespace psv {
+namespace {
+
+// column / row on the strided snapshot box
+__device__ __forceinline__ bool ix_on_lattice(const KArgs &a, int x) {
+    return x >= a.snap_x1 && x <= a.snap_x2 && (x - a.snap_x1) % a.snap_dx == 0;
+}
+__device__ __forceinline__ bool iz_on_lattice(const KArgs &a, int z) {
+    return z >= a.snap_z1 && z <= a.snap_z2 && (z - a.snap_z1) % a.snap_dz == 0;
+}
+
+// Per-step switches of a launch: from the arguments, or, in a replayed graph,
+// from the step table at the time index the graph's base counter gives.
+struct StepView {
+    int it, grad, store;
+    long long off;   // slab offset in the snapshot arrays
+};
+__device__ __forceinline__ StepView step_view(const KArgs &a, int it) {
+    if (!a.si) return {it, a.grad, a.store, 0};
+    const int t = *a.si_base + a.si_dir * a.si_k;
+    const StepInfo d = a.si[t];
+    return {t, a.grad && d.grad, a.store && d.store, d.off};
+}
+
+// Workgroups are dispatched round-robin over the 8 XCDs (each with its own L2).
+// The 1-D launch index is remapped so that every XCD owns one contiguous range
+// of work, ordered tile-major and shot-minor: the B shots of a tile run back to
+// back on one XCD (the tile's media and CPML profiles stay in that L2), then the
+// vertically adjacent segment (its z halo rows are the previous tile's rows).
+template <int TZ, int OUTW = kBX>
+__device__ __forceinline__ void tile_of(const KArgs &a, int N, int &bx, int &by, int &s) {
+    const int gx = (a.nx - 2 * N + OUTW - 1) / OUTW;
+    const int gy = ((a.nz - 2 * N + TZ - 1) / TZ + kBY - 1) / kBY;
+    const int n = gx * gy * a.B;
+    const int L = blockIdx.x, xcd = L & 7, loc = L >> 3;
+    const int q = n >> 3, r = n & 7;
+    const int W = xcd * q + min(xcd, r) + loc;
+    if (a.order == 0) {          // shot fastest, then z segments, then x tiles
+        s = W % a.B;
+        const int t = W / a.B;
+        by = t % gy;
+        bx = t / gy;
+    } else if (a.order == 1) {   // shot fastest, then x tiles, then z segments
+        s = W % a.B;
+        const int t = W / a.B;
+        bx = t % gx;
+        by = t / gx;
+    } else {                     // x tiles fastest, then z segments, shots outermost
+        const int t = W % (gx * gy);
+        s = W / (gx * gy);
+        bx = t % gx;
+        by = t / gx;
+    }
+}
+
+// value of v in lane (lane + d); d is a compile-time constant after unrolling
+__device__ __forceinline__ float lane_off(float v, int lane, int d) {
+    return d == 0 ? v : __shfl(v, lane + d);
+}
+
+// Wave-uniform base pointer plus a 32-bit byte offset per lane: the global
+// load / store takes its saddr + voffset form.
+template <class T>
+__device__ __forceinline__ T ld_at(const T *base, uint32_t off) {
+    return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(base) + off);
+}
+template <class T>
+__device__ __forceinline__ void st_at(T *base, uint32_t off, T v) {
+    *reinterpret_cast<T *>(reinterpret_cast<char *>(base) + off) = v;
+}
+__device__ __forceinline__ int wave_uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
+
+// z-strip test and strip row (the psi_z row) of a wave-uniform row
+__device__ __forceinline__ bool in_zstrip(const ZTab &zt, int z) { return z < zt.lo || z >= zt.hi; }
+__device__ __forceinline__ int strip_row(const ZTab &zt, int N, int z) {
+    return z < zt.lo ? z - N : zt.lo - N + (z - zt.hi);
+}
+
+// velocity-kernel launch modes of variants 5 and 6 (template flags): We += v^2,
+// adjoint (u += dt v, correlation on the step's switch), snapshot store, and the
+// misfit-derivative sums (read and written when the arguments' gform is 1; without
+// the flag gform must be 0)
+constexpr int kVelEnergy = 1, kVelAdjoint = 2, kVelStore = 4, kVelMisfit = 8;
+
+// the velocity launch mode of a set of arguments
+__host__ __device__ inline int vel_mode(const KArgs &a) {
+    return (a.energy ? kVelEnergy : 0) | (a.displ ? kVelAdjoint : 0) | (a.store ? kVelStore : 0) |
+           (a.displ && a.gform ? kVelMisfit : 0);
+}
+
+}  // namespace
+}  // namespace psv

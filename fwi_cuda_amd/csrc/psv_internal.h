@@ -137,8 +137,17 @@ struct RecArgs {
 // var 1: x taps as shifted global loads; var 2: x taps from neighbouring lanes, next row
 // prefetched; var 3: var 2 at six waves per SIMD; var 5: var 2 with wave-uniform addressing
 // and ordered loads (needs zt)
-void launch_stress(const KArgs &a, int N, int tz, int var, hipStream_t st, const ZTab *zt = nullptr);
-void launch_velocity(const KArgs &a, int N, int tz, int var, hipStream_t st, const ZTab *zt = nullptr);
+void launch_stress(const KArgs &a, int N, int tz, int var, hipStream_t st, const ZTab *zt = nullptr,
+                   int depth = 1);
+void launch_velocity(const KArgs &a, int N, int tz, int var, hipStream_t st, const ZTab *zt = nullptr,
+                     int depth = 1);
+// Variant 6 (psv_stencil6.hip): every load issued `depth` (1 or 2) rows ahead.  Its
+// rows per segment are one or two blocks of v6_block(N, depth) rows; v6_rows gives
+// the segment height launch6 uses for a requested tz.  which: 0 stress, 1 velocity;
+// false (nothing launched) for a velocity launch mode it has no kernel for.
+constexpr int v6_block(int N, int D) { return ((2 * N + D + D) / (D + 1)) * (D + 1); }
+int v6_rows(int N, int depth, int tz);
+bool launch6(int which, const KArgs &a, const ZTab &zt, int N, int depth, int tz, hipStream_t st);
 // stress, top free surface and velocity of one time step in one pass (kernel variant 4);
 // tz = rows per segment (32, 64), depth = rows loaded ahead (1 is the one instantiated)
 void launch_step(const KArgs &a, int N, int tz, int depth, hipStream_t st);

@@ -17,7 +17,7 @@
 // served by L1/L2).  CPML memory variables exist only in the active strips
 // (zmap / xmap >= 0); elsewhere the CPML update is the identity, as it is in
 // the reference where a = 0, b = 1, K = 1 outside the strips.
-#include "psv_internal.h"
+#include "psv_device.h"
 
 namespace psv {
 
@@ -25,27 +25,6 @@ namespace {
 
 __device__ __forceinline__ size_t fidx(const KArgs &a, int s, int z, int x) {
     return (size_t)s * a.fs + (size_t)z * a.P + x;
-}
-
-// column / row on the strided snapshot box
-__device__ __forceinline__ bool ix_on_lattice(const KArgs &a, int x) {
-    return x >= a.snap_x1 && x <= a.snap_x2 && (x - a.snap_x1) % a.snap_dx == 0;
-}
-__device__ __forceinline__ bool iz_on_lattice(const KArgs &a, int z) {
-    return z >= a.snap_z1 && z <= a.snap_z2 && (z - a.snap_z1) % a.snap_dz == 0;
-}
-
-// Per-step switches of a launch: from the arguments, or, in a replayed graph,
-// from the step table at the time index the graph's base counter gives.
-struct StepView {
-    int it, grad, store;
-    long long off;   // slab offset in the snapshot arrays
-};
-__device__ __forceinline__ StepView step_view(const KArgs &a, int it) {
-    if (!a.si) return {it, a.grad, a.store, 0};
-    const int t = *a.si_base + a.si_dir * a.si_k;
-    const StepInfo d = a.si[t];
-    return {t, a.grad && d.grad, a.store && d.store, d.off};
 }
 
 // Gradient sums at snapshot cell q (offset in the batch's [B][snz][snx] arrays).
@@ -68,36 +47,6 @@ __device__ __forceinline__ void grad_accumulate(const KArgs &a, size_t q, float 
     }
 }
 
-// Workgroups are dispatched round-robin over the 8 XCDs (each with its own L2).
-// The 1-D launch index is remapped so that every XCD owns one contiguous range
-// of work, ordered tile-major and shot-minor: the B shots of a tile run back to
-// back on one XCD (the tile's media and CPML profiles stay in that L2), then the
-// vertically adjacent segment (its z halo rows are the previous tile's rows).
-template <int TZ, int OUTW = kBX>
-__device__ __forceinline__ void tile_of(const KArgs &a, int N, int &bx, int &by, int &s) {
-    const int gx = (a.nx - 2 * N + OUTW - 1) / OUTW;
-    const int gy = ((a.nz - 2 * N + TZ - 1) / TZ + kBY - 1) / kBY;
-    const int n = gx * gy * a.B;
-    const int L = blockIdx.x, xcd = L & 7, loc = L >> 3;
-    const int q = n >> 3, r = n & 7;
-    const int W = xcd * q + min(xcd, r) + loc;
-    if (a.order == 0) {          // shot fastest, then z segments, then x tiles
-        s = W % a.B;
-        const int t = W / a.B;
-        by = t % gy;
-        bx = t / gy;
-    } else if (a.order == 1) {   // shot fastest, then x tiles, then z segments
-        s = W % a.B;
-        const int t = W / a.B;
-        bx = t % gx;
-        by = t / gx;
-    } else {                     // x tiles fastest, then z segments, shots outermost
-        const int t = W % (gx * gy);
-        s = W / (gx * gy);
-        bx = t % gx;
-        by = t / gx;
-    }
-}
 
 template <int N, int TZ>
 __global__ __launch_bounds__(kBX *kBY) void k_stress(const KArgs a) {
@@ -304,11 +253,6 @@ __global__ __launch_bounds__(kBX *kBY) void k_velocity(const KArgs a) {
 // loads of row z + 1 (queue entries, stresses, media) are issued before row z
 // is computed, which doubles the loads in flight per wavefront.
 // ---------------------------------------------------------------------------
-
-// value of v in lane (lane + d); d is a compile-time constant after unrolling
-__device__ __forceinline__ float lane_off(float v, int lane, int d) {
-    return d == 0 ? v : __shfl(v, lane + d);
-}
 
 template <int N, int TZ, int MINW = 1>
 __global__ __launch_bounds__(kBX *kBY, MINW) void k_stress2(const KArgs a) {
@@ -564,26 +508,6 @@ __global__ __launch_bounds__(kBX *kBY, MINW) void k_velocity2(const KArgs a) {
 // flight across both.  All lanes load (halo lanes read clamped, valid cells) and
 // only owned lanes store.  The velocity kernel's launch modes are template flags.
 // ---------------------------------------------------------------------------
-constexpr int kVelEnergy = 1, kVelAdjoint = 2, kVelStore = 4;
-
-template <class T>
-__device__ __forceinline__ T ld_at(const T *base, uint32_t off) {
-    return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(base) + off);
-}
-template <class T>
-__device__ __forceinline__ void st_at(T *base, uint32_t off, T v) {
-    *reinterpret_cast<T *>(reinterpret_cast<char *>(base) + off) = v;
-}
-__device__ __forceinline__ int wave_uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
-__device__ __forceinline__ float wave_uniform(float v) {
-    return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
-}
-
-// z-strip test and strip row (the psi_z row) of a wave-uniform row
-__device__ __forceinline__ bool in_zstrip(const ZTab &zt, int z) { return z < zt.lo || z >= zt.hi; }
-__device__ __forceinline__ int strip_row(const ZTab &zt, int N, int z) {
-    return z < zt.lo ? z - N : zt.lo - N + (z - zt.hi);
-}
 
 template <int N, int TZ>
 __global__ __launch_bounds__(kBX *kBY) void k_stress5(const KArgs a, const ZTab zt) {
@@ -784,8 +708,10 @@ __device__ __forceinline__ void velocity5_rows(const KArgs &a, const ZTab &zt, c
             Vz = ld_at(a.sn_vz + so, j4); Vx = ld_at(a.sn_vx + so, j4);
             gl = ld_at(med(a.lam, 0), vo); gm = ld_at(med(a.mu, 0), vo);
             g0 = ld_at(a.g_lam + qrow, j4); g1 = ld_at(a.g_mu + qrow, j4); g2 = ld_at(a.g_rho + qrow, j4);
-            g3 = ld_at(a.g_ext + qrow, j4);   // the misfit-derivative sums (gform 1) are always allocated
-            g4 = ld_at(a.g_ext + a.g_ext_stride + qrow, j4);
+            if constexpr ((MODE & kVelMisfit) != 0) {
+                g3 = ld_at(a.g_ext + qrow, j4);
+                g4 = ld_at(a.g_ext + a.g_ext_stride + qrow, j4);
+            }
         }
         const uint32_t v2 = min(vo + 2 * P4, vlast);
         hzz = ld_at(szzq, v2);
@@ -852,7 +778,7 @@ __device__ __forceinline__ void velocity5_rows(const KArgs &a, const ZTab &zt, c
                 const float s2 = 0.25f * im * (Szz - Sxx) * (c_zz - c_xx);
                 const float s3 = im * (Szx * szxc);
                 st_at(a.g_lam + qrow, j4, g0 + a.gw * s1);
-                if (!a.gform) {
+                if (!((MODE & kVelMisfit) != 0 && a.gform)) {
                     st_at(a.g_mu + qrow, j4, g1 + a.gw * (s3 + s1 + s2));
                     st_at(a.g_rho + qrow, j4, g2 + a.gw * (ux * Vx + uz * Vz));
                 } else {
@@ -1339,21 +1265,27 @@ static int two_kernel_tz(int tz, int var) {
     }
 
 // variant 5's velocity launch modes: the ones the passes run get their own code
-static void launch_velocity5(const KArgs &a, const ZTab *zt, int N, int tz, dim3 grid, dim3 block,
+// (false: a launch mode it has no kernel for)
+static bool launch_velocity5(const KArgs &a, const ZTab *zt, int N, int tz, dim3 grid, dim3 block,
                              hipStream_t st) {
-    const int mode = (a.energy ? kVelEnergy : 0) | (a.displ ? kVelAdjoint : 0) | (a.store ? kVelStore : 0);
-    switch (mode) {
-        case 0: PSV_N5_CASES(k_velocity5, , 0) break;                                   // recomputation
-        case kVelEnergy: PSV_N5_CASES(k_velocity5, , kVelEnergy) break;                 // forward
-        case kVelStore: PSV_N5_CASES(k_velocity5, , kVelStore) break;                   // recomputation, stored
-        case kVelEnergy | kVelStore: PSV_N5_CASES(k_velocity5, , kVelEnergy | kVelStore) break;
-        case kVelEnergy | kVelAdjoint: PSV_N5_CASES(k_velocity5, , kVelEnergy | kVelAdjoint) break;  // adjoint
-        default: PSV_N5_CASES(k_velocity5, , kVelEnergy | kVelAdjoint | kVelStore) break;
+    switch (vel_mode(a)) {
+        case 0: PSV_N5_CASES(k_velocity5, , 0) return true;                                   // recomputation
+        case kVelEnergy: PSV_N5_CASES(k_velocity5, , kVelEnergy) return true;                 // forward
+        case kVelStore: PSV_N5_CASES(k_velocity5, , kVelStore) return true;                   // recomputation, stored
+        case kVelEnergy | kVelStore: PSV_N5_CASES(k_velocity5, , kVelEnergy | kVelStore) return true;
+        case kVelEnergy | kVelAdjoint: PSV_N5_CASES(k_velocity5, , kVelEnergy | kVelAdjoint) return true;
+        case kVelEnergy | kVelAdjoint | kVelMisfit:
+            PSV_N5_CASES(k_velocity5, , kVelEnergy | kVelAdjoint | kVelMisfit) return true;
+        default: return false;
     }
 }
 
-void launch_stress(const KArgs &a, int N, int tz, int var, hipStream_t st, const ZTab *zt) {
-    if (var == 5 && !zt) var = 2;   // variant 5 needs the strip-row table
+void launch_stress(const KArgs &a, int N, int tz, int var, hipStream_t st, const ZTab *zt, int depth) {
+    if (var >= 5 && !zt) var = 2;   // variants 5 and 6 need the strip-row table
+    if (var == 6) {
+        launch6(0, a, *zt, N, depth, tz, st);
+        return;
+    }
     tz = two_kernel_tz(tz, var);
     const dim3 grid = stencil_grid(a, N, tz, var), block(kBX, kBY);
     if (var == 5) {
@@ -1367,13 +1299,21 @@ void launch_stress(const KArgs &a, int N, int tz, int var, hipStream_t st, const
     }
 }
 
-void launch_velocity(const KArgs &a, int N, int tz, int var, hipStream_t st, const ZTab *zt) {
-    if (var == 5 && (!zt || (a.grad && !a.displ))) var = 2;   // variant 5 correlates in adjoint launches only
+void launch_velocity(const KArgs &a, int N, int tz, int var, hipStream_t st, const ZTab *zt, int depth) {
+    // variants 5 and 6 need the strip-row table and correlate in adjoint launches only
+    if (var >= 5 && (!zt || (a.grad && !a.displ))) var = 2;
+    if (var == 6) {
+        if (launch6(1, a, *zt, N, depth, tz, st)) return;
+        var = 2;   // a launch mode variant 6 has no kernel for
+    }
+    if (var == 5) {
+        const int tz5 = two_kernel_tz(tz, 5);
+        if (launch_velocity5(a, zt, N, tz5, stencil_grid(a, N, tz5, 5), dim3(kBX, kBY), st)) return;
+        var = 2;   // a launch mode variant 5 has no kernel for
+    }
     tz = two_kernel_tz(tz, var);
     const dim3 grid = stencil_grid(a, N, tz, var), block(kBX, kBY);
-    if (var == 5) {
-        launch_velocity5(a, zt, N, tz, grid, block, st);
-    } else if (var == 3) {
+    if (var == 3) {
         PSV_N_CASES(k_velocity2, , 6)
     } else if (var == 2) {
         PSV_N_CASES(k_velocity2, )

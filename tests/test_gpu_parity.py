@@ -85,7 +85,7 @@ def test_shot_gradient_matches_reference_golden(psv, name):
 
 
 @pytest.mark.parametrize("surface", [True, False])
-@pytest.mark.parametrize("variant", [1, 2, 4, 5])
+@pytest.mark.parametrize("variant", [1, 2, 4, 5, 6])
 def test_order8_forward_vs_oracle(psv, O, surface, variant):
     c = cases.small_case(O, surface=surface, nshot=2)
     med = O.mat_av2(c)
@@ -100,7 +100,7 @@ def test_order8_forward_vs_oracle(psv, O, surface, variant):
         assert e < 1e-4
 
 
-@pytest.mark.parametrize("variant", [1, 2, 4, 5])
+@pytest.mark.parametrize("variant", [1, 2, 4, 5, 6])
 def test_order8_gradient_vs_oracle(psv, O, variant):
     c = cases.small_case(O, nshot=2, nt=450)
     med_true = O.mat_av2(c)
@@ -126,7 +126,7 @@ def test_order8_gradient_vs_oracle(psv, O, variant):
             assert e < 2e-3
 
 
-@pytest.mark.parametrize("variant", [1, 4, 5])
+@pytest.mark.parametrize("variant", [1, 4, 5, 6])
 def test_checkpointed_equals_full_storage(psv, O, variant):
     c = cases.small_case(O, nshot=2, nt=300, snap_dt=2)
     med_true = O.mat_av2(c)
@@ -148,7 +148,7 @@ def test_checkpointed_equals_full_storage(psv, O, variant):
             np.testing.assert_array_equal(G[i], ref[1][i], err_msg=str(key))
 
 
-@pytest.mark.parametrize("variant", [1, 4, 5])
+@pytest.mark.parametrize("variant", [1, 4, 5, 6])
 def test_batched_uneven_equals_per_shot(psv, O, variant):
     c = cases.small_case(O, nshot=4, nt=260, uneven=True)
     med_true = O.mat_av2(c)
@@ -260,19 +260,24 @@ def test_shared_cells_and_empty_shot(psv, O):
 
 
 @pytest.mark.parametrize("fdorder", [2, 8])
-@pytest.mark.parametrize("tz", [16, 32])
-def test_uniform_addressing_variant_matches_variant2(psv, O, fdorder, tz):
-    """Variant 5 (wave-uniform addressing, ordered loads, strip profiles from the
-    kernel arguments, halo stores into the pad column) against variant 2, the
-    same stencil: seismograms, stored snapshots, the wavefield, the shot
-    gradient and the misfit-derivative sums, over uneven shots in one batch."""
+@pytest.mark.parametrize("variant,blocks", [(5, 16), (5, 32), (6, 1), (6, 2)])
+def test_uniform_addressing_variant_matches_variant2(psv, O, fdorder, variant, blocks):
+    """Variants 5 (wave-uniform addressing, ordered loads, strip profiles from the
+    kernel arguments, halo stores into the pad column) and 6 (the same, every
+    load one row ahead in register rings; segments of one or two blocks of
+    fdorder + 2 rows) against variant 2, the same stencil: seismograms, stored
+    snapshots, the wavefield, the shot gradient and the misfit-derivative sums,
+    over uneven shots in one batch.  The 72-row test grid leaves a short last
+    segment, so the guarded remainder rows run too."""
     c = cases.small_case(O, nshot=3, nt=240, fdorder=fdorder, uneven=True)
     med_true = O.mat_av2(c)
     obs = cases.per_shot_obs(c, O, med_true)
     lam0, mu0, rho0 = cases.start_model(c)
+    tz = blocks if variant == 5 else blocks * (fdorder + 2)
     out = {}
-    for v in (2, 5):
-        with psv.Simulation(c, kernel_variant=v, tile_tz=tz, storage=psv._lib.STORE_FULL) as sim:
+    for v in (2, variant):
+        with psv.Simulation(c, kernel_variant=v, tile_tz=tz if v == variant else 16,
+                            storage=psv._lib.STORE_FULL) as sim:
             seis = sim.forward()
             wf = sim.wavefield(1, "szx")
             _, snaps = sim.forward_snapshots(1)
@@ -283,18 +288,20 @@ def test_uniform_addressing_variant_matches_variant2(psv, O, fdorder, tz):
             adj = [np.asarray(o, np.float64) * 1e-3 for o in obs[0]]
             mg = sim.misfit_gradient(*adj, shot0=0, nshots=1)
             out[v] = (seis, wf, snaps, m, G, mg)
+    got, ref = out[variant], out[2]
+    tag = f"order {fdorder} variant {variant} tz {tz}"
     for i in range(2):
-        e = cases.rel_l2(out[5][0][i], out[2][0][i])
-        _print(f"order {fdorder} tz {tz}: variant 5 vs 2 seismogram {i} rel-L2 {e:.2e}")
+        e = cases.rel_l2(got[0][i], ref[0][i])
+        _print(f"{tag}: seismogram {i} rel-L2 vs variant 2 {e:.2e}")
         assert e < 1e-6
-    assert cases.rel_l2(out[5][1], out[2][1]) < 1e-6
-    for k in out[2][2]:
-        assert cases.rel_l2(out[5][2][k], out[2][2][k]) < 1e-6, k
-    np.testing.assert_allclose(out[5][3], out[2][3], rtol=1e-5)
+    assert cases.rel_l2(got[1], ref[1]) < 1e-6
+    for k in ref[2]:
+        assert cases.rel_l2(got[2][k], ref[2][k]) < 1e-6, k
+    np.testing.assert_allclose(got[3], ref[3], rtol=1e-5)
     for i in range(3):
-        e = cases.rel_l2(out[5][4][i], out[2][4][i])
-        e2 = cases.rel_l2(out[5][5][i], out[2][5][i])
-        _print(f"order {fdorder} tz {tz}: variant 5 vs 2 gradient {i} rel-L2 {e:.2e}, misfit derivative {e2:.2e}")
+        e = cases.rel_l2(got[4][i], ref[4][i])
+        e2 = cases.rel_l2(got[5][i], ref[5][i])
+        _print(f"{tag}: gradient {i} rel-L2 vs variant 2 {e:.2e}, misfit derivative {e2:.2e}")
         assert e < 1e-4 and e2 < 1e-4
 
 
@@ -344,7 +351,7 @@ def test_work_orders_agree_bitwise(psv, O):
             np.testing.assert_array_equal(ux, ref[2])
 
 
-@pytest.mark.parametrize("variant", [1, 4, 5])
+@pytest.mark.parametrize("variant", [1, 4, 5, 6])
 @pytest.mark.parametrize("storage", ["full", "ckpt"])
 def test_graph_replay_equals_eager(psv, O, variant, storage):
     """use_graph: chunks of time steps captured once per propagation and replayed

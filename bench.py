@@ -29,6 +29,7 @@ from __future__ import annotations
 import argparse
 import json
 import os
+import subprocess
 import sys
 import time
 from pathlib import Path
@@ -89,7 +90,39 @@ def kernel_instantiation(kind, tune_log, N):
     base = {"stress": "k_stress", "velocity": "k_velocity"}[kind]
     if var == 1:
         return f"{base}<{N}, {tz}>"
+    if var == 5:   # the velocity kernel's launch modes share the prefix
+        return f"{base}5<{N}, {tz}>" if kind == "stress" else f"{base}5<{N}, {tz}, "
+    if var == 6:   # segments of tz = M blocks of 2N + 2 rows, loads one row ahead
+        m = max(1, tz // (2 * N + 2))
+        return f"{base}6<{N}, 1, {m}>" if kind == "stress" else f"{base}6<{N}, 1, {m}, "
     return f"{base}2<{N}, {tz}, {1 if var == 2 else 6}>"
+
+
+def copy_bandwidth(nbytes=1 << 30, reps=10):
+    """A measured ceiling for the roofline: GB/s of an elementwise multiply (one load and
+    one store per element, torch's kernel) over two nbytes buffers on this GPU, or None."""
+    try:
+        import torch
+        if not torch.cuda.is_available():
+            log("copy probe skipped: torch sees no GPU")
+            return None
+        n = nbytes // 4
+        src = torch.ones(n, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            torch.mul(src, 2.0, out=dst)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            torch.mul(src, 2.0, out=dst)
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / reps
+        del src, dst
+        return 8.0 * n / (ms / 1e3) / 1e9
+    except Exception as e:   # the probe is a reported reference, not the measurement
+        log(f"copy probe failed: {e!r}")
+        return None
 
 
 def measured_traffic(inst):
@@ -186,7 +219,12 @@ def main():
     ap.add_argument("--no-graph", action="store_true", help="skip the hipGraph-replay gradient")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a device copy of MIB MiB five times (calibrates rocprof byte counters)")
+    ap.add_argument("--copy-bandwidth", action="store_true",
+                    help="only measure the elementwise-multiply bandwidth and print it (run as a child process)")
     args = ap.parse_args()
+    if args.copy_bandwidth:
+        print(json.dumps({"copy_GBps": copy_bandwidth()}))
+        return
     if args.copy_probe:
         import torch
         n = args.copy_probe * (1 << 20) // 4
@@ -204,6 +242,17 @@ def main():
         print(f"bench.py measures one GPU: --gpus {args.gpus} (WORLD_SIZE {world}) is not run. Shots are "
               "independent; the multi-GPU design shards shots and all-reduces the gradient once.", file=sys.stderr)
         sys.exit(2)
+
+    # The measured ceiling comes from a child process that uses only torch, started before
+    # this process touches the GPU: torch's bundled HIP runtime does not initialise in a
+    # process where the library's runtime already has.
+    copy = None
+    try:
+        r = subprocess.run([sys.executable, str(Path(__file__).resolve()), "--copy-bandwidth"],
+                           capture_output=True, text=True, timeout=300)
+        copy = json.loads(r.stdout.strip().splitlines()[-1])["copy_GBps"] if r.returncode == 0 else None
+    except Exception as e:
+        log(f"copy probe failed: {e!r}")
 
     from fwi_cuda_amd import _lib, psv
 
@@ -322,6 +371,10 @@ def main():
     per_kernel = {k: {"achieved_GBps": round(v[0] / (v[1] / 1e3) / 1e9, 1) if v[1] else None,
                       "launches": v[2], "total_ms": round(v[1], 1)} for k, v in kern.items()}
     sim.close()
+    if copy:
+        roofline["copy_GBps"] = round(copy, 1)
+        roofline["copy_probe"] = "torch elementwise multiply, 1 GiB in, 1 GiB out, this GPU"
+        roofline["frac_of_copy"] = round(achieved / copy, 4)
 
     # the same gradient with chunks of time steps captured in hipGraphs and replayed
     # (per-kernel events do not exist inside a graph: wall clock only)

@@ -1,0 +1,79 @@
+"""Checkpoint-budget sweep of one gradient, through the C ABI.
+
+    python -m fwi_cuda_amd.ckpt_sweep --config C4 --budgets 240,120,60,30
+
+For each device-memory budget (GB) the planner picks the batch of shots and
+the checkpoint interval that fit; the same gradient (observed data modelled
+from the true model, start model = the true model smoothed) is timed and
+checked against the first budget's result, which it must equal bit for bit
+(checkpointed recomputation replays the forward exactly).  One line per
+budget: plan, device bytes, wall time, useful Gcell-updates/s.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import sys
+import time
+
+import numpy as np
+
+from . import _lib, configs, psv
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
+    ap.add_argument("--config", default="C4", choices=tuple(configs.BUILDERS))
+    ap.add_argument("--budgets", default="240,120,60,30", help="device-memory budgets in GB")
+    ap.add_argument("--nt", type=int, default=0)
+    ap.add_argument("--nshot", type=int, default=0)
+    ap.add_argument("--json", default="")
+    args = ap.parse_args(argv)
+    kw = {}
+    if args.nt:
+        kw["nt"] = args.nt
+    if args.nshot:
+        kw["nshot"] = args.nshot
+    from scipy.ndimage import gaussian_filter
+    c = configs.BUILDERS[args.config](**kw)
+    start = {k: gaussian_filter(c[k], 20.0, mode="nearest") for k in ("lam", "mu", "rho")}
+    nz, nx, nt, nshot = int(c["nz"]), int(c["nx"]), int(c["nt"]), int(c["nshot"])
+    useful = 2.0 * nz * nx * nt * nshot
+    print(f"{args.config}: {nz}x{nx}, nt {nt}, {nshot} shots", flush=True)
+
+    t0 = time.perf_counter()
+    with psv.Simulation(c) as sim:
+        obs = sim.forward()
+    print(f"observed data: {time.perf_counter() - t0:.1f} s", flush=True)
+
+    rows, ref = [], None
+    for gb in (float(v) for v in args.budgets.split(",")):
+        with psv.Simulation(c, storage=_lib.STORE_CHECKPOINT, mem_budget=int(gb * 1e9)) as sim:
+            plan = sim.plan()
+            sim.set_model(start["lam"], start["mu"], start["rho"])
+            sim.set_observed(*obs)
+            t0 = time.perf_counter()
+            m = sim.gradient()
+            sim.sync()
+            wall = time.perf_counter() - t0
+            G = sim.get_gradient()
+        same = None
+        if ref is None:
+            ref = (m, G)
+        else:
+            same = bool(np.array_equal(m, ref[0]) and all(np.array_equal(a, b) for a, b in zip(G, ref[1])))
+        row = {"budget_GB": gb, **plan, "seconds": round(wall, 2), "Gcell_per_s": round(useful / wall / 1e9, 3),
+               "identical_to_first": same}
+        rows.append(row)
+        print(json.dumps(row), flush=True)
+        if same is False:
+            print("gradient differs from the first budget's", flush=True)
+            return 1
+    if args.json:
+        with open(args.json, "w") as f:
+            json.dump({"config": args.config, "rows": rows}, f, indent=1)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -100,6 +100,20 @@ def make_case(*, nt, nz, nx, dt, dz, dx, fdorder, lam, mu, rho, z_src, x_src, sh
     return c
 
 
+def lame_to_velocity_gradient(g_lam, g_mu, g_rho, lam, mu, rho):
+    """Gradient with respect to (cp, cs, rho) from the gradient with respect to
+    (lam, mu, rho), by the chain rule through lam = rho (cp^2 - 2 cs^2), mu = rho cs^2:
+    dJ/dcp = 2 rho cp dJ/dlam, dJ/dcs = 2 rho cs (dJ/dmu - 2 dJ/dlam),
+    dJ/drho|cp,cs = (cp^2 - 2 cs^2) dJ/dlam + cs^2 dJ/dmu + dJ/drho."""
+    lam, mu, rho = (np.asarray(v, np.float64) for v in (lam, mu, rho))
+    cp = np.sqrt((lam + 2.0 * mu) / rho)
+    cs = np.sqrt(mu / rho)
+    g_cp = 2.0 * rho * cp * g_lam
+    g_cs = 2.0 * rho * cs * (g_mu - 2.0 * g_lam)
+    g_r = (cp * cp - 2.0 * cs * cs) * g_lam + cs * cs * g_mu + g_rho
+    return g_cp, g_cs, g_r
+
+
 def acquisition_csr(case: dict):
     """Sources grouped by shot (stable in source order, src_shot_to_fire of
     src/h_io_PSV.cpp:143-145) and receivers per shot.  Without ``rec_off`` every

@@ -85,7 +85,7 @@ def test_shot_gradient_matches_reference_golden(psv, name):
 
 
 @pytest.mark.parametrize("surface", [True, False])
-@pytest.mark.parametrize("variant", [1, 2, 4, 5, 6])
+@pytest.mark.parametrize("variant", [1, 2, 4, 5, 6, 7])
 def test_order8_forward_vs_oracle(psv, O, surface, variant):
     c = cases.small_case(O, surface=surface, nshot=2)
     med = O.mat_av2(c)
@@ -100,7 +100,7 @@ def test_order8_forward_vs_oracle(psv, O, surface, variant):
         assert e < 1e-4
 
 
-@pytest.mark.parametrize("variant", [1, 2, 4, 5, 6])
+@pytest.mark.parametrize("variant", [1, 2, 4, 5, 6, 7])
 def test_order8_gradient_vs_oracle(psv, O, variant):
     c = cases.small_case(O, nshot=2, nt=450)
     med_true = O.mat_av2(c)
@@ -126,7 +126,7 @@ def test_order8_gradient_vs_oracle(psv, O, variant):
             assert e < 2e-3
 
 
-@pytest.mark.parametrize("variant", [1, 4, 5, 6])
+@pytest.mark.parametrize("variant", [1, 4, 5, 6, 7])
 def test_checkpointed_equals_full_storage(psv, O, variant):
     c = cases.small_case(O, nshot=2, nt=300, snap_dt=2)
     med_true = O.mat_av2(c)
@@ -148,7 +148,37 @@ def test_checkpointed_equals_full_storage(psv, O, variant):
             np.testing.assert_array_equal(G[i], ref[1][i], err_msg=str(key))
 
 
-@pytest.mark.parametrize("variant", [1, 4, 5, 6])
+def test_c4_window_checkpointed_equals_full_storage(psv, O):
+    """The C4 model (15 layers and a dipping dyke, order 8) cropped to a 1001 x 2001
+    window, two shots, 1600 steps (the 100 Hz wavelet peaks at step ~620): checkpointed
+    recomputation at two intervals against full snapshot storage of the same window
+    (64 GB of snapshots per shot), bit for bit (misfits and gradient)."""
+    from scipy.ndimage import gaussian_filter
+    from fwi_cuda_amd import configs
+    c = configs.c4(nt=1600, nshot=2, nz=1001, nx=2001)
+    start = [gaussian_filter(c[k], 20.0, mode="nearest") for k in ("lam", "mu", "rho")]
+    with psv.Simulation(c) as sim:
+        obs = sim.forward()
+    res = {}
+    for storage, interval in (("full", 0), ("ckpt", 26), ("ckpt", 100)):
+        st = psv._lib.STORE_FULL if storage == "full" else psv._lib.STORE_CHECKPOINT
+        with psv.Simulation(c, storage=st, ckpt_interval=interval) as sim:
+            plan = sim.plan()
+            assert plan["storage"] == ("full" if storage == "full" else "checkpoint")
+            sim.set_model(*start)
+            sim.set_observed(*obs)
+            m = sim.gradient()
+            res[(storage, interval)] = (m, sim.get_gradient())
+            _print(f"C4 window {storage} {interval}: plan {plan}, misfits {m}")
+    ref = res[("full", 0)]
+    assert np.all(ref[0] > 0)
+    for key, (m, G) in res.items():
+        np.testing.assert_array_equal(m, ref[0])
+        for i in range(3):
+            np.testing.assert_array_equal(G[i], ref[1][i], err_msg=str(key))
+
+
+@pytest.mark.parametrize("variant", [1, 4, 5, 6, 7])
 def test_batched_uneven_equals_per_shot(psv, O, variant):
     c = cases.small_case(O, nshot=4, nt=260, uneven=True)
     med_true = O.mat_av2(c)
@@ -305,36 +335,46 @@ def test_uniform_addressing_variant_matches_variant2(psv, O, fdorder, variant, b
         assert e < 1e-4 and e2 < 1e-4
 
 
-def test_one_pass_step_matches_two_kernel_step(psv, O):
-    """Variant 4 (stress, surface and velocity in one pass, two time levels)
-    against variant 1: seismograms, stored snapshots (the source cells are
-    rewritten by the injection) and the shot gradient.  Same arithmetic, so the
-    differences are FP32 contraction choices only."""
-    c = cases.small_case(O, nshot=2, nt=300)
+@pytest.mark.parametrize("variant,tz,fdorder", [(4, 0, 8), (7, 1, 8), (7, 2, 8), (7, 1, 2), (7, 2, 4)])
+def test_one_pass_step_matches_two_kernel_step(psv, O, variant, tz, fdorder):
+    """The one-pass variants (stress, surface and velocity in one pass, two time
+    levels; 7 with segments of tz blocks of fdorder + 2 rows) against variant 1:
+    seismograms, stored snapshots (the source cells are rewritten by the
+    injection), the shot gradient and the misfit-derivative sums, over uneven
+    shots in one batch.  Same arithmetic, so the differences are FP32 contraction
+    choices only.  The 72-row grid leaves a short last segment."""
+    c = cases.small_case(O, nshot=3, nt=300, fdorder=fdorder, uneven=True)
     med_true = O.mat_av2(c)
     obs = cases.per_shot_obs(c, O, med_true)
     lam0, mu0, rho0 = cases.start_model(c)
     out = {}
-    for v in (1, 4):
-        with psv.Simulation(c, kernel_variant=v, storage=psv._lib.STORE_FULL) as sim:
+    for v in (1, variant):
+        t = 0 if v != 7 else tz * (fdorder + 2)
+        with psv.Simulation(c, kernel_variant=v, tile_tz=t, storage=psv._lib.STORE_FULL) as sim:
             seis = sim.forward()
             _, snaps = sim.forward_snapshots(1)
             sim.set_model(lam0, mu0, rho0)
             sim.set_observed([o[0] for o in obs], [o[1] for o in obs])
             m = sim.gradient()
-            out[v] = (seis, snaps, m, sim.get_gradient())
+            G = sim.get_gradient()
+            adj = [np.asarray(o, np.float64) * 1e-3 for o in obs[0]]
+            mg = sim.misfit_gradient(*adj, shot0=0, nshots=1)
+            out[v] = (seis, snaps, m, G, mg)
+    got, ref = out[variant], out[1]
     for i in range(2):
-        e = cases.rel_l2(out[4][0][i], out[1][0][i])
-        _print(f"one-pass vs two-kernel seismogram {i}: rel-L2 {e:.2e}")
+        e = cases.rel_l2(got[0][i], ref[0][i])
+        _print(f"variant {variant} tz {tz} order {fdorder} vs variant 1: seismogram {i} rel-L2 {e:.2e}")
         assert e < 1e-6
-    for k in out[1][1]:
-        e = cases.rel_l2(out[4][1][k], out[1][1][k])
+    for k in ref[1]:
+        e = cases.rel_l2(got[1][k], ref[1][k])
         assert e < 1e-6, (k, e)
-    np.testing.assert_allclose(out[4][2], out[1][2], rtol=1e-5)
+    np.testing.assert_allclose(got[2], ref[2], rtol=1e-5)
     for i in range(3):
-        e = cases.rel_l2(out[4][3][i], out[1][3][i])
-        _print(f"one-pass vs two-kernel gradient {i}: rel-L2 {e:.2e}")
-        assert e < 1e-4
+        e = cases.rel_l2(got[3][i], ref[3][i])
+        e2 = cases.rel_l2(got[4][i], ref[4][i])
+        _print(f"variant {variant} tz {tz} order {fdorder} vs variant 1: gradient {i} rel-L2 {e:.2e}, "
+               f"misfit derivative {e2:.2e}")
+        assert e < 1e-4 and e2 < 1e-4
 
 
 def test_work_orders_agree_bitwise(psv, O):
@@ -351,7 +391,7 @@ def test_work_orders_agree_bitwise(psv, O):
             np.testing.assert_array_equal(ux, ref[2])
 
 
-@pytest.mark.parametrize("variant", [1, 4, 5, 6])
+@pytest.mark.parametrize("variant", [1, 4, 5, 6, 7])
 @pytest.mark.parametrize("storage", ["full", "ckpt"])
 def test_graph_replay_equals_eager(psv, O, variant, storage):
     """use_graph: chunks of time steps captured once per propagation and replayed
@@ -410,10 +450,11 @@ def test_c3_grid_gradient_vs_fp64_oracle(psv, O):
         assert eg < 1e-4, (k, eg)
 
 
-def test_one_pass_step_refused_where_it_does_not_apply(psv, O):
+@pytest.mark.parametrize("variant", [4, 7])
+def test_one_pass_step_refused_where_it_does_not_apply(psv, O, variant):
     c = cases.small_case(O, nshot=1, nt=20)
     c["isurf"] = np.array([int(c["fdorder"]) // 2, int(c["nz"]) - 20, 0, 0], np.int32)   # a bottom surface too
-    with psv.Simulation(c, kernel_variant=4) as sim:
+    with psv.Simulation(c, kernel_variant=variant) as sim:
         with pytest.raises(psv._lib.FwiError) as e:
             sim.forward()
         assert e.value.status == -1

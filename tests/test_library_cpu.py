@@ -157,3 +157,39 @@ def test_workload_builders(name, kw, shape, nshot):
     # identical inputs from the same seed
     c2 = configs.BUILDERS[name](**kw)
     np.testing.assert_array_equal(c["lam"], c2["lam"])
+
+
+def test_velocity_parameter_chain_rule():
+    """(lam, mu, rho) gradient -> (cp, cs, rho) gradient against central differences
+    of a nonlinear test functional J(lam, mu, rho) evaluated through cp, cs."""
+    from fwi_cuda_amd import psv
+    rng = np.random.default_rng(3)
+    cp = rng.uniform(2000, 4500, 50)
+    cs = cp / rng.uniform(1.6, 2.0, 50)
+    rho = rng.uniform(1800, 2600, 50)
+    w = rng.standard_normal((3, 50))
+
+    def lame(cp_, cs_, rho_):
+        mu_ = rho_ * cs_ ** 2
+        return rho_ * cp_ ** 2 - 2 * mu_, mu_, rho_
+
+    def J(l, m, r):
+        return np.sum(w[0] * np.sin(l / 1e10) + w[1] * (m / 1e10) ** 2 + w[2] * np.log(r))
+
+    l, m, r = lame(cp, cs, rho)
+    g_l = w[0] * np.cos(l / 1e10) / 1e10
+    g_m = w[1] * 2 * m / 1e20
+    g_r = w[2] / r
+    g_cp, g_cs, g_rho = psv.lame_to_velocity_gradient(g_l, g_m, g_r, l, m, r)
+    for k, (g, v) in enumerate(((g_cp, cp), (g_cs, cs), (g_rho, rho))):
+        h = 1e-4 * v
+        up, dn = [cp.copy(), cs.copy(), rho.copy()], [cp.copy(), cs.copy(), rho.copy()]
+        fd = np.empty(50)
+        for i in range(50):
+            up[k][i] += h[i]
+            dn[k][i] -= h[i]
+            fd[i] = (J(*lame(*up)) - J(*lame(*dn))) / (2 * h[i])
+            up[k][i] -= h[i]
+            dn[k][i] += h[i]
+        # central differences of the summed functional: truncation and cancellation ~1e-5
+        np.testing.assert_allclose(g, fd, rtol=1e-4, atol=1e-12 * np.abs(fd).max())

@@ -23,7 +23,8 @@ BUILD = PKG / "_build"
 LIB = PKG / "libfwi_psv.so"
 DRIVER = PKG / "fwi_psv"
 
-HIP_SOURCES = ["psv_stencil6.hip", "psv_kernels.hip", "psv_misc.hip", "psv_plain.hip", "psv_api.hip"]
+HIP_SOURCES = ["psv_stencil6.hip", "psv_step7.hip", "psv_kernels.hip", "psv_misc.hip", "psv_plain.hip",
+               "psv_api.hip"]
 CPP_SOURCES = ["psv_preproc.cpp"]
 DRIVER_SOURCES = ["host/psv_io.cpp", "host/psv_main.cpp"]
 

@@ -382,6 +382,16 @@ RecArgs rec_args(fwi_ctx *c, int s0, int nb) {
     return r;
 }
 
+// variants that advance a whole time step in one pass over two time levels
+bool one_pass(int var) { return var == 4 || var == 7; }
+
+// one time step of a one-pass variant (variant 7 falls back to 4 for a launch mode it
+// has no kernel for)
+void launch_one_pass(fwi_ctx *c, const KArgs &a, int var, int tz, int depth) {
+    if (var == 7 && c->ztab_ok && launch_step7(a, c->ztab, c->N, tz, c->st)) return;
+    launch_step(a, c->N, var == 7 ? 32 : tz, depth, c->st);
+}
+
 bool has_surface(const fwi_ctx *c) {
     for (int i = 0; i < 4; ++i)
         if (c->g.isurf[i] > 0) return true;
@@ -422,10 +432,10 @@ void time_step(fwi_ctx *c, KArgs &a, const Wave &w, const SrcArgs &sa, int it, f
         a.st_vz = dst[0]; a.st_vx = dst[1]; a.st_szz = dst[2]; a.st_szx = dst[3]; a.st_sxx = dst[4];
         a.st_stride = c->snap_stride;
     }
-    if (c->var == 4 && !c->plain) {
+    if (one_pass(c->var) && !c->plain) {
         {
             Timer t(c, FWI_K_STEP);
-            launch_step(a, c->N, c->tz, c->depth, c->st);
+            launch_one_pass(c, a, c->var, c->tz, c->depth);
         }
         w.flip();
         set_state(a, w);
@@ -573,10 +583,10 @@ int autotune(fwi_ctx *c, int nb) {
     const bool order_forced = c->o.work_order >= 1 && c->o.work_order <= 3;
     // candidates (variant, rows per segment, prefetch depth); the one-pass step only where it applies
     std::vector<TuneVal> cand;
-    for (int var = 1; var <= 6; ++var) {
-        if ((fv && var != fv) || (var == 4 && !c->fusable) || (var >= 5 && !c->ztab_ok)) continue;
-        if (var == 6) {   // loads one row ahead, segments of one or two row blocks
-            for (int m = 1; m <= 2; ++m) {
+    for (int var = 1; var <= 7; ++var) {
+        if ((fv && var != fv) || (one_pass(var) && !c->fusable) || (var >= 5 && !c->ztab_ok)) continue;
+        if (var >= 6) {   // loads one row ahead, segments of 1, 2 (variant 6) or 2, 4 (variant 7) blocks
+            for (int m : var == 6 ? std::vector<int>{1, 2} : std::vector<int>{2, 4}) {
                 const int tz = v6_block(c->N, 1) * m;
                 if (c->o.tile_tz <= 0 || c->o.tile_tz == tz) cand.push_back({var, tz, 1, 0});
             }
@@ -596,6 +606,8 @@ int autotune(fwi_ctx *c, int nb) {
                                    "at the top and bottom of the grid)"
                        : fv == 6 ? " (variant 6 takes tile_tz 0, fdorder + 2 or 2 fdorder + 4, and z CPML "
                                    "strips of at most 64 rows at the top and bottom of the grid)"
+                       : fv == 7 ? " (variant 7 takes tile_tz 0, 2 fdorder + 4 or 4 fdorder + 8, variant 4's "
+                                   "conditions and variant 6's z CPML strips)"
                                  : "");
     if (cand.size() == 1 && order_forced) {   // nothing left to choose
         take(c, {cand[0].var, cand[0].tz, cand[0].depth, c->o.work_order - 1});
@@ -604,7 +616,7 @@ int autotune(fwi_ctx *c, int nb) {
     {
         std::lock_guard<std::mutex> lk(g_tune_mu);
         auto it = g_tuned.find(key);
-        if (it != g_tuned.end() && (!fv || fv == it->second.var) && (it->second.var != 4 || c->fusable) &&
+        if (it != g_tuned.end() && (!fv || fv == it->second.var) && (!one_pass(it->second.var) || c->fusable) &&
             (!order_forced || c->o.work_order - 1 == it->second.order) &&
             (c->o.tile_tz <= 0 || c->o.tile_tz == it->second.tz)) {
             take(c, it->second);
@@ -636,8 +648,8 @@ int autotune(fwi_ctx *c, int nb) {
     auto run = [&](const TuneVal &v) {
         for (KArgs *x : {&a, &as, &ag}) {
             x->order = a.order;
-            if (v.var == 4) {
-                launch_step(*x, c->N, v.tz, v.depth, c->st);
+            if (one_pass(v.var)) {
+                launch_one_pass(c, *x, v.var, v.tz, v.depth);
             } else {
                 launch_stress(*x, c->N, v.tz, v.var, c->st, &c->ztab, v.depth);
                 launch_velocity(*x, c->N, v.tz, v.var, c->st, &c->ztab, v.depth);
@@ -1226,11 +1238,11 @@ int validate(fwi_ctx *c, const fwi_grid *g, const fwi_acq *q) {
     if (g->nz < 2 * N + 1 || g->nx < 2 * N + 1) return c->fail(FWI_EINVAL, "grid smaller than the stencil");
     if (!(g->dt > 0) || !(g->dz > 0) || !(g->dx > 0)) return c->fail(FWI_EINVAL, "dt, dz, dx must be positive");
     const int tt = c->o.tile_tz;
-    if (tt < 0 || (c->o.kernel_variant != 6 && tt != 0 && tt != 8 && tt != 16 && tt != 32 && tt != 64))
-        return c->fail(FWI_EINVAL, "tile_tz %d (0, 8, 16, 32 or 64; variant 6: 0 or its row block times 1 or 2)",
-                       tt);
-    if (c->o.kernel_variant < 0 || c->o.kernel_variant > 6)
-        return c->fail(FWI_EINVAL, "kernel_variant %d (0..6)", c->o.kernel_variant);
+    if (tt < 0 || (c->o.kernel_variant < 6 && tt != 0 && tt != 8 && tt != 16 && tt != 32 && tt != 64))
+        return c->fail(FWI_EINVAL,
+                       "tile_tz %d (0, 8, 16, 32 or 64; variants 6, 7: 0 or their row block times 1 or 2)", tt);
+    if (c->o.kernel_variant < 0 || c->o.kernel_variant > 7)
+        return c->fail(FWI_EINVAL, "kernel_variant %d (0..7)", c->o.kernel_variant);
     if (c->o.work_order < 0 || c->o.work_order > 3)
         return c->fail(FWI_EINVAL, "work_order %d (0..3)", c->o.work_order);
     if (g->snap_dt < 1 || g->snap_dz < 1 || g->snap_dx < 1) return c->fail(FWI_EINVAL, "snap strides < 1");

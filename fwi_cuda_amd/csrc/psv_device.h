@@ -1,10 +1,23 @@
 // psv_device.h -- device helpers shared by the stencil translation units.
 #pragma once
 
+#include <utility>
+
 #include "psv_internal.h"
 
 namespace psv {
 namespace {
+
+template <class F, int... I>
+__device__ __forceinline__ void static_for_impl(F &&f, std::integer_sequence<int, I...>) {
+    (f(std::integral_constant<int, I>{}), ...);
+}
+// f(std::integral_constant<int, 0>{}) ... f(std::integral_constant<int, n - 1>{}): loop
+// positions that index register arrays must be compile-time constants
+template <int n, class F>
+__device__ __forceinline__ void static_for(F &&f) {
+    static_for_impl(f, std::make_integer_sequence<int, n>{});
+}
 
 // column / row on the strided snapshot box
 __device__ __forceinline__ bool ix_on_lattice(const KArgs &a, int x) {

@@ -22,22 +22,10 @@
 //                (src/cpu/n_kernel_lib_PSV.cpp:100-131, 134-178, 181-205)
 //   k_velocity6  sdiff2 + pml_diff2 + update_v2 (:207-270), fwi_grad2 (:439-484)
 //                on correlating steps, gard_fwd_storage2 (:398-436) on storing steps
-#include <utility>
-
 #include "psv_device.h"
 
 namespace psv {
 namespace {
-
-template <class F, int... I>
-__device__ __forceinline__ void static_for_impl(F &&f, std::integer_sequence<int, I...>) {
-    (f(std::integral_constant<int, I>{}), ...);
-}
-// f(std::integral_constant<int, 0>{}) ... f(std::integral_constant<int, n - 1>{})
-template <int n, class F>
-__device__ __forceinline__ void static_for(F &&f) {
-    static_for_impl(f, std::make_integer_sequence<int, n>{});
-}
 
 template <int N, int D, int M>
 __global__ __launch_bounds__(kBX *kBY) void k_stress6(const KArgs a, const ZTab zt) {

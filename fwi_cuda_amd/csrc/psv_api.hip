@@ -585,8 +585,8 @@ int autotune(fwi_ctx *c, int nb) {
     std::vector<TuneVal> cand;
     for (int var = 1; var <= 7; ++var) {
         if ((fv && var != fv) || (one_pass(var) && !c->fusable) || (var >= 5 && !c->ztab_ok)) continue;
-        if (var >= 6) {   // loads one row ahead, segments of 1, 2 (variant 6) or 2, 4 (variant 7) blocks
-            for (int m : var == 6 ? std::vector<int>{1, 2} : std::vector<int>{2, 4}) {
+        if (var >= 6) {   // loads one row ahead, segments of 1, 2 (variant 6) or 4, 8 (variant 7) blocks
+            for (int m : var == 6 ? std::vector<int>{1, 2} : std::vector<int>{4, 8}) {
                 const int tz = v6_block(c->N, 1) * m;
                 if (c->o.tile_tz <= 0 || c->o.tile_tz == tz) cand.push_back({var, tz, 1, 0});
             }
@@ -606,7 +606,7 @@ int autotune(fwi_ctx *c, int nb) {
                                    "at the top and bottom of the grid)"
                        : fv == 6 ? " (variant 6 takes tile_tz 0, fdorder + 2 or 2 fdorder + 4, and z CPML "
                                    "strips of at most 64 rows at the top and bottom of the grid)"
-                       : fv == 7 ? " (variant 7 takes tile_tz 0, 2 fdorder + 4 or 4 fdorder + 8, variant 4's "
+                       : fv == 7 ? " (variant 7 takes tile_tz 0, 4 fdorder + 8 or 8 fdorder + 16, variant 4's "
                                    "conditions and variant 6's z CPML strips)"
                                  : "");
     if (cand.size() == 1 && order_forced) {   // nothing left to choose
@@ -1240,7 +1240,8 @@ int validate(fwi_ctx *c, const fwi_grid *g, const fwi_acq *q) {
     const int tt = c->o.tile_tz;
     if (tt < 0 || (c->o.kernel_variant < 6 && tt != 0 && tt != 8 && tt != 16 && tt != 32 && tt != 64))
         return c->fail(FWI_EINVAL,
-                       "tile_tz %d (0, 8, 16, 32 or 64; variants 6, 7: 0 or their row block times 1 or 2)", tt);
+                       "tile_tz %d (0, 8, 16, 32 or 64; variant 6: 0 or its row block times 1 or 2; "
+                       "variant 7: 0 or its row block times 4 or 8)", tt);
     if (c->o.kernel_variant < 0 || c->o.kernel_variant > 7)
         return c->fail(FWI_EINVAL, "kernel_variant %d (0..7)", c->o.kernel_variant);
     if (c->o.work_order < 0 || c->o.work_order > 3)

@@ -88,6 +88,24 @@ __device__ __forceinline__ void st_at(T *base, uint32_t off, T v) {
 }
 __device__ __forceinline__ int wave_uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
+// Fields of the launch's KArgs loaded again from the kernel-argument segment where they
+// are used, for blocks that run on a few rows only, such as the CPML strips: they then
+// keep no base pointer live in SGPRs across the row loop.  The kernels that use it take
+// their KArgs as the first argument (segment offset 0).  karg_seg() passes the segment
+// address through an empty asm statement, so the scalar loads from it stay in the block
+// that calls it, and issue there together.
+using KSeg = const __attribute__((address_space(4))) char *;
+__device__ __forceinline__ KSeg karg_seg() {
+    KSeg p = (KSeg)__builtin_amdgcn_kernarg_segment_ptr();
+    asm volatile("" : "+s"(p));
+    return p;
+}
+template <class T>
+__device__ __forceinline__ T karg_at(KSeg seg, uint32_t off) {
+    return *(const __attribute__((address_space(4))) T *)(seg + off);
+}
+#define PSV_KARG(seg, field) (karg_at<decltype(KArgs::field)>(seg, offsetof(KArgs, field)))
+
 // z-strip test and strip row (the psi_z row) of a wave-uniform row
 __device__ __forceinline__ bool in_zstrip(const ZTab &zt, int z) { return z < zt.lo || z >= zt.hi; }
 __device__ __forceinline__ int strip_row(const ZTab &zt, int N, int z) {

@@ -164,22 +164,24 @@ __device__ __forceinline__ void step7_rows(const KArgs &a, const ZTab &zt, const
         const bool wrow = j >= z0 && j < z1;               // this segment's row
         const uint32_t so = (own && wrow) ? vo : vo + dpad;   // new stresses (and stress-side psi_z)
         if (jint && in_zstrip(zt, j)) {   // wave-uniform; the row's packed profile and strip row
-            const CpmlPoint pz = a.zp[j];
+            const KSeg ks = karg_seg();
+            const CpmlPoint pz = PSV_KARG(ks, zp)[j];
             const uint32_t zo = (uint32_t)pz.map * P4 + c4, zs = (own && wrow) ? zo : zo + dpad;
-            const float p1 = pz.b * ld_at(a.pvz_z + zso, zo) + pz.a * vz_z;
-            const float p2 = pz.bh * ld_at(a.pvx_z + zso, zo) + pz.ah * vx_z;
-            st_at(a.o_pvz_z + zso, zs, p1);
-            st_at(a.o_pvx_z + zso, zs, p2);
+            const float p1 = pz.b * ld_at(PSV_KARG(ks, pvz_z) + zso, zo) + pz.a * vz_z;
+            const float p2 = pz.bh * ld_at(PSV_KARG(ks, pvx_z) + zso, zo) + pz.ah * vx_z;
+            st_at(PSV_KARG(ks, o_pvz_z) + zso, zs, p1);
+            st_at(PSV_KARG(ks, o_pvx_z) + zso, zs, p2);
             vz_z = vz_z * pz.ik + p1;
             vx_z = vx_z * pz.ikh + p2;
         }
         if (jint && cxs >= 0) {
+            const KSeg ks = karg_seg();
             const uint32_t xo = (uint32_t)(j - rb) * nxs4 + xs4;
-            const float p1 = xq.b * ld_at(a.pvx_x + xso, xo) + xq.a * vx_x;
-            const float p2 = xq.bh * ld_at(a.pvz_x + xso, xo) + xq.ah * vz_x;
+            const float p1 = xq.b * ld_at(PSV_KARG(ks, pvx_x) + xso, xo) + xq.a * vx_x;
+            const float p2 = xq.bh * ld_at(PSV_KARG(ks, pvz_x) + xso, xo) + xq.ah * vz_x;
             if (own && wrow) {
-                st_at(a.o_pvx_x + xso, xo, p1);
-                st_at(a.o_pvz_x + xso, xo, p2);
+                st_at(PSV_KARG(ks, o_pvx_x) + xso, xo, p1);
+                st_at(PSV_KARG(ks, o_pvz_x) + xso, xo, p2);
             }
             vx_x = vx_x * xq.ik + p1;
             vz_x = vz_x * xq.ikh + p2;
@@ -220,21 +222,23 @@ __device__ __forceinline__ void step7_rows(const KArgs &a, const ZTab &zt, const
         // integer-point profiles for szz_z and sxx_x, half-point for szx_z and szx_x,
         // the reference's slot mapping (src/cpu/n_kernel_PSV.cpp:142-146)
         if (in_zstrip(zt, z)) {
-            const CpmlPoint pz = a.zp[z];
+            const KSeg ks = karg_seg();
+            const CpmlPoint pz = PSV_KARG(ks, zp)[z];
             const uint32_t zo = (uint32_t)pz.map * P4 + c4, zs = own ? zo : zo + dpad;
-            const float p1 = pz.b * ld_at(a.pszz_z + zso, zo) + pz.a * szz_z;
-            const float p2 = pz.bh * ld_at(a.pszx_z + zso, zo) + pz.ah * szx_z;
-            st_at(a.pszz_z + zso, zs, p1);
-            st_at(a.pszx_z + zso, zs, p2);
+            const float p1 = pz.b * ld_at(PSV_KARG(ks, pszz_z) + zso, zo) + pz.a * szz_z;
+            const float p2 = pz.bh * ld_at(PSV_KARG(ks, pszx_z) + zso, zo) + pz.ah * szx_z;
+            st_at(PSV_KARG(ks, pszz_z) + zso, zs, p1);
+            st_at(PSV_KARG(ks, pszx_z) + zso, zs, p2);
             szz_z = szz_z * pz.ik + p1;
             szx_z = szx_z * pz.ikh + p2;
         }
         if (cxv >= 0) {
+            const KSeg ks = karg_seg();
             const uint32_t xo = (uint32_t)(z - rb) * nxs4 + xv4;
-            const float p1 = xq.b * ld_at(a.psxx_x + xso, xo) + xq.a * sxx_x;
-            const float p2 = xq.bh * ld_at(a.pszx_x + xso, xo) + xq.ah * szx_x;
-            st_at(a.psxx_x + xso, xo, p1);
-            st_at(a.pszx_x + xso, xo, p2);
+            const float p1 = xq.b * ld_at(PSV_KARG(ks, psxx_x) + xso, xo) + xq.a * sxx_x;
+            const float p2 = xq.bh * ld_at(PSV_KARG(ks, pszx_x) + xso, xo) + xq.ah * szx_x;
+            st_at(PSV_KARG(ks, psxx_x) + xso, xo, p1);
+            st_at(PSV_KARG(ks, pszx_x) + xso, xo, p2);
             sxx_x = sxx_x * xq.ik + p1;
             szx_x = szx_x * xq.ikh + p2;
         }
@@ -349,17 +353,17 @@ bool step7_m(const KArgs &a, const ZTab &zt, dim3 grid, dim3 block, hipStream_t 
     }
 }
 
-// Segments of two or four blocks: every segment recomputes 2N halo rows of stresses,
-// so one block (2N + 2 rows) spends almost half its stress work on halos.
+// Segments of four or eight blocks: every segment recomputes 2N halo rows of stresses,
+// so one block (2N + 2 rows) alone would spend almost half its stress work on halos.
 template <int N>
 bool step7_n(const KArgs &a, const ZTab &zt, int M, dim3 grid, dim3 block, hipStream_t st) {
-    return M == 4 ? step7_m<N, 4>(a, zt, grid, block, st) : step7_m<N, 2>(a, zt, grid, block, st);
+    return M == 8 ? step7_m<N, 8>(a, zt, grid, block, st) : step7_m<N, 4>(a, zt, grid, block, st);
 }
 
 }  // namespace
 
 bool launch_step7(const KArgs &a, const ZTab &zt, int N, int tz, hipStream_t st) {
-    const int R = 2 * N + 2, M = tz >= 4 * R ? 4 : 2, TZ = R * M;
+    const int R = 2 * N + 2, M = tz >= 8 * R ? 8 : 4, TZ = R * M;
     const int nxi = a.nx - 2 * N, nzi = a.nz - 2 * N, outw = kBX - 4 * N;
     const int segs = (nzi + TZ - 1) / TZ;
     const dim3 grid(((nxi + outw - 1) / outw) * ((segs + kBY - 1) / kBY) * a.B), block(kBX, kBY);

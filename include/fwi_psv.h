@@ -126,7 +126,7 @@ typedef struct {
     int32_t tile_tz;        /* rows per thread in the stencil kernels; 0 = tuned;
                                8, 16, 32, 64 (variants 1-3), 32, 64 (variant 4), 16, 32
                                (variant 5), fdorder + 2 or 2 fdorder + 4 (variant 6),
-                               2 fdorder + 4 or 4 fdorder + 8 (variant 7) */
+                               4 fdorder + 8 or 8 fdorder + 16 (variant 7) */
     int32_t device;         /* HIP device ordinal */
     int32_t check_finite;   /* scan the wavefield for NaN/Inf after each propagation */
     int32_t kernel_variant; /* stencil kernels: 0 = tuned, 1 = x taps as shifted loads,

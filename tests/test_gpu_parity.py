@@ -335,7 +335,7 @@ def test_uniform_addressing_variant_matches_variant2(psv, O, fdorder, variant, b
         assert e < 1e-4 and e2 < 1e-4
 
 
-@pytest.mark.parametrize("variant,tz,fdorder", [(4, 0, 8), (7, 2, 8), (7, 4, 8), (7, 2, 2), (7, 4, 4)])
+@pytest.mark.parametrize("variant,tz,fdorder", [(4, 0, 8), (7, 4, 8), (7, 8, 8), (7, 4, 2), (7, 8, 4)])
 def test_one_pass_step_matches_two_kernel_step(psv, O, variant, tz, fdorder):
     """The one-pass variants (stress, surface and velocity in one pass, two time
     levels; 7 with segments of tz blocks of fdorder + 2 rows) against variant 1:

@@ -48,7 +48,14 @@ def main(argv=None):
 
     rows, ref = [], None
     for gb in (float(v) for v in args.budgets.split(",")):
-        with psv.Simulation(c, storage=_lib.STORE_CHECKPOINT, mem_budget=int(gb * 1e9)) as sim:
+        try:
+            sim = psv.Simulation(c, storage=_lib.STORE_CHECKPOINT, mem_budget=int(gb * 1e9))
+        except _lib.FwiError as e:   # the budget holds less than one shot's arrays
+            row = {"budget_GB": gb, "refused": str(e)}
+            rows.append(row)
+            print(json.dumps(row), flush=True)
+            continue
+        with sim:
             plan = sim.plan()
             sim.set_model(start["lam"], start["mu"], start["rho"])
             sim.set_observed(*obs)

@@ -11,10 +11,12 @@ adjoint.  Synthetic data: a Gaussian-random-field true model generates the
 observed seismograms (untimed); the gradient is taken at its smoothed version.
 
 Printed JSON (one line): Gcell-updates/s = 2 nz nx nt nshot / wall (forward
-and adjoint cells, recomputation not counted).  The timed steps run with time-
-step chunks captured in hipGraphs and replayed (the library's use_graph mode),
-wall clock only; a second, eager run of the same steps times every launch with
-HIP events for the roofline (--no-graph times the eager run instead).  With
+and adjoint cells, recomputation not counted) of the timed eager run, whose
+launches also carry the HIP events the roofline is computed from.  A second run
+of the same steps in a new context, with time-step chunks captured in hipGraphs
+and replayed (the library's use_graph mode, wall clock only), is reported as
+"graph" (--no-graph skips it).  The two runs' arrays land in different physical
+memory, which alone moves kernel times by up to about 10 %.  With
   roofline      the dominant kernel's algorithmic bytes per launch over its
                 average duration (HIP events on the library's stream), against
                 MI355X's 8 TB/s;
@@ -366,8 +368,9 @@ def main():
                                  "algorithmic_bytes_per_launch": int(b_all / max(n_all, 1))}}
     if traffic:
         roofline["traffic_source"] = ("profiles/roofline_traffic.txt: rocprofv3 FETCH_SIZE x 2 + WRITE_SIZE per "
-                                      "launch over the same launch mix (all_launches)")
-        roofline["traffic_over_algorithmic"] = round(traffic / max(b_all / max(n_all, 1), 1.0), 3)
+                                      "launch, averaged over the gradient's launch mix (forward, recomputation, "
+                                      "adjoint)")
+        roofline["traffic_over_algorithmic"] = round(traffic / max(b / max(n, 1), 1.0), 3)
     per_kernel = {k: {"achieved_GBps": round(v[0] / (v[1] / 1e3) / 1e9, 1) if v[1] else None,
                       "launches": v[2], "total_ms": round(v[1], 1)} for k, v in kern.items()}
     sim.close()
@@ -397,7 +400,6 @@ def main():
             tg = time.perf_counter() - t0
         if not np.allclose(gm, misfit, rtol=1e-12, atol=0):
             log("graph replay misfits differ from the eager run's:", gm, misfit)
-        head = tg
         graph = {"ms_per_step": round(tg / K * 1e3, 2), "Gcell_per_s": round(2.0 * cells * nt * nshot * K / tg / 1e9, 3),
                  "speedup_vs_eager_with_events": round(wall / tg, 3),
                  "speedup_vs_unprofiled_eager": round(t_unprofiled * K / tg, 3) if t_unprofiled else None}
@@ -458,7 +460,7 @@ def main():
         "hbm_frac": round(hbm_frac, 4),
         "roofline": roofline,
         "kernels": per_kernel,
-        "timed_run": "hipGraph replay of time-step chunks (use_graph=1)" if graph else "eager launches with events",
+        "timed_run": "eager launches with per-launch HIP events (the roofline's launches)",
         "graph": graph,
         "eager": eager,
         "plain": plain,

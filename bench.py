@@ -91,7 +91,9 @@ def kernel_instantiation(kind, tune_log, N):
         return None
     var, tz = int(m.group(1)), int(m.group(2))
     if kind == "step":   # several launch modes share the prefix
-        return f"k_step7<{N}, {max(1, tz // (2 * N + 2))}, " if var == 7 else f"k_step<{N}, {tz}, "
+        if var in (7, 8):   # variant 8: the block's wavefronts side by side in x
+            return f"k_step7<{N}, {max(1, tz // (2 * N + 2))}, *, {'true' if var == 8 else 'false'}>"
+        return f"k_step<{N}, {tz}, "
     base = {"stress": "k_stress", "velocity": "k_velocity"}[kind]
     if var == 1:
         return f"{base}<{N}, {tz}>"

@@ -383,13 +383,13 @@ RecArgs rec_args(fwi_ctx *c, int s0, int nb) {
 }
 
 // variants that advance a whole time step in one pass over two time levels
-bool one_pass(int var) { return var == 4 || var == 7; }
+bool one_pass(int var) { return var == 4 || var == 7 || var == 8; }
 
-// one time step of a one-pass variant (variant 7 falls back to 4 for a launch mode it
-// has no kernel for)
+// one time step of a one-pass variant (variants 7 and 8 fall back to 4 for a launch mode
+// they have no kernel for)
 void launch_one_pass(fwi_ctx *c, const KArgs &a, int var, int tz, int depth) {
-    if (var == 7 && c->ztab_ok && launch_step7(a, c->ztab, c->N, tz, c->st)) return;
-    launch_step(a, c->N, var == 7 ? 32 : tz, depth, c->st);
+    if (var >= 7 && c->ztab_ok && launch_step7(a, c->ztab, c->N, tz, var == 8, c->st)) return;
+    launch_step(a, c->N, var >= 7 ? 32 : tz, depth, c->st);
 }
 
 bool has_surface(const fwi_ctx *c) {
@@ -583,9 +583,9 @@ int autotune(fwi_ctx *c, int nb) {
     const bool order_forced = c->o.work_order >= 1 && c->o.work_order <= 3;
     // candidates (variant, rows per segment, prefetch depth); the one-pass step only where it applies
     std::vector<TuneVal> cand;
-    for (int var = 1; var <= 7; ++var) {
+    for (int var = 1; var <= 8; ++var) {
         if ((fv && var != fv) || (one_pass(var) && !c->fusable) || (var >= 5 && !c->ztab_ok)) continue;
-        if (var >= 6) {   // loads one row ahead, segments of 1, 2 (variant 6) or 4, 8 (variant 7) blocks
+        if (var >= 6) {   // loads one row ahead, segments of 1, 2 (variant 6) or 4, 8 (variants 7, 8) blocks
             for (int m : var == 6 ? std::vector<int>{1, 2} : std::vector<int>{4, 8}) {
                 const int tz = v6_block(c->N, 1) * m;
                 if (c->o.tile_tz <= 0 || c->o.tile_tz == tz) cand.push_back({var, tz, 1, 0});
@@ -606,8 +606,8 @@ int autotune(fwi_ctx *c, int nb) {
                                    "at the top and bottom of the grid)"
                        : fv == 6 ? " (variant 6 takes tile_tz 0, fdorder + 2 or 2 fdorder + 4, and z CPML "
                                    "strips of at most 64 rows at the top and bottom of the grid)"
-                       : fv == 7 ? " (variant 7 takes tile_tz 0, 4 fdorder + 8 or 8 fdorder + 16, variant 4's "
-                                   "conditions and variant 6's z CPML strips)"
+                       : fv >= 7 ? " (variants 7 and 8 take tile_tz 0, 4 fdorder + 8 or 8 fdorder + 16, "
+                                   "variant 4's conditions and variant 6's z CPML strips)"
                                  : "");
     if (cand.size() == 1 && order_forced) {   // nothing left to choose
         take(c, {cand[0].var, cand[0].tz, cand[0].depth, c->o.work_order - 1});
@@ -1241,9 +1241,9 @@ int validate(fwi_ctx *c, const fwi_grid *g, const fwi_acq *q) {
     if (tt < 0 || (c->o.kernel_variant < 6 && tt != 0 && tt != 8 && tt != 16 && tt != 32 && tt != 64))
         return c->fail(FWI_EINVAL,
                        "tile_tz %d (0, 8, 16, 32 or 64; variant 6: 0 or its row block times 1 or 2; "
-                       "variant 7: 0 or its row block times 4 or 8)", tt);
-    if (c->o.kernel_variant < 0 || c->o.kernel_variant > 7)
-        return c->fail(FWI_EINVAL, "kernel_variant %d (0..7)", c->o.kernel_variant);
+                       "variants 7, 8: 0 or their row block times 4 or 8)", tt);
+    if (c->o.kernel_variant < 0 || c->o.kernel_variant > 8)
+        return c->fail(FWI_EINVAL, "kernel_variant %d (0..8)", c->o.kernel_variant);
     if (c->o.work_order < 0 || c->o.work_order > 3)
         return c->fail(FWI_EINVAL, "work_order %d (0..3)", c->o.work_order);
     if (g->snap_dt < 1 || g->snap_dz < 1 || g->snap_dx < 1) return c->fail(FWI_EINVAL, "snap strides < 1");

@@ -45,10 +45,13 @@ __device__ __forceinline__ StepView step_view(const KArgs &a, int it) {
 // of work, ordered tile-major and shot-minor: the B shots of a tile run back to
 // back on one XCD (the tile's media and CPML profiles stay in that L2), then the
 // vertically adjacent segment (its z halo rows are the previous tile's rows).
-template <int TZ, int OUTW = kBX>
+// With XW the kBY wavefronts of a block sit side by side in x (a tile is kBY * OUTW
+// columns of one z segment), otherwise one above the other in z (kBY segments of one
+// OUTW-column tile).
+template <int TZ, int OUTW = kBX, bool XW = false>
 __device__ __forceinline__ void tile_of(const KArgs &a, int N, int &bx, int &by, int &s) {
-    const int gx = (a.nx - 2 * N + OUTW - 1) / OUTW;
-    const int gy = ((a.nz - 2 * N + TZ - 1) / TZ + kBY - 1) / kBY;
+    const int gx = XW ? ((a.nx - 2 * N + OUTW - 1) / OUTW + kBY - 1) / kBY : (a.nx - 2 * N + OUTW - 1) / OUTW;
+    const int gy = XW ? (a.nz - 2 * N + TZ - 1) / TZ : ((a.nz - 2 * N + TZ - 1) / TZ + kBY - 1) / kBY;
     const int n = gx * gy * a.B;
     const int L = blockIdx.x, xcd = L & 7, loc = L >> 3;
     const int q = n >> 3, r = n & 7;

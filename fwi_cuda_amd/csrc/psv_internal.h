@@ -149,10 +149,11 @@ constexpr int v6_block(int N, int D) { return ((2 * N + D + D) / (D + 1)) * (D +
 int v6_rows(int N, int depth, int tz);
 bool launch6(int which, const KArgs &a, const ZTab &zt, int N, int depth, int tz, hipStream_t st);
 // Variant 7 (psv_step7.hip): the one-pass step with variant 6's addressing and row-ahead
-// loads; segments of two or four blocks of 2N + 2 rows (tz >= 4 (2N + 2) picks four).
-// Same conditions as launch_step (top free surface at row N only, interior sources);
-// false for a launch mode it has no kernel for.
-bool launch_step7(const KArgs &a, const ZTab &zt, int N, int tz, hipStream_t st);
+// loads; segments of four or eight blocks of 2N + 2 rows (tz >= 8 (2N + 2) picks eight);
+// xw sets a block's wavefronts side by side in x (variant 8) instead of one above the
+// other in z.  Same conditions as launch_step (top free surface at row N only, interior
+// sources); false for a launch mode it has no kernel for.
+bool launch_step7(const KArgs &a, const ZTab &zt, int N, int tz, bool xw, hipStream_t st);
 // stress, top free surface and velocity of one time step in one pass (kernel variant 4);
 // tz = rows per segment (32, 64), depth = rows loaded ahead (1 is the one instantiated)
 void launch_step(const KArgs &a, int N, int tz, int depth, hipStream_t st);

@@ -23,6 +23,10 @@
 // updated in place by their owner.  Compulsory traffic per cell-step: 68 B forward
 // (R+W five fields, R five media, R+W We), the survey's single-pass count.
 //
+// A block's kBY wavefronts are kBY consecutive z segments of one column tile, or, in
+// variant 8 (XW), kBY consecutive column tiles of one z segment: the block then reads
+// kBY * 256 contiguous bytes of each row it touches.
+//
 //   stress   vdiff2 + pml_diff2 + update_s2 (src/cpu/n_kernel_lib_PSV.cpp:100-131, 134-178,
 //            181-205); surf_mirror's top row (:297-301)
 //   velocity sdiff2 + pml_diff2 + update_v2 (:207-270), fwi_grad2 (:439-484) on correlating
@@ -32,17 +36,18 @@
 namespace psv {
 namespace {
 
-template <int N, int M, int MODE, bool GRAD>
+template <int N, int M, int MODE, bool GRAD, bool XW>
 __device__ __forceinline__ void step7_rows(const KArgs &a, const ZTab &zt, const StepView &sv) {
     constexpr int R = 2 * N + 2, RS = N + 1, TZ = R * M, OUTW = kBX - 4 * N;
     constexpr bool adj = (MODE & kVelAdjoint) != 0, energy = (MODE & kVelEnergy) != 0;
     int tbx, tby, s;
-    tile_of<TZ, OUTW>(a, N, tbx, tby, s);
-    const int lane = threadIdx.x;
-    const int x = tbx * OUTW + lane - N;   // lane 2N holds the first owned column N + tbx * OUTW
-    const int z0 = N + (tby * kBY + wave_uniform((int)threadIdx.y)) * TZ;
+    tile_of<TZ, OUTW, XW>(a, N, tbx, tby, s);
+    const int lane = threadIdx.x, wy = wave_uniform((int)threadIdx.y);
+    const int tx = XW ? tbx * kBY + wy : tbx;   // the wavefront's column tile
+    const int x = tx * OUTW + lane - N;         // lane 2N holds the first owned column N + tx * OUTW
+    const int z0 = N + (XW ? tby : tby * kBY + wy) * TZ;
     const int nzi = a.nz - N;
-    if (z0 >= nzi) return;
+    if (z0 >= nzi || tx * OUTW >= a.nx - 2 * N) return;
     const int z1 = min(z0 + TZ, nzi);
     const int cx = min(max(x, 0), a.nx - 1);   // lanes off the grid read a valid column
     // stresses are updated where the lane has its x taps and the column is interior;
@@ -322,32 +327,33 @@ __device__ __forceinline__ void step7_rows(const KArgs &a, const ZTab &zt, const
 }
 
 // kVelAdjoint launches branch once, on the step's correlation switch
-template <int N, int M, int MODE>
+template <int N, int M, int MODE, bool XW>
 __global__ __launch_bounds__(kBX *kBY) void k_step7(const KArgs a, const ZTab zt) {
     const StepView sv = step_view(a, 0);
     if constexpr ((MODE & kVelAdjoint) != 0) {
         if (sv.grad) {
-            step7_rows<N, M, MODE, true>(a, zt, sv);
+            step7_rows<N, M, MODE, true, XW>(a, zt, sv);
             return;
         }
     }
-    step7_rows<N, M, MODE, false>(a, zt, sv);
+    step7_rows<N, M, MODE, false, XW>(a, zt, sv);
 }
 
-template <int N, int M>
+template <int N, int M, bool XW>
 bool step7_m(const KArgs &a, const ZTab &zt, dim3 grid, dim3 block, hipStream_t st) {
     switch (vel_mode(a)) {
-        case 0: hipLaunchKernelGGL((k_step7<N, M, 0>), grid, block, 0, st, a, zt); return true;
-        case kVelEnergy: hipLaunchKernelGGL((k_step7<N, M, kVelEnergy>), grid, block, 0, st, a, zt); return true;
-        case kVelStore: hipLaunchKernelGGL((k_step7<N, M, kVelStore>), grid, block, 0, st, a, zt); return true;
+        case 0: hipLaunchKernelGGL((k_step7<N, M, 0, XW>), grid, block, 0, st, a, zt); return true;
+        case kVelEnergy: hipLaunchKernelGGL((k_step7<N, M, kVelEnergy, XW>), grid, block, 0, st, a, zt); return true;
+        case kVelStore: hipLaunchKernelGGL((k_step7<N, M, kVelStore, XW>), grid, block, 0, st, a, zt); return true;
         case kVelEnergy | kVelStore:
-            hipLaunchKernelGGL((k_step7<N, M, kVelEnergy | kVelStore>), grid, block, 0, st, a, zt);
+            hipLaunchKernelGGL((k_step7<N, M, kVelEnergy | kVelStore, XW>), grid, block, 0, st, a, zt);
             return true;
         case kVelEnergy | kVelAdjoint:
-            hipLaunchKernelGGL((k_step7<N, M, kVelEnergy | kVelAdjoint>), grid, block, 0, st, a, zt);
+            hipLaunchKernelGGL((k_step7<N, M, kVelEnergy | kVelAdjoint, XW>), grid, block, 0, st, a, zt);
             return true;
         case kVelEnergy | kVelAdjoint | kVelMisfit:
-            hipLaunchKernelGGL((k_step7<N, M, kVelEnergy | kVelAdjoint | kVelMisfit>), grid, block, 0, st, a, zt);
+            hipLaunchKernelGGL((k_step7<N, M, kVelEnergy | kVelAdjoint | kVelMisfit, XW>), grid, block, 0, st, a,
+                               zt);
             return true;
         default: return false;
     }
@@ -356,24 +362,27 @@ bool step7_m(const KArgs &a, const ZTab &zt, dim3 grid, dim3 block, hipStream_t 
 // Segments of four or eight blocks: every segment recomputes 2N halo rows of stresses,
 // so one block (2N + 2 rows) alone would spend almost half its stress work on halos.
 template <int N>
-bool step7_n(const KArgs &a, const ZTab &zt, int M, dim3 grid, dim3 block, hipStream_t st) {
-    return M == 8 ? step7_m<N, 8>(a, zt, grid, block, st) : step7_m<N, 4>(a, zt, grid, block, st);
+bool step7_n(const KArgs &a, const ZTab &zt, int M, bool xw, dim3 grid, dim3 block, hipStream_t st) {
+    if (xw) return M == 8 ? step7_m<N, 8, true>(a, zt, grid, block, st) : step7_m<N, 4, true>(a, zt, grid, block, st);
+    return M == 8 ? step7_m<N, 8, false>(a, zt, grid, block, st) : step7_m<N, 4, false>(a, zt, grid, block, st);
 }
 
 }  // namespace
 
-bool launch_step7(const KArgs &a, const ZTab &zt, int N, int tz, hipStream_t st) {
+bool launch_step7(const KArgs &a, const ZTab &zt, int N, int tz, bool xw, hipStream_t st) {
     const int R = 2 * N + 2, M = tz >= 8 * R ? 8 : 4, TZ = R * M;
     const int nxi = a.nx - 2 * N, nzi = a.nz - 2 * N, outw = kBX - 4 * N;
-    const int segs = (nzi + TZ - 1) / TZ;
-    const dim3 grid(((nxi + outw - 1) / outw) * ((segs + kBY - 1) / kBY) * a.B), block(kBX, kBY);
+    const int segs = (nzi + TZ - 1) / TZ, tiles = (nxi + outw - 1) / outw;
+    // grid as tile_of<TZ, OUTW, XW> counts it
+    const int nblk = xw ? ((tiles + kBY - 1) / kBY) * segs : tiles * ((segs + kBY - 1) / kBY);
+    const dim3 grid(nblk * a.B), block(kBX, kBY);
     switch (N) {
-        case 1: return step7_n<1>(a, zt, M, grid, block, st);
-        case 2: return step7_n<2>(a, zt, M, grid, block, st);
-        case 3: return step7_n<3>(a, zt, M, grid, block, st);
-        case 4: return step7_n<4>(a, zt, M, grid, block, st);
-        case 5: return step7_n<5>(a, zt, M, grid, block, st);
-        default: return step7_n<6>(a, zt, M, grid, block, st);
+        case 1: return step7_n<1>(a, zt, M, xw, grid, block, st);
+        case 2: return step7_n<2>(a, zt, M, xw, grid, block, st);
+        case 3: return step7_n<3>(a, zt, M, xw, grid, block, st);
+        case 4: return step7_n<4>(a, zt, M, xw, grid, block, st);
+        case 5: return step7_n<5>(a, zt, M, xw, grid, block, st);
+        default: return step7_n<6>(a, zt, M, xw, grid, block, st);
     }
 }
 

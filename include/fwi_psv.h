@@ -126,7 +126,7 @@ typedef struct {
     int32_t tile_tz;        /* rows per thread in the stencil kernels; 0 = tuned;
                                8, 16, 32, 64 (variants 1-3), 32, 64 (variant 4), 16, 32
                                (variant 5), fdorder + 2 or 2 fdorder + 4 (variant 6),
-                               4 fdorder + 8 or 8 fdorder + 16 (variant 7) */
+                               4 fdorder + 8 or 8 fdorder + 16 (variants 7, 8) */
     int32_t device;         /* HIP device ordinal */
     int32_t check_finite;   /* scan the wavefield for NaN/Inf after each propagation */
     int32_t kernel_variant; /* stencil kernels: 0 = tuned, 1 = x taps as shifted loads,
@@ -137,8 +137,9 @@ typedef struct {
                                    receivers inside the interior),
                                5 = variant 2 with wave-uniform addressing and ordered loads,
                                6 = variant 5 with every load issued one row ahead,
-                               7 = variant 4 built as variant 6 (variant 4's conditions)
-                               (5, 6 and 7 need z CPML strips of at most 64 rows, at the
+                               7 = variant 4 built as variant 6 (variant 4's conditions),
+                               8 = variant 7 with a block's wavefronts side by side in x
+                               (5 to 8 need z CPML strips of at most 64 rows, at the
                                top and bottom of the grid) */
     int32_t work_order;     /* order of the stencil tiles over the grid: 0 = tuned,
                                1 = shot fastest then z then x, 2 = shot fastest then x then z,

@@ -85,7 +85,7 @@ def test_shot_gradient_matches_reference_golden(psv, name):
 
 
 @pytest.mark.parametrize("surface", [True, False])
-@pytest.mark.parametrize("variant", [1, 2, 4, 5, 6, 7])
+@pytest.mark.parametrize("variant", [1, 2, 4, 5, 6, 7, 8])
 def test_order8_forward_vs_oracle(psv, O, surface, variant):
     c = cases.small_case(O, surface=surface, nshot=2)
     med = O.mat_av2(c)
@@ -100,7 +100,7 @@ def test_order8_forward_vs_oracle(psv, O, surface, variant):
         assert e < 1e-4
 
 
-@pytest.mark.parametrize("variant", [1, 2, 4, 5, 6, 7])
+@pytest.mark.parametrize("variant", [1, 2, 4, 5, 6, 7, 8])
 def test_order8_gradient_vs_oracle(psv, O, variant):
     c = cases.small_case(O, nshot=2, nt=450)
     med_true = O.mat_av2(c)
@@ -126,7 +126,7 @@ def test_order8_gradient_vs_oracle(psv, O, variant):
             assert e < 2e-3
 
 
-@pytest.mark.parametrize("variant", [1, 4, 5, 6, 7])
+@pytest.mark.parametrize("variant", [1, 4, 5, 6, 7, 8])
 def test_checkpointed_equals_full_storage(psv, O, variant):
     c = cases.small_case(O, nshot=2, nt=300, snap_dt=2)
     med_true = O.mat_av2(c)
@@ -178,7 +178,7 @@ def test_c4_window_checkpointed_equals_full_storage(psv, O):
             np.testing.assert_array_equal(G[i], ref[1][i], err_msg=str(key))
 
 
-@pytest.mark.parametrize("variant", [1, 4, 5, 6, 7])
+@pytest.mark.parametrize("variant", [1, 4, 5, 6, 7, 8])
 def test_batched_uneven_equals_per_shot(psv, O, variant):
     c = cases.small_case(O, nshot=4, nt=260, uneven=True)
     med_true = O.mat_av2(c)
@@ -335,10 +335,12 @@ def test_uniform_addressing_variant_matches_variant2(psv, O, fdorder, variant, b
         assert e < 1e-4 and e2 < 1e-4
 
 
-@pytest.mark.parametrize("variant,tz,fdorder", [(4, 0, 8), (7, 4, 8), (7, 8, 8), (7, 4, 2), (7, 8, 4)])
+@pytest.mark.parametrize("variant,tz,fdorder", [(4, 0, 8), (7, 4, 8), (7, 8, 8), (7, 4, 2), (7, 8, 4),
+                                                 (8, 4, 8), (8, 8, 8), (8, 4, 2)])
 def test_one_pass_step_matches_two_kernel_step(psv, O, variant, tz, fdorder):
     """The one-pass variants (stress, surface and velocity in one pass, two time
-    levels; 7 with segments of tz blocks of fdorder + 2 rows) against variant 1:
+    levels; 7 and 8 with segments of tz blocks of fdorder + 2 rows, 8 with a
+    block's wavefronts side by side in x) against variant 1:
     seismograms, stored snapshots (the source cells are rewritten by the
     injection), the shot gradient and the misfit-derivative sums, over uneven
     shots in one batch.  Same arithmetic, so the differences are FP32 contraction
@@ -349,7 +351,7 @@ def test_one_pass_step_matches_two_kernel_step(psv, O, variant, tz, fdorder):
     lam0, mu0, rho0 = cases.start_model(c)
     out = {}
     for v in (1, variant):
-        t = 0 if v != 7 else tz * (fdorder + 2)
+        t = tz * (fdorder + 2) if v in (7, 8) else 0
         with psv.Simulation(c, kernel_variant=v, tile_tz=t, storage=psv._lib.STORE_FULL) as sim:
             seis = sim.forward()
             _, snaps = sim.forward_snapshots(1)
@@ -391,7 +393,7 @@ def test_work_orders_agree_bitwise(psv, O):
             np.testing.assert_array_equal(ux, ref[2])
 
 
-@pytest.mark.parametrize("variant", [1, 4, 5, 6, 7])
+@pytest.mark.parametrize("variant", [1, 4, 5, 6, 7, 8])
 @pytest.mark.parametrize("storage", ["full", "ckpt"])
 def test_graph_replay_equals_eager(psv, O, variant, storage):
     """use_graph: chunks of time steps captured once per propagation and replayed
@@ -450,7 +452,7 @@ def test_c3_grid_gradient_vs_fp64_oracle(psv, O):
         assert eg < 1e-4, (k, eg)
 
 
-@pytest.mark.parametrize("variant", [4, 7])
+@pytest.mark.parametrize("variant", [4, 7, 8])
 def test_one_pass_step_refused_where_it_does_not_apply(psv, O, variant):
     c = cases.small_case(O, nshot=1, nt=20)
     c["isurf"] = np.array([int(c["fdorder"]) // 2, int(c["nz"]) - 20, 0, 0], np.int32)   # a bottom surface too

@@ -1,6 +1,7 @@
 // psv_device.h -- device helpers shared by the stencil translation units.
 #pragma once
 
+#include <type_traits>
 #include <utility>
 
 #include "psv_internal.h"
@@ -108,6 +109,8 @@ __device__ __forceinline__ T karg_at(KSeg seg, uint32_t off) {
     return *(const __attribute__((address_space(4))) T *)(seg + off);
 }
 #define PSV_KARG(seg, field) (karg_at<decltype(KArgs::field)>(seg, offsetof(KArgs, field)))
+static_assert(std::is_standard_layout<KArgs>::value && std::is_trivially_copyable<KArgs>::value,
+              "KArgs is passed by value and read back at offsetof() positions");
 
 // z-strip test and strip row (the psi_z row) of a wave-uniform row
 __device__ __forceinline__ bool in_zstrip(const ZTab &zt, int z) { return z < zt.lo || z >= zt.hi; }

@@ -326,7 +326,8 @@ __device__ __forceinline__ void step7_rows(const KArgs &a, const ZTab &zt, const
         });
 }
 
-// kVelAdjoint launches branch once, on the step's correlation switch
+// kVelAdjoint launches branch once, on the step's correlation switch.  KArgs stays the
+// first kernel argument: the CPML blocks read its pointers back at segment offset 0.
 template <int N, int M, int MODE, bool XW>
 __global__ __launch_bounds__(kBX *kBY) void k_step7(const KArgs a, const ZTab zt) {
     const StepView sv = step_view(a, 0);

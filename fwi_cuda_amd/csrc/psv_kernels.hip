@@ -831,7 +831,7 @@ __global__ __launch_bounds__(kBX *kBY) void k_velocity5(const KArgs a, const ZTa
 // read from one buffer set and written to the other (the host swaps them each
 // step).  The velocity-side memory is only touched by the owning lane and is
 // updated in place.  Compulsory traffic per cell-step: R+W five fields, R five
-// media, R+W We (68 B), the survey's single-pass byte count.
+// media, R+W We (68 B): each array read and written once.
 // ---------------------------------------------------------------------------
 // MODE bits (compile time, so each launch mode keeps only the pointers it uses):
 // 1 = accumulate We, 2 = adjoint (u += dt v, gradient on snapshot steps), 4 = store

@@ -21,7 +21,7 @@
 // stress-side CPML memory are read at the old level and written at the new one
 // (the host swaps the levels every step); the velocity-side memory, u and We are
 // updated in place by their owner.  Compulsory traffic per cell-step: 68 B forward
-// (R+W five fields, R five media, R+W We), the survey's single-pass count.
+// (R+W five fields, R five media, R+W We): each array read and written once.
 //
 // A block's kBY wavefronts are kBY consecutive z segments of one column tile, or, in
 // variant 8 (XW), kBY consecutive column tiles of one z segment: the block then reads

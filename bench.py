@@ -220,7 +220,7 @@ def main():
     ap.add_argument("--ckpt-interval", type=int, default=0)
     ap.add_argument("--tile-tz", type=int, default=0)
     ap.add_argument("--variant", type=int, default=0, help="stencil kernel variant (0 = tuned)")
-    ap.add_argument("--order", type=int, default=0, help="stencil work order 1..3 (0 = tuned)")
+    ap.add_argument("--order", type=int, default=0, help="stencil work order 1..4 (0 = tuned)")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-graph", action="store_true", help="skip the hipGraph-replay gradient")

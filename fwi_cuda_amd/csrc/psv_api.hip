@@ -580,7 +580,7 @@ int autotune(fwi_ctx *c, int nb) {
     if (c->plain || c->tuned) return FWI_OK;
     const TuneKey key{c->g.nz, c->g.nx, c->N, nb};
     const int fv = c->o.kernel_variant;   // 0 = any
-    const bool order_forced = c->o.work_order >= 1 && c->o.work_order <= 3;
+    const bool order_forced = c->o.work_order >= 1 && c->o.work_order <= 4;
     // candidates (variant, rows per segment, prefetch depth); the one-pass step only where it applies
     std::vector<TuneVal> cand;
     for (int var = 1; var <= 8; ++var) {
@@ -657,7 +657,7 @@ int autotune(fwi_ctx *c, int nb) {
         }
     };
     float best = 1e30f;
-    const int o_lo = order_forced ? c->o.work_order - 1 : 0, o_hi = order_forced ? o_lo + 1 : 3;
+    const int o_lo = order_forced ? c->o.work_order - 1 : 0, o_hi = order_forced ? o_lo + 1 : 4;
     TuneVal bv = cand[0];
     bv.order = o_lo;
     for (int order = o_lo; order < o_hi; ++order) {
@@ -1244,8 +1244,8 @@ int validate(fwi_ctx *c, const fwi_grid *g, const fwi_acq *q) {
                        "variants 7, 8: 0 or their row block times 4 or 8)", tt);
     if (c->o.kernel_variant < 0 || c->o.kernel_variant > 8)
         return c->fail(FWI_EINVAL, "kernel_variant %d (0..8)", c->o.kernel_variant);
-    if (c->o.work_order < 0 || c->o.work_order > 3)
-        return c->fail(FWI_EINVAL, "work_order %d (0..3)", c->o.work_order);
+    if (c->o.work_order < 0 || c->o.work_order > 4)
+        return c->fail(FWI_EINVAL, "work_order %d (0..4)", c->o.work_order);
     if (g->snap_dt < 1 || g->snap_dz < 1 || g->snap_dx < 1) return c->fail(FWI_EINVAL, "snap strides < 1");
     if (g->snap_z1 < 0 || g->snap_z2 >= g->nz || g->snap_z1 > g->snap_z2 || g->snap_x1 < 0 ||
         g->snap_x2 >= g->nx || g->snap_x1 > g->snap_x2)

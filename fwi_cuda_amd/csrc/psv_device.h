@@ -67,11 +67,16 @@ __device__ __forceinline__ void tile_of(const KArgs &a, int N, int &bx, int &by,
         const int t = W / a.B;
         bx = t % gx;
         by = t / gx;
-    } else {                     // x tiles fastest, then z segments, shots outermost
+    } else if (a.order == 2) {   // x tiles fastest, then z segments, shots outermost
         const int t = W % (gx * gy);
         s = W / (gx * gy);
         bx = t % gx;
         by = t / gx;
+    } else {                     // x tiles fastest, then shots, z segments outermost
+        bx = W % gx;
+        const int t = W / gx;
+        s = t % a.B;
+        by = t / a.B;
     }
 }
 

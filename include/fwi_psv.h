@@ -143,7 +143,8 @@ typedef struct {
                                top and bottom of the grid) */
     int32_t work_order;     /* order of the stencil tiles over the grid: 0 = tuned,
                                1 = shot fastest then z then x, 2 = shot fastest then x then z,
-                               3 = x tiles fastest, shots outermost */
+                               3 = x tiles fastest, shots outermost,
+                               4 = x tiles fastest, then shots, z segments outermost */
 } fwi_opts;
 
 /* Per-kernel timing collected while profiling is on (HIP events on the

@@ -126,7 +126,7 @@ def test_invalid_options_are_rejected_before_any_device_work():
     from fwi_cuda_amd import _lib, psv
     O.load("oracle")
     c = cases.small_case(O, nshot=1, nt=10)
-    for kw in ({"tile_tz": 5}, {"kernel_variant": 9}, {"work_order": 4}):
+    for kw in ({"tile_tz": 5}, {"kernel_variant": 9}, {"work_order": 5}):
         with pytest.raises(_lib.FwiError) as e:
             psv.Simulation(c, **kw)
         assert e.value.status == -1, kw

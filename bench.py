@@ -19,7 +19,9 @@ and replayed (the library's use_graph mode, wall clock only), is reported as
 memory, which alone moves kernel times by up to about 10 %.  With
   roofline      the dominant kernel's algorithmic bytes per launch over its
                 average duration (HIP events on the library's stream), against
-                MI355X's 8 TB/s;
+                MI355X's 8 TB/s, and beside two measured ceilings: torch's
+                elementwise multiply (copy_GBps) and stream_probe, a kernel with
+                the stencil's access shape and no arithmetic (shape_GBps);
   hbm_frac      the whole step's algorithmic bytes (fwd 68 B, adjoint 84 B,
                 gradient 44 B and snapshot 20 B per snapshot step, recomputed
                 step 60 B per cell-step, CPML strips 32 B per cell-step per
@@ -129,6 +131,29 @@ def copy_bandwidth(nbytes=1 << 30, reps=10):
         return 8.0 * n / (ms / 1e3) / 1e9
     except Exception as e:   # the probe is a reported reference, not the measurement
         log(f"copy probe failed: {e!r}")
+        return None
+
+
+def shape_bandwidth(nz, P, B):
+    """A second measured ceiling: GB/s of fwi_cuda_amd/stream_probe, a kernel with the
+    stencil's access shape and no arithmetic (10 [B][nz][P] arrays read and 5 written,
+    a wavefront marching 64 columns down 40 rows, the one-pass step's block layout), run
+    as a child process; None when the probe is not built or fails."""
+    exe = ROOT / "fwi_cuda_amd" / "stream_probe"
+    if not exe.exists():
+        log("shape probe skipped: fwi_cuda_amd/stream_probe is not built")
+        return None
+    P = (P + 255) // 256 * 256
+    while B > 1 and nz * P * B * 4 >= 1 << 31:   # the probe keeps each array below 2 GiB
+        B //= 2
+    try:
+        r = subprocess.run([str(exe), str(nz), str(P), str(B)], capture_output=True, text=True, timeout=120)
+        if r.returncode != 0:
+            log(f"shape probe failed: {r.stderr.strip()}")
+            return None
+        return json.loads(r.stdout.strip().splitlines()[-1])["shape_GBps"]
+    except Exception as e:
+        log(f"shape probe failed: {e!r}")
         return None
 
 
@@ -380,6 +405,13 @@ def main():
         roofline["copy_GBps"] = round(copy, 1)
         roofline["copy_probe"] = "torch elementwise multiply, 1 GiB in, 1 GiB out, this GPU"
         roofline["frac_of_copy"] = round(achieved / copy, 4)
+    # the stencil's own access shape, on the batch's array size (after the context is closed)
+    shape = shape_bandwidth(nz, (nx + 1 + 63) // 64 * 64, int(plan["batch"]))
+    if shape:
+        roofline["shape_GBps"] = round(shape, 1)
+        roofline["shape_probe"] = ("fwi_cuda_amd/stream_probe: 10 [B][nz][P] arrays read, 5 written, 64-column "
+                                   "wavefronts marching 40 rows, no arithmetic, this GPU")
+        roofline["frac_of_shape"] = round(achieved / shape, 4)
 
     # The timed steps: the same gradient with chunks of time steps captured in hipGraphs
     # and replayed (per-kernel events do not exist inside a graph: wall clock only).

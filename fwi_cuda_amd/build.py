@@ -3,6 +3,8 @@
     python -m fwi_cuda_amd.build            # library
     python -m fwi_cuda_amd.build --driver   # also the fwi_psv host driver executable
 
+Both also build stream_probe, the access-shape bandwidth probe bench.py runs.
+
 Objects go to fwi_cuda_amd/_build/ (git-ignored); the .so lands next to this
 file so it travels with the source tree.
 """
@@ -22,6 +24,7 @@ CSRC = PKG / "csrc"
 BUILD = PKG / "_build"
 LIB = PKG / "libfwi_psv.so"
 DRIVER = PKG / "fwi_psv"
+PROBE = PKG / "stream_probe"   # bench.py's access-shape bandwidth probe
 
 HIP_SOURCES = ["psv_stencil6.hip", "psv_step7.hip", "psv_kernels.hip", "psv_misc.hip", "psv_plain.hip",
                "psv_api.hip"]
@@ -57,6 +60,7 @@ def build(driver: bool = False, verbose: bool = False) -> Path:
         jobs.append([cc, *COMMON, *HIP_FLAGS, "-c", str(CSRC / s), "-o", str(BUILD / (Path(s).stem + ".o"))])
     for s in CPP_SOURCES:
         jobs.append(["g++", *COMMON, "-Wall", "-c", str(CSRC / s), "-o", str(BUILD / (Path(s).stem + ".o"))])
+    jobs.append([cc, *COMMON, *HIP_FLAGS, str(CSRC / "stream_probe.hip"), "-o", str(PROBE)])
     with cf.ThreadPoolExecutor(max_workers=min(8, len(jobs))) as ex:
         for r in ex.map(_run, jobs):
             if verbose and r.stderr:

@@ -193,3 +193,22 @@ def test_velocity_parameter_chain_rule():
             dn[k][i] += h[i]
         # central differences of the summed functional: truncation and cancellation ~1e-5
         np.testing.assert_allclose(g, fd, rtol=1e-4, atol=1e-12 * np.abs(fd).max())
+
+
+def test_bench_names_the_tuned_instance_and_finds_its_traffic():
+    """bench.py's roofline names the kernel instance the tuner chose (as the profiler
+    names it) and picks that instance's counter traffic out of the committed summary:
+    the one-pass variants 7 and 8 differ only in the block arrangement flag."""
+    import sys
+    sys.path.insert(0, str(ROOT))
+    import bench
+    log = "chosen o2/v8/tz40/d1; o2/v7/tz40/d1 4.880; "
+    assert bench.kernel_instantiation("step", log, 4) == "k_step7<4, 4, *, true>"
+    assert bench.kernel_instantiation("step", "chosen o2/v7/tz80/d1; ", 4) == "k_step7<4, 8, *, false>"
+    assert bench.kernel_instantiation("step", "chosen o0/v4/tz32/d1; ", 4) == "k_step<4, 32, "
+    assert bench.kernel_instantiation("velocity", "chosen o1/v6/tz20/d1; ", 4) == "k_velocity6<4, 1, 2, "
+    assert bench.kernel_instantiation("stress", "chosen o2/v2/tz16/d1; ", 4) == "k_stress2<4, 16, 1>"
+    t7 = bench.measured_traffic("k_step7<4, 4, *, false>")
+    t8 = bench.measured_traffic("k_step7<4, 4, *, true>")
+    assert t7 and t8 and t8 < t7
+    assert bench.measured_traffic("k_step7<4, 8, *, true>") is None

@@ -84,6 +84,7 @@ def make_case(*, nt, nz, nx, dt, dz, dx, fdorder, lam, mu, rho, z_src, x_src, sh
              hc=hc, surf=int(any(s > 0 for s in isurf)), isurf=np.asarray(isurf, np.int32),
              pml_z=int(npml[0] > 0 or npml[1] > 0), pml_x=int(npml[2] > 0 or npml[3] > 0),
              npml=np.asarray(npml, np.int32), fc=float(fc),
+             cpml_par=(float(npower), float(damp_v), float(rcoef), float(k_max)),
              snap=np.asarray(snap, np.int32), taper=np.asarray(taper, np.int32),
              lam=lam, mu=mu, rho=rho, stf_type=int(stf_type),
              stf_z=np.ascontiguousarray(stf_z, np.float64), stf_x=np.ascontiguousarray(stf_x, np.float64),

@@ -84,3 +84,28 @@ def test_fwi_mode_with_descent_sign_writes_models(tmp_path):
     for it in range(2):
         m = np.fromfile(tmp_path / "bin" / f"iter{it}_mat.bin", np.float64)
         assert m.size == 3 * nz * nx and np.isfinite(m).all()
+
+
+def test_driver_runs_input_files_written_from_a_workload(tmp_path):
+    """fwi_cuda_amd.refio writes a seeded workload (C1, shortened) as the reference's
+    input files; the driver's forward mode reads them and writes seismograms equal
+    to the library's forward of the same case with the reference's horizontal force
+    (src/h_simulate_PSV.cpp:183-184)."""
+    from fwi_cuda_amd import configs, psv, refio
+    nt = 300
+    c = configs.c1(nt=nt)
+    meta, arrays = refio.inputs_from_case(c)
+    refio.write_inputs(tmp_path, meta, arrays, c["lam"], c["mu"], c["rho"])
+    r = _run(tmp_path)
+    assert r.returncode == 0, r.stderr
+    h = dict(c)
+    w = psv.wavelet(nt, c["dt"], 1.0, c["fc"])
+    h["stf_z"] = np.zeros_like(c["stf_z"])
+    h["stf_x"] = np.tile(w, (len(c["z_src"]), 1))
+    with psv.Simulation(h) as sim:
+        uz, ux = sim.forward()
+    for comp, ref in (("uz", uz), ("ux", ux)):
+        got = np.fromfile(tmp_path / "bin" / f"shot0_rtf_{comp}.bin", np.float64).reshape(-1, nt)
+        e = cases.rel_l2(got, ref)
+        print(f"driver on refio inputs, {comp}: rel-L2 vs the library's forward {e:.2e}", flush=True)
+        assert e < 1e-6

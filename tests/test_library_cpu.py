@@ -212,3 +212,48 @@ def test_bench_names_the_tuned_instance_and_finds_its_traffic():
     t8 = bench.measured_traffic("k_step7<4, 4, *, true>")
     assert t7 and t8 and t8 < t7
     assert bench.measured_traffic("k_step7<4, 8, *, true>") is None
+
+
+def test_reference_input_files_round_trip_the_fixture(tmp_path):
+    """fwi_cuda_amd.refio reads the input files the reference executable consumed
+    when the fwi_bin fixture was made (metaint, intarray, metafloat, mat in the
+    layout of scripts/pre_proc_fwi.py:240-276) and writes them back byte for byte."""
+    from conftest import load_fixture
+    from fwi_cuda_amd import refio
+    fx = load_fixture("fwi_bin")
+    src, dst = tmp_path / "src", tmp_path / "dst"
+    (src / "bin").mkdir(parents=True)
+    files = fx["files_fwd"]
+    files["metaint"].astype(np.int32).tofile(src / "bin" / "metaint.bin")
+    files["intarray"].astype(np.int32).tofile(src / "bin" / "intarray.bin")
+    files["metafloat"].astype(np.float64).tofile(src / "bin" / "metafloat.bin")
+    lam, mu, rho = (fx["true"][k].astype(np.float64) for k in ("lam", "mu", "rho"))
+    np.concatenate([lam.ravel(), mu.ravel(), rho.ravel()]).tofile(src / "bin" / "mat.bin")
+    meta, arrays, l2, m2, r2 = refio.read_inputs(src)
+    assert (meta["nt"], meta["nz"], meta["nx"], meta["nsrc"], meta["nrec"], meta["nshot"], meta["fdorder"]) == \
+        (250, 40, 56, 2, 5, 2, 2)
+    assert list(arrays["x_rec"]) == [12, 20, 28, 36, 44] and meta["fpad"] == 1
+    np.testing.assert_array_equal(l2, lam)
+    refio.write_inputs(dst, meta, arrays, l2, m2, r2)
+    for f in ("metaint", "intarray", "metafloat", "mat"):
+        assert (src / "bin" / f"{f}.bin").read_bytes() == (dst / "bin" / f"{f}.bin").read_bytes(), f
+
+
+def test_reference_input_files_from_a_workload(tmp_path):
+    """A seeded workload (C1) as the reference's input files: the metadata carries
+    its grid, acquisition, snapshot box, taper and CPML parameters; a workload
+    whose shots have different receivers (C3) is refused."""
+    from fwi_cuda_amd import configs, refio
+    c = configs.c1(nt=30)
+    meta, arrays = refio.inputs_from_case(c, fwinv=True)
+    refio.write_inputs(tmp_path, meta, arrays, c["lam"], c["mu"], c["rho"])
+    m2, a2, lam, mu, rho = refio.read_inputs(tmp_path)
+    assert m2 == meta
+    for k in refio.ARRAYS:
+        np.testing.assert_array_equal(a2[k], arrays[k])
+    np.testing.assert_array_equal(lam, c["lam"])
+    assert (m2["nz"], m2["nx"], m2["nrec"], m2["nshot"], m2["fdorder"], m2["surf"]) == (201, 401, 40, 1, 8, 1)
+    assert list(a2["isurf"]) == [4, 0, 0, 0] and list(a2["npml"]) == [0, 20, 20, 20]
+    assert m2["freq_pml"] == c["fc"] and m2["dt"] == c["dt"] and m2["fwinv"] == 1
+    with pytest.raises(ValueError):
+        refio.inputs_from_case(configs.c3(nt=10, nshot=3, nz=101, nx=201))

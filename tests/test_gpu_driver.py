@@ -109,3 +109,18 @@ def test_driver_runs_input_files_written_from_a_workload(tmp_path):
         e = cases.rel_l2(got, ref)
         print(f"driver on refio inputs, {comp}: rel-L2 vs the library's forward {e:.2e}", flush=True)
         assert e < 1e-6
+
+
+def test_stream_probe_reports_its_bandwidth():
+    """bench.py's access-shape probe runs, checks its arguments and prints one JSON
+    line whose byte count is the 15 arrays it streams."""
+    import json
+    probe = ROOT / "fwi_cuda_amd" / "stream_probe"
+    assert probe.exists(), "build the library: python -m fwi_cuda_amd.build"
+    r = subprocess.run([str(probe), "201", "512", "4"], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    d = json.loads(r.stdout.strip().splitlines()[-1])
+    assert d["bytes_per_launch"] == 15 * 201 * 512 * 4 * 4
+    assert d["shape_GBps"] > 100
+    bad = subprocess.run([str(probe), "201", "500", "4"], capture_output=True, text=True, timeout=60)
+    assert bad.returncode == 2

@@ -212,6 +212,9 @@ def test_bench_names_the_tuned_instance_and_finds_its_traffic():
     t8 = bench.measured_traffic("k_step7<4, 4, *, true>")
     assert t7 and t8 and t8 < t7
     assert bench.measured_traffic("k_step7<4, 8, *, true>") is None
+    for log in ("chosen o2/v5/tz16/d1; ", "chosen o2/v6/tz20/d1; "):
+        inst = bench.kernel_instantiation("velocity", log, 4)
+        assert bench.measured_traffic(inst) > 4.5e9, inst
 
 
 def test_reference_input_files_round_trip_the_fixture(tmp_path):

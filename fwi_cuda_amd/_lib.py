@@ -19,6 +19,8 @@ STATUS = {0: "FWI_OK", -1: "FWI_EINVAL", -2: "FWI_ENOMEM", -3: "FWI_EHIP", -4: "
           -5: "FWI_ECFL", -6: "FWI_ESTEP", -7: "FWI_ESTATE", -8: "FWI_ENODEV"}
 STORE_AUTO, STORE_FULL, STORE_CHECKPOINT = 0, 1, 2
 FORM_FUSED, FORM_PLAIN = 0, 1
+PARAM_LAME, PARAM_VELOCITY = 0, 1
+ABI_VERSION = 2
 KERNELS = ("stress", "velocity", "surface", "source", "receiver", "store", "other", "step", "graph")
 
 
@@ -42,7 +44,8 @@ class FwiOpts(C.Structure):
     _fields_ = [("batch", C.c_int32), ("storage", C.c_int32), ("ckpt_interval", C.c_int32),
                 ("mem_budget", C.c_int64), ("form", C.c_int32), ("grad_sign", C.c_int32),
                 ("use_graph", C.c_int32), ("tile_tz", C.c_int32), ("device", C.c_int32),
-                ("check_finite", C.c_int32), ("kernel_variant", C.c_int32), ("work_order", C.c_int32)]
+                ("check_finite", C.c_int32), ("kernel_variant", C.c_int32), ("work_order", C.c_int32),
+                ("param", C.c_int32), ("vel_bounds", C.c_double * 4)]
 
 
 class FwiKernelTime(C.Structure):
@@ -107,7 +110,7 @@ def load() -> C.CDLL:
         fn = getattr(lib, name)
         fn.restype = res
         fn.argtypes = args
-    if lib.fwi_abi_version() != 1:
+    if lib.fwi_abi_version() != ABI_VERSION:
         raise RuntimeError("libfwi_psv.so ABI version mismatch")
     _lib = lib
     return lib

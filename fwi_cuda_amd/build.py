@@ -2,8 +2,8 @@
 
     python -m fwi_cuda_amd.build            # library
     python -m fwi_cuda_amd.build --driver   # also the fwi_psv host driver executable
-
-Both also build stream_probe, the access-shape bandwidth probe bench.py runs.
+    python -m fwi_cuda_amd.build --changed  # recompile only objects older than their
+                                            # source or a header of csrc/ or include/
 
 Objects go to fwi_cuda_amd/_build/ (git-ignored); the .so lands next to this
 file so it travels with the source tree.
@@ -24,7 +24,6 @@ CSRC = PKG / "csrc"
 BUILD = PKG / "_build"
 LIB = PKG / "libfwi_psv.so"
 DRIVER = PKG / "fwi_psv"
-PROBE = PKG / "stream_probe"   # bench.py's access-shape bandwidth probe
 
 HIP_SOURCES = ["psv_stencil6.hip", "psv_step7.hip", "psv_kernels.hip", "psv_misc.hip", "psv_plain.hip",
                "psv_api.hip"]
@@ -52,28 +51,43 @@ def _run(cmd):
     return r
 
 
-def build(driver: bool = False, verbose: bool = False) -> Path:
+def _stale(src: Path, obj: Path) -> bool:
+    """An object is stale when it is missing or older than its source or any header."""
+    if not obj.exists():
+        return True
+    t = obj.stat().st_mtime
+    headers = list(CSRC.glob("*.h")) + list((ROOT / "include").glob("*.h")) + list((CSRC / "host").glob("*.h"))
+    return any(p.stat().st_mtime > t for p in [src, *headers])
+
+
+def build(driver: bool = False, verbose: bool = False, changed_only: bool = False) -> Path:
     BUILD.mkdir(exist_ok=True)
     cc = hipcc()
     jobs = []
     for s in HIP_SOURCES:
-        jobs.append([cc, *COMMON, *HIP_FLAGS, "-c", str(CSRC / s), "-o", str(BUILD / (Path(s).stem + ".o"))])
+        o = BUILD / (Path(s).stem + ".o")
+        if not changed_only or _stale(CSRC / s, o):
+            jobs.append([cc, *COMMON, *HIP_FLAGS, "-c", str(CSRC / s), "-o", str(o)])
     for s in CPP_SOURCES:
-        jobs.append(["g++", *COMMON, "-Wall", "-c", str(CSRC / s), "-o", str(BUILD / (Path(s).stem + ".o"))])
-    jobs.append([cc, *COMMON, *HIP_FLAGS, str(CSRC / "stream_probe.hip"), "-o", str(PROBE)])
-    with cf.ThreadPoolExecutor(max_workers=min(8, len(jobs))) as ex:
-        for r in ex.map(_run, jobs):
-            if verbose and r.stderr:
-                print(r.stderr, file=sys.stderr)
+        o = BUILD / (Path(s).stem + ".o")
+        if not changed_only or _stale(CSRC / s, o):
+            jobs.append(["g++", *COMMON, "-Wall", "-c", str(CSRC / s), "-o", str(o)])
+    if jobs:
+        with cf.ThreadPoolExecutor(max_workers=min(8, len(jobs))) as ex:
+            for r in ex.map(_run, jobs):
+                if verbose and r.stderr:
+                    print(r.stderr, file=sys.stderr)
     objs = [str(BUILD / (Path(s).stem + ".o")) for s in HIP_SOURCES + CPP_SOURCES]
-    tmp = LIB.with_suffix(".so.tmp")
-    _run([cc, *HIP_FLAGS, "-shared", "-fPIC", "-o", str(tmp), *objs])
-    os.replace(tmp, LIB)
+    if jobs or not LIB.exists():
+        tmp = LIB.with_suffix(".so.tmp")
+        _run([cc, *HIP_FLAGS, "-shared", "-fPIC", "-o", str(tmp), *objs])
+        os.replace(tmp, LIB)
     if driver:
         dobjs = []
         for s in DRIVER_SOURCES:
             o = BUILD / (Path(s).stem + ".o")
-            _run(["g++", *COMMON, "-Wall", "-c", str(CSRC / s), "-o", str(o)])
+            if not changed_only or _stale(CSRC / s, o):
+                _run(["g++", *COMMON, "-Wall", "-c", str(CSRC / s), "-o", str(o)])
             dobjs.append(str(o))
         _run(["g++", "-o", str(DRIVER), *dobjs, f"-L{PKG}", "-lfwi_psv", f"-Wl,-rpath,$ORIGIN"])
     return LIB
@@ -82,6 +96,7 @@ def build(driver: bool = False, verbose: bool = False) -> Path:
 if __name__ == "__main__":
     ap = argparse.ArgumentParser()
     ap.add_argument("--driver", action="store_true")
+    ap.add_argument("--changed", action="store_true")
     ap.add_argument("-v", "--verbose", action="store_true")
     a = ap.parse_args()
-    print(build(driver=a.driver, verbose=a.verbose))
+    print(build(driver=a.driver, verbose=a.verbose, changed_only=a.changed))

@@ -138,24 +138,27 @@ def c4(nt=8000, seed=4, nshot=16, nz=2001, nx=4001):
     return _case(nz, nx, 0.5, nt, cp, cs, rho, src, rec, 100.0, seed)
 
 
-def c5(nt=4000, seed=5, nshot=32, true_model=True, nz=1001, nx=2001):
+def c5(nt=4000, seed=5, nshot=32, true_model=True, nz=1001, nx=2001, nrec=150):
+    """C5; a smaller nz x nx keeps the layout with the anomalies and the start model's
+    smoothing scaled by nz / 1001 (the seeded draws are the same at full size)."""
     from scipy.ndimage import gaussian_filter
+    f_sz = nz / 1001.0
     cp = layers(nz, nx, 8, 2000.0, 4500.0, seed)
     cs = cp / np.sqrt(3.0)
     rng = np.random.default_rng(seed)
     zz, xx = np.mgrid[0:nz, 0:nx]
     for k in range(3):   # three elliptical anomalies, +8 % / -8 % / +8 % in cp and cs
         zc, xc = rng.integers(nz // 5, nz - nz // 4), rng.integers(nx // 5, nx - nx // 5)
-        a, b = rng.integers(30, 80), rng.integers(40, 120)
+        a, b = rng.integers(30, 80) * f_sz, rng.integers(40, 120) * f_sz
         m = ((zz - zc) / a) ** 2 + ((xx - xc) / b) ** 2 < 1
         f = 1.08 if k % 2 == 0 else 0.92
         cp[m] *= f
         cs[m] *= f
     if not true_model:   # the start model: the true model smoothed with a 40-cell Gaussian
-        cp = gaussian_filter(cp, 40.0, mode="nearest")
-        cs = gaussian_filter(cs, 40.0, mode="nearest")
+        cp = gaussian_filter(cp, 40.0 * f_sz, mode="nearest")
+        cs = gaussian_filter(cs, 40.0 * f_sz, mode="nearest")
     src = np.linspace(NPML + 10, nx - NPML - 11, nshot).astype(np.int32)
-    rec = np.linspace(NPML, nx - NPML - 1, 150).astype(np.int32)
+    rec = np.linspace(NPML, nx - NPML - 1, nrec).astype(np.int32)
     return _case(nz, nx, 0.5, nt, cp, cs, gardner(cp), src, rec, 150.0, seed)
 
 

@@ -68,6 +68,8 @@ struct fwi_ctx {
     double *m_lam = nullptr, *m_mu = nullptr, *m_rho = nullptr, *o_lam = nullptr, *o_mu = nullptr, *o_rho = nullptr;
     double *G[3] = {nullptr, nullptr, nullptr};
     double *tmp = nullptr, *red = nullptr;  // precondition scratch, reduction scratch
+    double *vel = nullptr;   // FWI_PARAM_VELOCITY: [4][nz][nx] cp, cs of the saved model, cp / cs gradient
+    double vb[4] = {1000.0, 6000.0, 500.0, 3500.0};   // cp_min, cp_max, cs_min, cs_max
     int *zmap = nullptr, *xmap = nullptr, *nonfinite = nullptr;
     float *pf[12] = {};   // b a 1/K  b_half a_half 1/K_half, z then x
     CpmlPoint *zp = nullptr, *xp = nullptr;   // the same, packed per grid line
@@ -931,6 +933,7 @@ Need need_bytes(const fwi_ctx *c, int storage, int S) {
     Need n{};
     n.shared = 5 * fs * 4 + 9 * nz * nx * 8 + 2 * nz * nx * 8 + (size_t)c->nrec_tot * nt * 8 * 6 +
                (size_t)c->nsrc_tot * nt * 8 * 2 + (64 << 20);
+    if (c->o.param == FWI_PARAM_VELOCITY) n.shared += 4 * nz * nx * 8;
     const size_t state_ck = 5 * fs * 4 + 4 * (c->psiz() + c->psix()) * 4;
     n.per_shot = 9 * fs * 4 + 8 * (c->psiz() + c->psix()) * 4 + 5 * snapcell * 4;
     if (c->plain) n.per_shot += 12 * fs * 4;
@@ -1012,6 +1015,7 @@ int allocate(fwi_ctx *c) {
     for (int i = 0; i < 3; ++i) c->G[i] = D(nn);
     c->tmp = D(2 * nn);
     c->red = D(4 + 3 * 1024);
+    if (c->o.param == FWI_PARAM_VELOCITY) c->vel = D(4 * nn);
     c->nonfinite = I(1);
     c->zmap = I(nz);
     c->xmap = I(nx);
@@ -1246,6 +1250,16 @@ int validate(fwi_ctx *c, const fwi_grid *g, const fwi_acq *q) {
         return c->fail(FWI_EINVAL, "kernel_variant %d (0..8)", c->o.kernel_variant);
     if (c->o.work_order < 0 || c->o.work_order > 4)
         return c->fail(FWI_EINVAL, "work_order %d (0..4)", c->o.work_order);
+    if (c->o.param != FWI_PARAM_LAME && c->o.param != FWI_PARAM_VELOCITY)
+        return c->fail(FWI_EINVAL, "param %d (0 lam/mu/rho, 1 cp/cs)", c->o.param);
+    if (c->o.param == FWI_PARAM_VELOCITY) {
+        const double *b = c->o.vel_bounds;
+        if (b[0] != 0.0 || b[1] != 0.0 || b[2] != 0.0 || b[3] != 0.0) {
+            if (!(b[0] >= 0.0 && b[0] < b[1] && b[2] >= 0.0 && b[2] < b[3]))
+                return c->fail(FWI_EINVAL, "vel_bounds need 0 <= cp_min < cp_max and 0 <= cs_min < cs_max");
+            std::memcpy(c->vb, b, sizeof c->vb);
+        }
+    }
     if (g->snap_dt < 1 || g->snap_dz < 1 || g->snap_dx < 1) return c->fail(FWI_EINVAL, "snap strides < 1");
     if (g->snap_z1 < 0 || g->snap_z2 >= g->nz || g->snap_z1 > g->snap_z2 || g->snap_x1 < 0 ||
         g->snap_x2 >= g->nx || g->snap_x1 > g->snap_x2)
@@ -1689,6 +1703,10 @@ int32_t fwi_save_model(fwi_ctx *c) {
     HIPCHK(hipMemcpyAsync(c->o_lam, c->m_lam, n, hipMemcpyDeviceToDevice, c->st));
     HIPCHK(hipMemcpyAsync(c->o_mu, c->m_mu, n, hipMemcpyDeviceToDevice, c->st));
     HIPCHK(hipMemcpyAsync(c->o_rho, c->m_rho, n, hipMemcpyDeviceToDevice, c->st));
+    if (c->o.param == FWI_PARAM_VELOCITY) {   // cp, cs of the saved model: the update's starting point
+        const size_t nn = (size_t)c->g.nz * c->g.nx;
+        dev_velocity_of(c->o_lam, c->o_mu, c->o_rho, c->vel, c->vel + nn, nn, c->st);
+    }
     return sync(c);
 }
 
@@ -1696,6 +1714,23 @@ int32_t fwi_save_model(fwi_ctx *c) {
 int32_t fwi_update_model(fwi_ctx *c, double eps, double rho_factor, double rho_min) {
     if (!c || !c->have_model) return c ? c->fail(FWI_ESTATE, "no model") : FWI_EINVAL;
     const size_t n = (size_t)c->g.nz * c->g.nx;
+    if (c->o.param == FWI_PARAM_VELOCITY) {
+        // cp, cs gradient at the saved model, then update_mat2's factor and clipped step
+        // on each (src/cpu/n_kernel_lib_PSV.cpp:785-832) and lam, mu at the saved rho
+        double *cp = c->vel, *cs = c->vel + n, *gcp = c->vel + 2 * n, *gcs = c->vel + 3 * n;
+        dev_velocity_gradient(c->G[0], c->G[1], c->o_rho, cp, cs, gcp, gcs, n, c->st);
+        double f[2];
+        const double *vv[2] = {cp, cs}, *gg[2] = {gcp, gcs};
+        for (int p = 0; p < 2; ++p) {
+            double gmax = 0, am = 0;
+            RETIF(absmax_host(c, gg[p], n, &gmax));
+            RETIF(absmax_host(c, vv[p], n, &am));
+            if (am < c->vb[2 * p + 1]) am = c->vb[2 * p + 1];
+            f[p] = eps * (am / gmax);
+        }
+        dev_velocity_update(c->m_lam, c->m_mu, c->m_rho, c->o_rho, cp, cs, gcp, gcs, f[0], f[1], c->vb, n, c->st);
+        return refresh_media(c);
+    }
     double *m[3] = {c->m_lam, c->m_mu, c->m_rho};
     const double *old[3] = {c->o_lam, c->o_mu, c->o_rho};
     const double mmax[3] = {4.8e10, 2.7e10, 3000.0}, mmin[3] = {0.0, 0.0, rho_min};

@@ -193,6 +193,15 @@ void dev_taper(double *A, int nz, int nx, int z1, int z2, int x1, int x2, const 
                hipStream_t st);
 void dev_update(double *mat, const double *old, const double *G, double factor, double mat_max,
                 double mat_min, size_t n, hipStream_t st);
+// FWI_PARAM_VELOCITY: cp, cs of a model; the cp / cs gradient by the chain rule at fixed
+// rho; update_mat2 on cp and cs (f_cp, f_cs = step x update factor) and the new lam, mu.
+void dev_velocity_of(const double *lam, const double *mu, const double *rho, double *cp, double *cs, size_t n,
+                     hipStream_t st);
+void dev_velocity_gradient(const double *G_lam, const double *G_mu, const double *rho, const double *cp,
+                           const double *cs, double *g_cp, double *g_cs, size_t n, hipStream_t st);
+void dev_velocity_update(double *lam, double *mu, double *rho, const double *rho_old, const double *cp_old,
+                         const double *cs_old, const double *g_cp, const double *g_cs, double f_cp, double f_cs,
+                         const double bounds[4], size_t n, hipStream_t st);
 void dev_count_nonfinite(const float *x, size_t n, int *out, hipStream_t st);
 void dev_fill(float *p, float v, size_t n, hipStream_t st);
 

@@ -232,6 +232,42 @@ __global__ void k_update(double *mat, const double *old, const double *G, double
     }
 }
 
+// cp = sqrt((lam + 2 mu) / rho), cs = sqrt(mu / rho)
+__global__ void k_vel_of(const double *lam, const double *mu, const double *rho, double *cp, double *cs, size_t n) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        cp[i] = sqrt((lam[i] + 2.0 * mu[i]) / rho[i]);
+        cs[i] = sqrt(mu[i] / rho[i]);
+    }
+}
+
+// chain rule through lam = rho (cp^2 - 2 cs^2), mu = rho cs^2 at fixed rho
+__global__ void k_vel_grad(const double *Gl, const double *Gm, const double *rho, const double *cp, const double *cs,
+                           double *gcp, double *gcs, size_t n) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        gcp[i] = 2.0 * rho[i] * cp[i] * Gl[i];
+        gcs[i] = 2.0 * rho[i] * cs[i] * (Gm[i] - 2.0 * Gl[i]);
+    }
+}
+
+// update_mat2's clipped step on cp and cs, then lam and mu at the saved rho
+__global__ void k_vel_update(double *lam, double *mu, double *rho, const double *rho_old, const double *cp_old,
+                             const double *cs_old, const double *gcp, const double *gcs, double fcp, double fcs,
+                             double cp_lo, double cp_hi, double cs_lo, double cs_hi, size_t n) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        double cp = cp_old[i] + fcp * gcp[i];
+        if (cp > cp_hi) cp = cp_hi;
+        if (cp < cp_lo) cp = cp_lo;
+        double cs = cs_old[i] + fcs * gcs[i];
+        if (cs > cs_hi) cs = cs_hi;
+        if (cs < cs_lo) cs = cs_lo;
+        const double r = rho_old[i];
+        const double m = r * cs * cs;
+        mu[i] = m;
+        lam[i] = r * cp * cp - 2.0 * m;
+        rho[i] = r;
+    }
+}
+
 __global__ void k_nonfinite(const float *x, size_t n, int *out) {
     int c = 0;
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
@@ -315,6 +351,23 @@ void dev_taper(double *A, int nz, int nx, int z1, int z2, int x1, int x2, const 
 void dev_update(double *mat, const double *old, const double *G, double factor, double mat_max, double mat_min,
                 size_t n, hipStream_t st) {
     hipLaunchKernelGGL(k_update, dim3(nblocks(n)), dim3(kRB), 0, st, mat, old, G, factor, mat_max, mat_min, n);
+}
+
+void dev_velocity_of(const double *lam, const double *mu, const double *rho, double *cp, double *cs, size_t n,
+                     hipStream_t st) {
+    hipLaunchKernelGGL(k_vel_of, dim3(nblocks(n)), dim3(kRB), 0, st, lam, mu, rho, cp, cs, n);
+}
+
+void dev_velocity_gradient(const double *G_lam, const double *G_mu, const double *rho, const double *cp,
+                           const double *cs, double *g_cp, double *g_cs, size_t n, hipStream_t st) {
+    hipLaunchKernelGGL(k_vel_grad, dim3(nblocks(n)), dim3(kRB), 0, st, G_lam, G_mu, rho, cp, cs, g_cp, g_cs, n);
+}
+
+void dev_velocity_update(double *lam, double *mu, double *rho, const double *rho_old, const double *cp_old,
+                         const double *cs_old, const double *g_cp, const double *g_cs, double f_cp, double f_cs,
+                         const double bounds[4], size_t n, hipStream_t st) {
+    hipLaunchKernelGGL(k_vel_update, dim3(nblocks(n)), dim3(kRB), 0, st, lam, mu, rho, rho_old, cp_old, cs_old, g_cp,
+                       g_cs, f_cp, f_cs, bounds[0], bounds[1], bounds[2], bounds[3], n);
 }
 
 void dev_count_nonfinite(const float *x, size_t n, int *out, hipStream_t st) {

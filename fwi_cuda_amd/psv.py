@@ -151,7 +151,7 @@ class Simulation:
 
     def __init__(self, case: dict, *, batch=0, storage=_lib.STORE_AUTO, ckpt_interval=0, mem_budget=0,
                  form=_lib.FORM_FUSED, grad_sign=-1, tile_tz=0, device=0, check_finite=1, kernel_variant=0,
-                 work_order=0, use_graph=0):
+                 work_order=0, use_graph=0, param=_lib.PARAM_LAME, vel_bounds=None):
         lib = _lib.load()
         self.lib = lib
         self.case = case
@@ -186,6 +186,9 @@ class Simulation:
         o.check_finite = int(check_finite)
         o.kernel_variant, o.work_order = int(kernel_variant), int(work_order)
         o.use_graph = int(use_graph)
+        o.param = int(param)
+        if vel_bounds is not None:
+            o.vel_bounds[:] = [float(v) for v in vel_bounds]
         h = C.c_void_p()
         check(lib.fwi_create(C.byref(g), C.byref(q), C.byref(o), C.byref(h)), None)
         self.ctx = h

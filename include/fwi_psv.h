@@ -52,7 +52,7 @@
 extern "C" {
 #endif
 
-#define FWI_PSV_ABI_VERSION 1
+#define FWI_PSV_ABI_VERSION 2
 
 enum fwi_status {
     FWI_OK = 0,
@@ -71,6 +71,14 @@ enum fwi_storage {
     FWI_STORE_AUTO = 0,        /* full snapshots if they fit the budget, else checkpoints */
     FWI_STORE_FULL = 1,        /* every snap_dt-th forward state kept (the reference's scheme) */
     FWI_STORE_CHECKPOINT = 2   /* checkpoints + forward recomputation per segment */
+};
+
+/* Model parameters that fwi_update_model and the step-length trials move. */
+enum fwi_param {
+    FWI_PARAM_LAME = 0,      /* lam, mu, rho, each by update_mat2 (src/cpu/n_simulate_PSV.cpp:542-546) */
+    FWI_PARAM_VELOCITY = 1   /* cp and cs by update_mat2 with rho held fixed; the global lam / mu
+                                gradient is carried to them by the chain rule through
+                                lam = rho (cp^2 - 2 cs^2), mu = rho cs^2 at the saved model */
 };
 
 /* Kernel structure of a time step. */
@@ -145,6 +153,10 @@ typedef struct {
                                1 = shot fastest then z then x, 2 = shot fastest then x then z,
                                3 = x tiles fastest, shots outermost,
                                4 = x tiles fastest, then shots, z segments outermost */
+    int32_t param;          /* enum fwi_param */
+    double vel_bounds[4];   /* FWI_PARAM_VELOCITY: cp_min, cp_max, cs_min, cs_max in m/s, the
+                               clip bounds and update scales of update_mat2 (all 0: 1000, 6000,
+                               500, 3500) */
 } fwi_opts;
 
 /* Per-kernel timing collected while profiling is on (HIP events on the
@@ -237,7 +249,9 @@ int32_t fwi_taper_gradient(fwi_ctx *ctx);
 /* Sum of the shots' L2 misfits for the current model (forward only). */
 int32_t fwi_misfit(fwi_ctx *ctx, int32_t shot0, int32_t nshots, double *L2);
 int32_t fwi_save_model(fwi_ctx *ctx);
-/* m = clip(m_old + eps f G): lam, mu with eps, rho with rho_factor * eps. */
+/* m = clip(m_old + eps f G): lam, mu with eps, rho with rho_factor * eps.  With
+ * FWI_PARAM_VELOCITY: cp, cs with eps (f from the cp / cs gradient and vel_bounds),
+ * rho = rho_old; rho_factor and rho_min are not used. */
 int32_t fwi_update_model(fwi_ctx *ctx, double eps, double rho_factor, double rho_min);
 int32_t fwi_step_length(fwi_ctx *ctx, double est, double L2_0, double *step_out, int32_t *ntrials);
 /* One simulate_fwi_PSV iteration over all shots; returns its misfit and step. */

@@ -1,6 +1,6 @@
 """Checkpoint-budget sweep of one gradient, through the C ABI.
 
-    python -m fwi_cuda_amd.ckpt_sweep --config C4 --budgets 240,120,60,30
+    python measure/ckpt_sweep.py --config C4 --budgets 240,120,60,30
 
 For each device-memory budget (GB) the planner picks the batch of shots and
 the checkpoint interval that fit; the same gradient (observed data modelled
@@ -15,10 +15,12 @@ import argparse
 import json
 import sys
 import time
+from pathlib import Path
 
 import numpy as np
 
-from . import _lib, configs, psv
+sys.path.insert(0, str(Path(__file__).resolve().parents[1]))
+from fwi_cuda_amd import _lib, configs, psv
 
 
 def main(argv=None):

@@ -94,22 +94,22 @@ int main(int argc, char **argv) {
     }
     const int grid = (a.P / (64 * kWaves)) * ((a.nz + kTZ - 1) / kTZ) * a.B;
     hipEvent_t e0, e1;
-    hipEventCreate(&e0);
-    hipEventCreate(&e1);
+    (void)hipEventCreate(&e0);
+    (void)hipEventCreate(&e1);
     hipLaunchKernelGGL(k_probe, dim3(grid), dim3(64 * kWaves), 0, 0, a);   // warm-up
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return fail("launch", e);
     const int reps = 10;
-    hipEventRecord(e0);
+    (void)hipEventRecord(e0);
     for (int i = 0; i < reps; ++i) hipLaunchKernelGGL(k_probe, dim3(grid), dim3(64 * kWaves), 0, 0, a);
-    hipEventRecord(e1);
+    (void)hipEventRecord(e1);
     e = hipEventSynchronize(e1);
     if (e != hipSuccess) return fail("kernel", e);
     float ms = 0.f;
-    hipEventElapsedTime(&ms, e0, e1);
+    (void)hipEventElapsedTime(&ms, e0, e1);
     const double moved = (double)bytes * (kIn + kOut);
     std::printf("{\"shape_GBps\": %.1f, \"bytes_per_launch\": %.0f, \"avg_launch_ms\": %.4f}\n",
                 moved / (ms / reps / 1e3) / 1e9, moved, ms / reps);
-    for (void *p : bufs) hipFree(p);
+    for (void *p : bufs) (void)hipFree(p);
     return 0;
 }

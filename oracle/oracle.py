@@ -413,11 +413,44 @@ def misfit_all(c, lam, mu, rho, obs, which="oracle"):
     return L2
 
 
-def step_length(c, est, L2_0, lam_old, mu_old, rho_old, G, obs, which="oracle", log=None):
+VEL_BOUNDS = (1000.0, 6000.0, 500.0, 3500.0)   # cp_min, cp_max, cs_min, cs_max (m/s)
+
+
+def velocity_of(lam, mu, rho):
+    """cp = sqrt((lam + 2 mu) / rho), cs = sqrt(mu / rho)."""
+    return np.sqrt((lam + 2.0 * mu) / rho), np.sqrt(mu / rho)
+
+
+def velocity_gradient(G_lam, G_mu, rho, cp, cs):
+    """The lam / mu gradient carried to cp / cs at fixed rho by the chain rule through
+    lam = rho (cp^2 - 2 cs^2), mu = rho cs^2: dJ/dcp = 2 rho cp dJ/dlam,
+    dJ/dcs = 2 rho cs (dJ/dmu - 2 dJ/dlam)."""
+    return 2.0 * rho * cp * G_lam, 2.0 * rho * cs * (G_mu - 2.0 * G_lam)
+
+
+def update_model(old, G, step, *, param="lame", rho_factor=0.5, rho_min=1.25, bounds=VEL_BOUNDS):
+    """The model update of simulate_fwi_PSV (src/cpu/n_simulate_PSV.cpp:542-546):
+    update_mat2 on lam, mu (step) and rho (rho_factor * step, floor rho_min); or, for
+    param "velocity", update_mat2 on cp and cs with the chain-ruled gradient and the
+    bounds (cp_min, cp_max, cs_min, cs_max), rho kept, lam and mu from cp, cs, rho."""
+    lam_old, mu_old, rho_old = old
+    if param == "lame":
+        return (update_mat2(lam_old, G[0], LAM_MAX, 0.0, step), update_mat2(mu_old, G[1], MU_MAX, 0.0, step),
+                update_mat2(rho_old, G[2], RHO_MAX, rho_min, rho_factor * step))
+    cp_old, cs_old = velocity_of(lam_old, mu_old, rho_old)
+    g_cp, g_cs = velocity_gradient(G[0], G[1], rho_old, cp_old, cs_old)
+    cp = update_mat2(cp_old, g_cp, bounds[1], bounds[0], step)
+    cs = update_mat2(cs_old, g_cs, bounds[3], bounds[2], step)
+    mu = rho_old * cs * cs
+    return rho_old * cp * cp - 2.0 * mu, mu, rho_old.copy()
+
+
+def step_length(c, est, L2_0, lam_old, mu_old, rho_old, G, obs, which="oracle", log=None, param="lame",
+                bounds=VEL_BOUNDS):
     """step_length_PSV with update_param 0 (src/cpu/n_step_length_PSV.cpp:13-209):
     bracketing with scalefac 2, at most 30 halvings and 4 enlargements, then the
-    parabola vertex.  Returns (step, trials) or raises RuntimeError where the
-    reference calls exit(0)."""
+    parabola vertex.  Trial models as update_model (rho at 0.5 step, floor 1.5).
+    Returns (step, trials) or raises RuntimeError where the reference calls exit(0)."""
     scalefac, stepmax_1, stepmax_2 = 2.0, 30, 4
     L2t = [L2_0, 0.0, 0.0]
     sl = [0.0, 0.0, 0.0]
@@ -427,9 +460,8 @@ def step_length(c, est, L2_0, lam_old, mu_old, rho_old, G, obs, which="oracle", 
     trials = []
     while step2 != 1 or step1 != 1:
         for itest in range(itests, iteste + 1):
-            lam = update_mat2(lam_old, G[0], LAM_MAX, 0.0, est)
-            mu = update_mat2(mu_old, G[1], MU_MAX, 0.0, est)
-            rho = update_mat2(rho_old, G[2], RHO_MAX, 1.5, 0.5 * est)
+            lam, mu, rho = update_model((lam_old, mu_old, rho_old), G, est, param=param, rho_factor=0.5,
+                                        rho_min=1.5, bounds=bounds)
             L2tmp = misfit_all(c, lam, mu, rho, obs, which=which)
             L2t[itest] = L2tmp
             sl[itest] = est
@@ -458,6 +490,31 @@ def step_length(c, est, L2_0, lam_old, mu_old, rho_old, G, obs, which="oracle", 
     if step1 == 1:
         est = calc_opt_step(L2t, sl)
     return est, trials, (list(L2t), list(sl))
+
+
+def fwi_iteration(c, model, G, obs, est, *, param="lame", grad_sign=-1, bounds=VEL_BOUNDS, which="oracle"):
+    """One iteration of simulate_fwi_PSV (src/cpu/n_simulate_PSV.cpp:249-582) composed from
+    the pieces above: every shot's preconditioned gradient added into the global gradient
+    G (modified in place and never reset, as in the reference), taper2 on the accumulated
+    G, the step-length search from est, and the update at the chosen step (rho at half the
+    step, floor 1.25).  grad_sign -1 is the CPU path's "grad -= ..." (its shot gradients
+    as shot_gradient returns them), +1 the GPU twin's "+=" (the same sums negated).
+    Returns (L2, step, new model, trials)."""
+    lam_old, mu_old, rho_old = (np.asarray(m, np.float64).copy() for m in model)
+    media = mat_av2(c, lam_old, mu_old, rho_old, which=which)
+    L2 = 0.0
+    for s in range(int(c["nshot"])):
+        sg = shot_gradient(c, media, s, *obs[s], which=which)
+        L2 += sg["L2"]
+        for i, k in enumerate(("g_lam", "g_mu", "g_rho")):
+            G[i] += -grad_sign * sg[k]
+    for i in range(3):
+        G[i][...] = taper2(c, G[i], which=which)
+    step, trials, _ = step_length(c, est, L2, lam_old, mu_old, rho_old, G, obs, which=which, param=param,
+                                  bounds=bounds)
+    new = update_model((lam_old, mu_old, rho_old), G, step, param=param, rho_factor=0.5, rho_min=1.25,
+                       bounds=bounds)
+    return L2, step, new, trials
 
 
 def ref_step_length(c, est, L2_0, lam_old, mu_old, rho_old, G, obs):

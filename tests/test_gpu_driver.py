@@ -115,8 +115,8 @@ def test_stream_probe_reports_its_bandwidth():
     """bench.py's access-shape probe runs, checks its arguments and prints one JSON
     line whose byte count is the 15 arrays it streams."""
     import json
-    probe = ROOT / "fwi_cuda_amd" / "stream_probe"
-    assert probe.exists(), "build the library: python -m fwi_cuda_amd.build"
+    probe = ROOT / "measure" / "stream_probe"
+    assert probe.exists(), "build it: make -C measure"
     r = subprocess.run([str(probe), "201", "512", "4"], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stderr
     d = json.loads(r.stdout.strip().splitlines()[-1])

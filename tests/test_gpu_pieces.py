@@ -1,0 +1,207 @@
+"""HIP path for the pieces of the gradient that the main parity tests leave at their
+identity settings: taper2 on the device, the strided snapshot lattice with its
+interpol_grad2, and free surfaces on the bottom, left and right sides.
+
+Tolerances:
+  taper2 (FP64 on the device)                 rel 1e-12 against the reference's own output
+  seismograms / final state, order 8          rel-L2 <= 1e-4 against the FP64 oracle
+  raw and preconditioned gradients            rel-L2 <= 2e-3 against the FP64 oracle (the
+                                              residual is about 1e-3 of the data here)
+"""
+import numpy as np
+import pytest
+
+import cases
+from conftest import gpu_available, load_fixture
+
+pytestmark = [pytest.mark.gpu, pytest.mark.skipif(not gpu_available(), reason="needs a GPU")]
+
+
+@pytest.fixture(scope="module")
+def psv():
+    from fwi_cuda_amd import psv as P
+    return P
+
+
+@pytest.fixture(scope="module")
+def O():
+    from oracle import oracle as O_
+    O_.load("oracle")
+    return O_
+
+
+def _print(*a):
+    print(*a, flush=True)
+
+
+def test_taper_matches_reference_output(psv, O):
+    """taper2 of the reference (src/cpu/n_kernel_lib_PSV.cpp:942-1010), run by the
+    compiled reference on a 30 x 41 array with an uneven snap box and ramp widths
+    (tests/golden/pieces.npz), against fwi_taper_gradient on the three gradient
+    arrays; and against the oracle on a second, seeded array and box."""
+    p = load_fixture("pieces")["taper"]
+    A, snap, taper, want = p["A"], p["snap"], p["taper"], p["out"]
+    nz, nx = A.shape
+    c = cases.small_case(O, nz=nz, nx=nx, nt=4, fdorder=2, nshot=1, npml=4, surface=False)
+    c["snap"] = np.array([snap[0], snap[1], snap[2], snap[3], 1, 1, 1], np.int32)   # taper2 ignores strides
+    c["taper"] = np.asarray(taper, np.int32)
+    with psv.Simulation(c) as sim:
+        sim.set_gradient(A, 2.0 * A, -A)
+        sim.taper2()
+        G = sim.get_gradient()
+    np.testing.assert_allclose(G[0], want, rtol=1e-12, atol=1e-300)
+    np.testing.assert_allclose(G[1], 2.0 * want, rtol=1e-12, atol=1e-300)
+    np.testing.assert_allclose(G[2], -want, rtol=1e-12, atol=1e-300)
+    assert np.count_nonzero(want == 0.0) > 0 and np.count_nonzero((want != A) & (want != 0.0)) > 0
+
+    # a second box with ramps on every side, on the oracle
+    c = cases.small_case(O, nz=64, nx=90, nt=4, fdorder=8, nshot=1, npml=10, surface=True)
+    c["taper"] = np.array([6, 14, 50, 41, 13, 21, 77, 70], np.int32)
+    B = np.random.default_rng(11).standard_normal((64, 90))
+    want = O.taper2(c, B.copy())
+    with psv.Simulation(c) as sim:
+        sim.set_gradient(B, B, B)
+        sim.taper2()
+        G = sim.get_gradient()
+    for g in G:
+        np.testing.assert_allclose(g, want, rtol=1e-12, atol=1e-300)
+
+
+@pytest.mark.parametrize("variant", [1, 5, 7, 8])
+def test_strided_snapshot_gradient_vs_oracle(psv, O, variant):
+    """Snapshots on a lattice of every 2nd row and 3rd column (snap_dz = 2,
+    snap_dx = 3, snap_dt = 2, the strides of the golden interpol_grad2 piece): the
+    correlation is taken on the lattice only and interpol_grad2
+    (src/cpu/n_kernel_lib_PSV.cpp:628-693) fills the box between lattice points on
+    the device.  Raw sums on the lattice and the preconditioned gradient against
+    the FP64 oracle (its own or_interpol_grad2), two shots in one batch."""
+    c = cases.small_case(O, nshot=2, nt=450)
+    z1, z2, x1, x2 = (int(v) for v in c["snap"][:4])
+    assert (z2 - z1) % 2 == 0 and (x2 - x1) % 3 == 0
+    c["snap"] = np.array([z1, z2, x1, x2, 2, 2, 3], np.int32)
+    med_true = O.mat_av2(c)
+    obs = cases.per_shot_obs(c, O, med_true)
+    lam0, mu0, rho0 = cases.start_model(c)
+    med0 = O.mat_av2(c, lam0, mu0, rho0)
+    G = [np.zeros((int(c["nz"]), int(c["nx"]))) for _ in range(3)]
+    L2 = 0.0
+    for s in range(2):
+        sg = O.shot_gradient(c, med0, s, *obs[s])
+        L2 += sg["L2"]
+        for i, k in enumerate(("g_lam", "g_mu", "g_rho")):
+            G[i] += sg[k]
+    # raw lattice sums of shot 1 for given adjoint sources (the residual of the start model)
+    fwd = O.kernel_psv(c, med0, 1, grad=False, accu=True)
+    rz, rx = fwd["uz"] - obs[1][0], fwd["ux"] - obs[1][1]
+    adj = O.kernel_psv(c, med0, 1, grad=True, accu=False, snaps=fwd["snaps"], adj=(rz, rx))
+    raw = [adj["g_lam"], adj["g_mu"], adj["g_rho"]]
+    tz = 40 if variant in (7, 8) else 0
+    with psv.Simulation(c, kernel_variant=variant, tile_tz=tz) as sim:
+        sim.set_model(lam0, mu0, rho0)
+        sim.set_observed([o[0] for o in obs], [o[1] for o in obs])
+        m = sim.gradient()
+        Gg = sim.get_gradient()
+        rg = sim.gradient_raw(rz, rx, 1, 1)
+    assert m.sum() == pytest.approx(L2, rel=1e-4)
+    assert rg[0].shape == raw[0].shape == (1 + (z2 - z1) // 2, 1 + (x2 - x1) // 3)
+    for i, k in enumerate(("lam", "mu", "rho")):
+        e = cases.rel_l2(Gg[i], G[i])
+        er = cases.rel_l2(rg[i], raw[i])
+        _print(f"variant {variant} strided lattice {k}: preconditioned rel-L2 {e:.2e}, raw lattice sums {er:.2e}")
+        assert e < 2e-3 and er < 2e-3
+        # between lattice points the gradient is the linear interpolation of its neighbours
+        zz = z1 + 1
+        np.testing.assert_allclose(Gg[i][zz, x1 + 3:x2 - 3], G[i][zz, x1 + 3:x2 - 3],
+                                   atol=2e-3 * np.abs(G[i]).max())
+
+
+def _surface_case(O, isurf, npml, nt=360, nshot=2):
+    """A 72 x 96 order-8 case with free surfaces at the given rows / columns and
+    CPML on the sides given by npml (top, bottom, left, right); sources and
+    receivers inside the region the surfaces enclose."""
+    nz, nx, N = 72, 96, 4
+    lam, mu, rho = cases.layered_model(nz, nx, 5)
+    hc = O.holberg(8)
+    dt = 0.5 * O.dtstab(lam, mu, rho, hc, 1.0, max(npml))
+    top = isurf[0] if isurf[0] else max(npml[0], N)
+    bot = isurf[1] if isurf[1] else nz - max(npml[1], N) - 1
+    left = isurf[2] if isurf[2] else max(npml[2], N)
+    right = isurf[3] if isurf[3] else nx - max(npml[3], N) - 1
+    xs = np.linspace(left + 12, right - 12, nshot).astype(np.int32)
+    zs = np.full(nshot, top + 6, np.int32)
+    zr = [top + 2] * 6 + [bot - 2] * 4
+    xr = list(np.linspace(left + 3, right - 3, 6).astype(int)) + list(np.linspace(left + 5, right - 5, 4).astype(int))
+    zr += list(np.linspace(top + 4, bot - 4, 4).astype(int)) * 2
+    xr += [left + 2] * 4 + [right - 2] * 4
+    snap = (top, bot, left, right, 2, 1, 1)
+    return O.make_case(nt=nt, nz=nz, nx=nx, dt=dt, dz=1.0, dx=1.0, fdorder=8, lam=lam, mu=mu, rho=rho,
+                       z_src=zs, x_src=xs, shot_to_fire=np.arange(nshot), fc=300.0, amp_z=1.0, amp_x=0.5,
+                       npml=npml, isurf=isurf, z_rec=zr, x_rec=xr, snap=snap,
+                       taper=(top + 2, top + 6, bot - 2, bot - 6, left + 2, left + 6, right - 2, right - 6))
+
+
+SURFACES = {
+    # bottom, left and right surfaces with their image rows / columns, top CPML
+    "bottom-left-right": ((0, 64, 7, 88), (10, 0, 0, 0)),
+    # all four sides, no CPML (rigid edges beyond the image rows)
+    "all-four": ((6, 65, 5, 90), (0, 0, 0, 0)),
+    # a bottom surface at the last interior row (no image rows inside the interior)
+    "bottom-edge": ((4, 67, 0, 0), (0, 0, 10, 10)),
+}
+
+
+@pytest.mark.parametrize("variant", [1, 2, 5, 6])
+@pytest.mark.parametrize("name", sorted(SURFACES))
+def test_free_surfaces_on_every_side_vs_oracle(psv, O, name, variant):
+    """surf_mirror on the bottom, left and right sides (src/cpu/n_kernel_lib_PSV.cpp:317-394):
+    the tangential stress set (not incremented) on the surface line, the normal and
+    shear stresses zeroed, and the negated images between the interior edge and the
+    surface; seismograms of receivers near every surface and the final stresses
+    against the FP64 oracle.  The one-pass variants refuse these grids."""
+    isurf, npml = SURFACES[name]
+    c = _surface_case(O, isurf, npml)
+    med = O.mat_av2(c)
+    ref = [O.kernel_psv(c, med, s, accu=False) for s in range(2)]
+    with psv.Simulation(c, kernel_variant=variant) as sim:
+        uz, ux = sim.forward()
+        for s, (a, b) in enumerate(zip(sim.split(uz), sim.split(ux))):
+            ez, ex = cases.rel_l2(a, ref[s]["uz"]), cases.rel_l2(b, ref[s]["ux"])
+            _print(f"surfaces {name} variant {variant} shot {s}: rel-L2 uz {ez:.2e} ux {ex:.2e}")
+            assert ez < 1e-4 and ex < 1e-4
+        for k in ("szz", "szx", "sxx", "vz", "vx"):
+            e = cases.rel_l2(sim.wavefield(1, k), ref[1]["state"][k])
+            assert e < 1e-4, (k, e)
+        # the surface line carries the assigned stress and zeros
+        st = {k: sim.wavefield(1, k) for k in ("szz", "szx", "sxx")}
+    if isurf[1]:   # (the left and right sides, processed after it, set szz where they cross it)
+        assert np.all(st["szx"][isurf[1], 4:-4] == 0)
+    if isurf[2]:
+        assert np.all(st["sxx"][4:-4, isurf[2]] == 0) and np.all(st["szx"][4:-4, isurf[2]] == 0)
+
+
+@pytest.mark.parametrize("variant", [1, 6])
+def test_free_surfaces_gradient_vs_oracle(psv, O, variant):
+    """A gradient with surfaces on the bottom, left and right (the adjoint pass
+    applies them too): misfit and preconditioned gradient against the FP64 oracle."""
+    isurf, npml = SURFACES["bottom-left-right"]
+    c = _surface_case(O, isurf, npml, nt=420)
+    obs = cases.per_shot_obs(c, O, O.mat_av2(c))
+    lam0, mu0, rho0 = cases.layered_model(int(c["nz"]), int(c["nx"]), 5, inclusion=False)
+    med0 = O.mat_av2(c, lam0, mu0, rho0)
+    G = [np.zeros((int(c["nz"]), int(c["nx"]))) for _ in range(3)]
+    L2 = 0.0
+    for s in range(2):
+        sg = O.shot_gradient(c, med0, s, *obs[s])
+        L2 += sg["L2"]
+        for i, k in enumerate(("g_lam", "g_mu", "g_rho")):
+            G[i] += sg[k]
+    with psv.Simulation(c, kernel_variant=variant) as sim:
+        sim.set_model(lam0, mu0, rho0)
+        sim.set_observed([o[0] for o in obs], [o[1] for o in obs])
+        m = sim.gradient()
+        Gg = sim.get_gradient()
+    assert m.sum() == pytest.approx(L2, rel=1e-4)
+    for i, k in enumerate(("lam", "mu", "rho")):
+        e = cases.rel_l2(Gg[i], G[i])
+        _print(f"surfaces bottom-left-right variant {variant}: preconditioned {k} rel-L2 {e:.2e}")
+        assert e < 2e-3

@@ -48,8 +48,8 @@ def test_struct_layouts_match_the_header(tmp_path):
         "int main(void) {\n"
         '  printf("%zu %zu %zu %zu %d\\n", sizeof(fwi_grid), sizeof(fwi_acq), sizeof(fwi_opts),\n'
         "         sizeof(fwi_kernel_time), (int)FWI_K_COUNT);\n"
-        '  printf("%zu %zu %zu\\n", offsetof(fwi_grid, taper), offsetof(fwi_opts, work_order),\n'
-        "         offsetof(fwi_acq, rec_x));\n"
+        '  printf("%zu %zu %zu %zu %zu\\n", offsetof(fwi_grid, taper), offsetof(fwi_opts, work_order),\n'
+        "         offsetof(fwi_acq, rec_x), offsetof(fwi_opts, param), offsetof(fwi_opts, vel_bounds));\n"
         "  return 0;\n}\n")
     exe = tmp_path / "layout"
     subprocess.run(["gcc", "-I", str(ROOT / "include"), str(src), "-o", str(exe)], check=True)
@@ -59,7 +59,8 @@ def test_struct_layouts_match_the_header(tmp_path):
     assert sizes[:4] == [C.sizeof(_lib.FwiGrid), C.sizeof(_lib.FwiAcq), C.sizeof(_lib.FwiOpts),
                          C.sizeof(_lib.FwiKernelTime)]
     assert sizes[4] == len(_lib.KERNELS)
-    assert offs == [_lib.FwiGrid.taper.offset, _lib.FwiOpts.work_order.offset, _lib.FwiAcq.rec_x.offset]
+    assert offs == [_lib.FwiGrid.taper.offset, _lib.FwiOpts.work_order.offset, _lib.FwiAcq.rec_x.offset,
+                    _lib.FwiOpts.param.offset, _lib.FwiOpts.vel_bounds.offset]
 
 
 def test_default_options(lib):
@@ -68,7 +69,8 @@ def test_default_options(lib):
     lib.fwi_default_opts(C.byref(o))
     assert (o.storage, o.form, o.grad_sign, o.check_finite, o.kernel_variant, o.work_order) == \
         (_lib.STORE_AUTO, _lib.FORM_FUSED, -1, 1, 0, 0)
-    assert lib.fwi_abi_version() == 1
+    assert lib.fwi_abi_version() == _lib.ABI_VERSION == 2
+    assert o.param == _lib.PARAM_LAME and list(o.vel_bounds) == [0.0] * 4
     assert lib.fwi_strerror(-8)
 
 
@@ -195,10 +197,11 @@ def test_velocity_parameter_chain_rule():
         np.testing.assert_allclose(g, fd, rtol=1e-4, atol=1e-12 * np.abs(fd).max())
 
 
-def test_bench_names_the_tuned_instance_and_finds_its_traffic():
+def test_bench_names_the_tuned_instance_and_finds_its_traffic(tmp_path):
     """bench.py's roofline names the kernel instance the tuner chose (as the profiler
-    names it) and picks that instance's counter traffic out of the committed summary:
-    the one-pass variants 7 and 8 differ only in the block arrangement flag."""
+    names it) and takes that instance's counter traffic out of a summary only when the
+    summary's line was measured at the same algorithmic bytes per launch (same grid,
+    batch and launch mix): a line of the same instance at another batch is refused."""
     import sys
     sys.path.insert(0, str(ROOT))
     import bench
@@ -208,13 +211,34 @@ def test_bench_names_the_tuned_instance_and_finds_its_traffic():
     assert bench.kernel_instantiation("step", "chosen o0/v4/tz32/d1; ", 4) == "k_step<4, 32, "
     assert bench.kernel_instantiation("velocity", "chosen o1/v6/tz20/d1; ", 4) == "k_velocity6<4, 1, 2, "
     assert bench.kernel_instantiation("stress", "chosen o2/v2/tz16/d1; ", 4) == "k_stress2<4, 16, 1>"
-    t7 = bench.measured_traffic("k_step7<4, 4, *, false>")
-    t8 = bench.measured_traffic("k_step7<4, 4, *, true>")
-    assert t7 and t8 and t8 < t7
-    assert bench.measured_traffic("k_step7<4, 8, *, true>") is None
-    for log in ("chosen o2/v5/tz16/d1; ", "chosen o2/v6/tz20/d1; "):
-        inst = bench.kernel_instantiation("velocity", log, 4)
-        assert bench.measured_traffic(inst) > 4.5e9, inst
+    f = tmp_path / "traffic.txt"
+    f.write_text("header\n"
+                 "traffic k_step7<4, 4, *, false> gradient-mix bytes_per_launch=9.0e9 "
+                 "algorithmic_bytes_per_launch=5.9e9 launches=10\n"
+                 "traffic k_step7<4, 4, *, true> gradient-mix bytes_per_launch=8.0e9 "
+                 "algorithmic_bytes_per_launch=5.9e9 launches=10\n"
+                 "traffic k_step7<4, 4, *, true> gradient-mix bytes_per_launch=2.1e9 "
+                 "algorithmic_bytes_per_launch=1.48e9 launches=10\n")
+    assert bench.measured_traffic("k_step7<4, 4, *, false>", 5.9e9, f) == 9.0e9
+    assert bench.measured_traffic("k_step7<4, 4, *, true>", 5.91e9, f) == 8.0e9
+    # the same instance at a quarter of the batch picks its own line, and none at another size
+    assert bench.measured_traffic("k_step7<4, 4, *, true>", 1.476e9, f) == 2.1e9
+    assert bench.measured_traffic("k_step7<4, 4, *, true>", 2.95e9, f) is None
+    assert bench.measured_traffic("k_step7<4, 8, *, true>", 5.9e9, f) is None
+    # the committed summary parses and every line carries both byte counts
+    lines = [l for l in (ROOT / "profiles" / "roofline_traffic.txt").read_text().splitlines()
+             if l.startswith("traffic ")]
+    assert lines and all("bytes_per_launch=" in l and "algorithmic_bytes_per_launch=" in l for l in lines)
+
+
+def test_bench_shot_groups_cover_the_workload():
+    """A bench step is the gradient of one shot group; the groups tile the shots."""
+    import sys
+    sys.path.insert(0, str(ROOT))
+    import bench
+    assert bench.shot_groups(64, 8) == [(8 * k, 8) for k in range(8)]
+    g = bench.shot_groups(10, 4)
+    assert g == [(0, 4), (4, 4), (8, 2)] and sum(n for _, n in g) == 10
 
 
 def test_reference_input_files_round_trip_the_fixture(tmp_path):

@@ -394,7 +394,8 @@ def main():
     tz, tune_log = sim.tuning()
     K = args.steps
     log(f"timed {K} step(s), {shots_timed} shot gradients: {wall:.2f} s ({wall / K * 1e3:.0f} ms per step); "
-        f"tile rows {tz} ({tune_log[:200]})")
+        f"tile rows {tz}")
+    log("tuning:", tune_log)
     log("kernel times", {k: (n, round(ms, 1)) for k, (n, ms) in kt.items()})
 
     cells = nz * nx

@@ -1,0 +1,207 @@
+"""The benchmark configurations themselves against the FP64 oracle (the CPU path restated).
+
+  C1  full size (201 x 401, one shot, 40 receivers, nt 2000, snap_dt 1): forward
+      seismograms, the misfit and preconditioned gradient, and one FWI iteration
+      (gradient, taper, step-length search, update) with the GPU twin's sign
+  C2  one shot of the C2 grid (301 x 601, 120 receivers, nt 3000, snapshots every 4th
+      step in both paths): misfit, raw correlation sums and preconditioned gradient
+  C3  one shot of the C3 grid (1001 x 2001) over 1400 steps, past the 100 Hz wavelet's
+      peak at step ~524 (snapshots every 10th step in both paths), and the full 4000-step
+      forward seismograms
+  C5  the C5 layout reduced to 121 x 241 (4 shots, 24 receivers, nt 1500): three
+      iterations of steepest descent on cp and cs (rho fixed) with the reference's
+      taper and parabolic step, misfit, step and model per iteration
+
+Tolerances (FP32 wavefields, FP64 receiver sums, misfits and preconditioning):
+  seismograms rel-L2 <= 1e-4; misfit rel 1e-4; gradients rel-L2 <= 1e-4 where the
+  residual is a large part of the data (C1, C3), <= 1e-3 otherwise; steps rel 1e-2;
+  model updates (m_k - m_0) rel-L2 <= 1e-2 and models rel-L2 <= 1e-4.
+"""
+import numpy as np
+import pytest
+
+import cases
+from conftest import gpu_available
+
+pytestmark = [pytest.mark.gpu, pytest.mark.skipif(not gpu_available(), reason="needs a GPU")]
+
+
+@pytest.fixture(scope="module")
+def psv():
+    from fwi_cuda_amd import psv as P
+    return P
+
+
+@pytest.fixture(scope="module")
+def O():
+    from oracle import oracle as O_
+    O_.load("oracle")
+    return O_
+
+
+def _print(*a):
+    print(*a, flush=True)
+
+
+def _case(builder, **kw):
+    c = dict(builder(**kw))
+    c.pop("seed", None)
+    return c
+
+
+def _oracle_gradient(O, c, model, obs, sign=-1):
+    """Preconditioned global gradient over every shot and the summed misfit (oracle)."""
+    med = O.mat_av2(c, *model)
+    nz, nx = int(c["nz"]), int(c["nx"])
+    G = [np.zeros((nz, nx)) for _ in range(3)]
+    L2 = 0.0
+    raws = []
+    for s in range(int(c["nshot"])):
+        cs = cases.shot_case(c, s)
+        sg = O.shot_gradient(cs, med, s, *obs[s])
+        L2 += sg["L2"]
+        raws.append(sg["raw"])
+        for i, k in enumerate(("g_lam", "g_mu", "g_rho")):
+            G[i] += -sign * sg[k]
+    return G, L2, raws
+
+
+def test_c1_full_size_vs_oracle(psv, O):
+    from fwi_cuda_amd import configs
+    c = _case(configs.c1)
+    start = _case(configs.c1, true_model=False)
+    m0 = (start["lam"], start["mu"], start["rho"])
+    obs = cases.per_shot_obs(c, O, O.mat_av2(c))
+    G, L2, _ = _oracle_gradient(O, c, m0, obs)
+    Go = [g.copy() for g in G]
+    for i in range(3):
+        Go[i] = -O.taper2(c, Go[i])   # the GPU twin's sign, tapered: the iteration's direction
+    Gz = [np.zeros_like(g) for g in G]
+    L2_it, step_o, m1_o, trials = O.fwi_iteration(c, m0, Gz, obs, 0.01, grad_sign=+1)
+    for i in range(3):
+        np.testing.assert_allclose(Gz[i], Go[i], rtol=1e-12, atol=1e-12 * np.abs(Go[i]).max())
+    with psv.Simulation(c) as sim:
+        uz, ux = sim.forward()
+        ez, ex = cases.rel_l2(uz, obs[0][0]), cases.rel_l2(ux, obs[0][1])
+        sim.set_model(*m0)
+        sim.set_observed(*obs[0])
+        m = sim.gradient()
+        Gg = sim.get_gradient()
+    _print(f"C1: seismograms rel-L2 uz {ez:.2e} ux {ex:.2e}; misfit {m[0]:.6e} vs {L2:.6e}")
+    assert ez < 1e-4 and ex < 1e-4
+    assert m[0] == pytest.approx(L2, rel=1e-4)
+    for i, k in enumerate(("lam", "mu", "rho")):
+        e = cases.rel_l2(Gg[i], G[i])
+        _print(f"C1: preconditioned gradient {k} rel-L2 vs FP64 oracle {e:.2e}")
+        assert e < 1e-4
+    with psv.Simulation(c, grad_sign=+1) as sim:
+        sim.set_model(*m0)
+        sim.set_observed(*obs[0])
+        L2_d, step_d = sim.fwi_iteration()
+        m1 = sim.get_model()
+    _print(f"C1 iteration: misfit {L2_d:.6e} vs {L2_it:.6e}; step {step_d:.6g} vs {step_o:.6g} "
+           f"({len(trials)} oracle trials)")
+    assert L2_d == pytest.approx(L2_it, rel=1e-4)
+    assert step_d == pytest.approx(step_o, rel=1e-2)
+    for i, k in enumerate(("lam", "mu", "rho")):
+        du = cases.rel_l2(m1[i] - m0[i], m1_o[i] - m0[i])
+        e = cases.rel_l2(m1[i], m1_o[i])
+        _print(f"C1 iteration: {k} update rel-L2 {du:.2e}, model rel-L2 {e:.2e}")
+        assert du < 1e-2 and e < 1e-4
+
+
+def test_c2_shot_gradient_vs_oracle(psv, O):
+    from fwi_cuda_amd import configs
+    c = _case(configs.c2, nshot=1)
+    c["snap"] = c["snap"].copy()
+    c["snap"][4] = 4   # every 4th step correlated, in both paths
+    c["x_src"] = np.array([300], np.int32)
+    from scipy.ndimage import gaussian_filter
+    m0 = tuple(gaussian_filter(c[k], 10.0, mode="nearest") for k in ("lam", "mu", "rho"))
+    obs = cases.per_shot_obs(c, O, O.mat_av2(c))
+    G, L2, raws = _oracle_gradient(O, c, m0, obs)
+    with psv.Simulation(c) as sim:
+        sim.set_model(*m0)
+        sim.set_observed(*obs[0])
+        m = sim.gradient()
+        Gg = sim.get_gradient()
+        fz, fx = sim.forward()
+        raw = sim.gradient_raw(fz - obs[0][0], fx - obs[0][1])
+    _print(f"C2 shot: misfit {m[0]:.6e} vs {L2:.6e}")
+    assert m[0] == pytest.approx(L2, rel=1e-4)
+    for i, k in enumerate(("lam", "mu", "rho")):
+        e = cases.rel_l2(Gg[i], G[i])
+        er = cases.rel_l2(raw[i], raws[0][i])
+        _print(f"C2 shot: preconditioned {k} rel-L2 {e:.2e}, raw sums {er:.2e}")
+        assert e < 1e-3 and er < 1e-3
+
+
+def test_c3_grid_past_the_wavelet_peak_vs_oracle(psv, O):
+    from scipy.ndimage import gaussian_filter
+    from fwi_cuda_amd import configs
+    c = _case(configs.c3, nt=1400, nshot=1)
+    c["snap"] = c["snap"].copy()
+    c["snap"][4] = 10   # every 10th step correlated, in both paths
+    m0 = tuple(gaussian_filter(c[k], 20.0, mode="nearest") for k in ("lam", "mu", "rho"))
+    obs = cases.per_shot_obs(c, O, O.mat_av2(c))
+    G, L2, _ = _oracle_gradient(O, c, m0, obs)
+    with psv.Simulation(c) as sim:
+        uz, _ = sim.forward()
+        e = cases.rel_l2(uz, obs[0][0])
+        sim.set_model(*m0)
+        sim.set_observed(*obs[0])
+        m = sim.gradient()
+        Gg = sim.get_gradient()
+    _print(f"C3 grid, 1400 steps: seismogram rel-L2 {e:.2e}, misfit {m[0]:.6e} vs {L2:.6e}")
+    assert e < 1e-4
+    assert m[0] == pytest.approx(L2, rel=1e-4)
+    for i, k in enumerate(("lam", "mu", "rho")):
+        eg = cases.rel_l2(Gg[i], G[i])
+        _print(f"C3 grid, 1400 steps: preconditioned {k} rel-L2 vs FP64 oracle {eg:.2e}")
+        assert eg < 1e-4
+    # the whole 4000-step record of one shot
+    c = _case(configs.c3, nt=4000, nshot=1)
+    ref = O.kernel_psv(cases.shot_case(c, 0), O.mat_av2(c), 0, accu=False)
+    with psv.Simulation(c) as sim:
+        uz, ux = sim.forward()
+    ez, ex = cases.rel_l2(uz, ref["uz"]), cases.rel_l2(ux, ref["ux"])
+    _print(f"C3 grid, 4000 steps: seismogram rel-L2 uz {ez:.2e} ux {ex:.2e}")
+    assert ez < 1e-4 and ex < 1e-4
+
+
+def test_c5_shaped_fwi_iterations_vs_oracle(psv, O):
+    from fwi_cuda_amd import configs
+    kw = dict(nt=1500, nshot=4, nz=121, nx=241, nrec=24)
+    c = _case(configs.c5, **kw)
+    start = _case(configs.c5, true_model=False, **kw)
+    model = (start["lam"], start["mu"], start["rho"])
+    obs = cases.per_shot_obs(c, O, O.mat_av2(c))
+    nz, nx = int(c["nz"]), int(c["nx"])
+    G = [np.zeros((nz, nx)) for _ in range(3)]
+    est, ora = 0.01, []
+    for k in range(3):
+        L2, step, model, trials = O.fwi_iteration(c, model, G, obs, est, param="velocity", grad_sign=+1)
+        ora.append((L2, step, model, len(trials)))
+        est = step
+    m0 = (start["lam"], start["mu"], start["rho"])
+    cp_true, _ = O.velocity_of(c["lam"], c["mu"], c["rho"])
+    with psv.Simulation(c, grad_sign=+1, param=psv._lib.PARAM_VELOCITY) as sim:
+        sim.set_model(*m0)
+        sim.set_observed([o[0] for o in obs], [o[1] for o in obs])
+        for k in range(3):
+            L2, step = sim.fwi_iteration()
+            m = sim.get_model()
+            L2o, stepo, mo, ntr = ora[k]
+            cp, cs = O.velocity_of(*m)
+            cpo, cso = O.velocity_of(*mo)
+            cp0, cs0 = O.velocity_of(*m0)
+            du_cp, du_cs = cases.rel_l2(cp - cp0, cpo - cp0), cases.rel_l2(cs - cs0, cso - cs0)
+            _print(f"C5-shaped iteration {k}: misfit {L2:.6e} vs {L2o:.6e}, step {step:.6g} vs {stepo:.6g} "
+                   f"({ntr} oracle trials); cp update rel-L2 {du_cp:.2e}, cs {du_cs:.2e}; cp rel-L2 vs true "
+                   f"{cases.rel_l2(cp, cp_true):.6e} (oracle {cases.rel_l2(cpo, cp_true):.6e})")
+            assert L2 == pytest.approx(L2o, rel=1e-4 if k == 0 else 1e-3)
+            assert step == pytest.approx(stepo, rel=1e-2)
+            np.testing.assert_array_equal(m[2], m0[2])   # rho is held fixed
+            assert du_cp < 1e-2 and du_cs < 1e-2
+            assert cases.rel_l2(cp, cpo) < 1e-4 and cases.rel_l2(cs, cso) < 1e-4
+    assert ora[2][0] < ora[0][0]

@@ -424,36 +424,6 @@ def test_graph_replay_equals_eager(psv, O, variant, storage):
         np.testing.assert_array_equal(out[1][3][i], out[0][3][i])
 
 
-def test_c3_grid_gradient_vs_fp64_oracle(psv, O):
-    """The benchmark grid itself (1001 x 2001, dx 0.5 m, Gaussian random field,
-    order 8, CPML 20, top free surface), one shot, 400 steps: seismograms and the
-    preconditioned gradient of the FP32 batched path against the FP64 oracle
-    (gradient rel-L2 <= 1e-4: the residual here is a large part of the data)."""
-    from scipy.ndimage import gaussian_filter
-    from fwi_cuda_amd import configs
-    c = configs.c3(nt=400, nshot=1)
-    c = dict(c)
-    c.pop("seed", None)
-    start = [gaussian_filter(c[k], 20.0, mode="nearest") for k in ("lam", "mu", "rho")]
-    obs = cases.per_shot_obs(c, O, O.mat_av2(c))
-    cs = cases.shot_case(c, 0)
-    sg = O.shot_gradient(cs, O.mat_av2(cs, *start), 0, *obs[0])
-    with psv.Simulation(c) as sim:
-        uz, ux = sim.forward()
-        e = cases.rel_l2(uz, obs[0][0])
-        sim.set_model(*start)
-        sim.set_observed(*obs[0])
-        m = sim.gradient()
-        G = sim.get_gradient()
-    _print(f"C3 grid: seismogram rel-L2 vs FP64 oracle {e:.2e}, misfit {m[0]:.6e} vs {sg['L2']:.6e}")
-    assert e < 1e-4
-    assert m[0] == pytest.approx(sg["L2"], rel=1e-4)
-    for i, k in enumerate(("g_lam", "g_mu", "g_rho")):
-        eg = cases.rel_l2(G[i], sg[k])
-        _print(f"C3 grid: preconditioned {k} rel-L2 vs FP64 oracle {eg:.2e}")
-        assert eg < 1e-4, (k, eg)
-
-
 @pytest.mark.parametrize("variant", [4, 7, 8])
 def test_one_pass_step_refused_where_it_does_not_apply(psv, O, variant):
     c = cases.small_case(O, nshot=1, nt=20)

@@ -1303,6 +1303,8 @@ thread_local std::string g_create_error;
 
 }  // namespace
 
+void psv::set_global_error(const char *msg) { g_create_error = msg ? msg : ""; }
+
 // ===========================================================================
 // C ABI
 // ===========================================================================

@@ -176,6 +176,13 @@ void plain_step(const KArgs &a, const PlainBufs &p, int N, const int isurf[4], i
 void plain_grad(const KArgs &a, hipStream_t st);
 
 // ---- small kernels on the model and gradient (psv_misc.hip) ------------------------
+// mat_av2: float lam, mu and averaged media [nz][P] from the FP64 model; the means of lam,
+// mu, rho over (nz - 1)(nx - 1) cells into means[3], summed in a fixed order from
+// mat_av_parts(nz, nx) <= kMatAvMaxParts block partial triples in part.
+constexpr int kMatAvMaxParts = 512;
+int mat_av_parts(int nz, int nx);
+void dev_mat_av_into(const double *lam, const double *mu, const double *rho, float *flam, float *fmu, float *muzx,
+                     float *rhoz, float *rhox, double *part, double *means, int nz, int nx, int P, hipStream_t st);
 void dev_mat_av(const double *lam, const double *mu, const double *rho, float *flam, float *fmu,
                 float *muzx, float *rhoz, float *rhox, double *sums3, int nz, int nx, int P,
                 hipStream_t st);
@@ -204,5 +211,8 @@ void dev_velocity_update(double *lam, double *mu, double *rho, const double *rho
                          const double bounds[4], size_t n, hipStream_t st);
 void dev_count_nonfinite(const float *x, size_t n, int *out, hipStream_t st);
 void dev_fill(float *p, float v, size_t n, hipStream_t st);
+
+// the message fwi_last_error(NULL) returns (calls without a context)
+void set_global_error(const char *msg);
 
 }  // namespace psv

@@ -287,14 +287,20 @@ inline int nblocks(size_t n, int cap = 1024) {
 
 }  // namespace
 
-void dev_mat_av(const double *lam, const double *mu, const double *rho, float *flam, float *fmu, float *muzx,
-                float *rhoz, float *rhox, double *sums3, int nz, int nx, int P, hipStream_t st) {
-    const int nb = nblocks((size_t)nz * nx, 512);
-    // partial sums live right after the three means
-    double *part = sums3 + 4;
+int mat_av_parts(int nz, int nx) { return nblocks((size_t)nz * nx, kMatAvMaxParts); }
+
+void dev_mat_av_into(const double *lam, const double *mu, const double *rho, float *flam, float *fmu, float *muzx,
+                     float *rhoz, float *rhox, double *part, double *means, int nz, int nx, int P, hipStream_t st) {
+    const int nb = mat_av_parts(nz, nx);
     hipLaunchKernelGGL(k_mat_av, dim3(nb), dim3(kRB), 0, st, lam, mu, rho, flam, fmu, muzx, rhoz, rhox, part, nz,
                        nx, P);
-    hipLaunchKernelGGL(k_sum_parts3, dim3(1), dim3(1), 0, st, part, nb, sums3, (double)((nz - 1) * (nx - 1)));
+    hipLaunchKernelGGL(k_sum_parts3, dim3(1), dim3(1), 0, st, part, nb, means, (double)((nz - 1) * (nx - 1)));
+}
+
+void dev_mat_av(const double *lam, const double *mu, const double *rho, float *flam, float *fmu, float *muzx,
+                float *rhoz, float *rhox, double *sums3, int nz, int nx, int P, hipStream_t st) {
+    // partial sums live right after the three means
+    dev_mat_av_into(lam, mu, rho, flam, fmu, muzx, rhoz, rhox, sums3 + 4, sums3, nz, nx, P, st);
 }
 
 void dev_residual(const double *seis_z, const double *seis_x, const double *obs_z, const double *obs_x,

@@ -257,6 +257,24 @@ int32_t fwi_step_length(fwi_ctx *ctx, double est, double L2_0, double *step_out,
 /* One simulate_fwi_PSV iteration over all shots; returns its misfit and step. */
 int32_t fwi_iteration(fwi_ctx *ctx, double *L2, double *step);
 
+/* ---- allocation-free entry point -------------------------------------------------- */
+/* No counterpart in the reference: for callers that cannot allocate device memory
+ * (code under HIP graph capture, serving loops, harnesses that forbid hipMalloc).
+ * mat_av2 (src/cpu/n_kernel_lib_PSV.cpp:853-922) as fwi_set_model runs it, on arrays
+ * the caller owns, all device pointers: lam, mu, rho [nz][nx] FP64 in; lam_f, mu_f
+ * (the model in FP32), mu_zx, rho_zp, rho_xp [nz][nx] FP32 and means[3] (mean lam, mu,
+ * rho over the (nz - 1)(nx - 1) cells) out; the last row and column of the averaged
+ * media are written as 0.  Asynchronous on `stream` (a hipStream_t, NULL = the null
+ * stream).  The block partial sums of the means go to a fixed 12 KiB device array of
+ * the library (one per GPU, held until the process ends) instead of a buffer the call
+ * allocates: nothing is allocated or freed.  Calls that overlap on one GPU share that
+ * array and race.  Returns FWI_EINVAL for nz or nx below 2 or more than
+ * FWI_NOALLOC_MAX_CELLS cells (message in fwi_last_error(NULL)). */
+#define FWI_NOALLOC_MAX_CELLS (1LL << 30)
+int32_t fwi_mat_av2_noalloc(const double *lam, const double *mu, const double *rho, int32_t nz, int32_t nx,
+                            float *lam_f, float *mu_f, float *mu_zx, float *rho_zp, float *rho_xp, double *means,
+                            void *stream);
+
 /* ---- profiling ------------------------------------------------------------------ */
 /* Rows per thread the stencil kernels use (the autotuner's choice once a
  * propagation has run) and the tuner's timings as text. */

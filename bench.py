@@ -111,6 +111,8 @@ def kernel_instantiation(kind, tune_log, N):
         return None
     var, tz = int(m.group(1)), int(m.group(2))
     if kind == "step":   # several launch modes share the prefix
+        if var == 9:   # balanced launches: variant 8's layout, segments of 4 blocks
+            return f"k_step7<{N}, 4, *, true>"
         if var in (7, 8):   # variant 8: the block's wavefronts side by side in x
             return f"k_step7<{N}, {max(1, tz // (2 * N + 2))}, *, {'true' if var == 8 else 'false'}>"
         return f"k_step<{N}, {tz}, "

@@ -19,6 +19,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <functional>
 #include <map>
 #include <mutex>
 #include <string>
@@ -60,6 +61,13 @@ struct fwi_ctx {
 
     // ---- plan
     int B = 1, storage = FWI_STORE_FULL, S = 0, nseg = 0;
+    // CHECKPOINT: nseg units of S steps and `slots` checkpoint buffers; with slots < nseg - 1
+    // the reversal advances from held states on a binomial schedule (ck_top / ck_rev: best
+    // split of a chain of n units with s free slots, index n * (slots + 1) + s)
+    int slots = 0;
+    std::vector<int> ck_top, ck_rev;
+    std::vector<int> spine_slot;   // [nseg]: slot the forward pass saves a unit's start state into, -1 none
+    long long extra_units = 0;     // units advanced again beyond the forward pass and the leaves
     size_t bytes = 0;
     std::vector<void *> allocs;
 
@@ -69,6 +77,12 @@ struct fwi_ctx {
     double *G[3] = {nullptr, nullptr, nullptr};
     double *tmp = nullptr, *red = nullptr;  // precondition scratch, reduction scratch
     double *vel = nullptr;   // FWI_PARAM_VELOCITY: [4][nz][nx] cp, cs of the saved model, cp / cs gradient
+    double *raw = nullptr;     // misfit derivative: FP64 sums over shots [5][snz][snx] (first use)
+    double *mg_out = nullptr;  // misfit derivative for host callers [3][nz][nx] (first use)
+    int adj_on_device = 0;     // adjoint sources of this call are device arrays
+    double *pcg = nullptr;     // conjugate-gradient state [3 params][dir, old][nz][nx] (first use)
+    double *gfilt = nullptr;   // Gaussian filter of the last smoothing (first use)
+    int gfilt_hfs = -1;
     double vb[4] = {1000.0, 6000.0, 500.0, 3500.0};   // cp_min, cp_max, cs_min, cs_max
     int *zmap = nullptr, *xmap = nullptr, *nonfinite = nullptr;
     float *pf[12] = {};   // b a 1/K  b_half a_half 1/K_half, z then x
@@ -88,6 +102,10 @@ struct fwi_ctx {
     float *psi[8] = {};    // pvz_z pvx_z pszz_z pszx_z [B][nzs][P]; pvx_x pvz_x psxx_x pszx_x [B][nz][nxs]
     float *gr[3] = {};     // [B][snz][snx]
     float *gx = nullptr;   // [2][B][snz][snx]: the extra sums of the misfit derivative (gform 1)
+    // FP64 totals of the five sums [5][B][snz][snx]: the FP32 sums above cover at most one
+    // chunk of kFlushSteps steps (or one checkpoint segment) before they are added in here and
+    // zeroed, so their rounding does not grow with nt over sums of oscillating terms
+    double *gr64 = nullptr;
     int gform = 0;         // 1 while fwi_misfit_gradient runs
     // graph replay: per-step tables of the forward, recomputation and adjoint passes [nt],
     // and the base time index of the chunk being replayed
@@ -104,7 +122,8 @@ struct fwi_ctx {
     // checkpoint mode
     float *rf[5] = {}, *rpsi[8] = {};   // recompute state
     float *rf2[5] = {}, *rpsi2[4] = {};
-    float *ck = nullptr;                // [nseg][13 arrays of the batch]
+    float *ck = nullptr;                // [slots][13 arrays of the batch]
+    float *rf_canon = nullptr;          // rf[0] as allocated: the recompute state's level order
     size_t ck_bytes_per_seg = 0;
     int last_s0 = -1, last_nb = 0;
     // plain form: full-grid derivative scratch and CPML memory (one shot)
@@ -385,12 +404,13 @@ RecArgs rec_args(fwi_ctx *c, int s0, int nb) {
 }
 
 // variants that advance a whole time step in one pass over two time levels
-bool one_pass(int var) { return var == 4 || var == 7 || var == 8; }
+bool one_pass(int var) { return var == 4 || var == 7 || var == 8 || var == 9; }
 
-// one time step of a one-pass variant (variants 7 and 8 fall back to 4 for a launch mode
-// they have no kernel for)
+// one time step of a one-pass variant (variants 7 to 9 fall back to 4 for a launch mode
+// they have no kernel for).  Variant 9 is variant 8's kernel in balanced mode: tz is the
+// number of resident-block waves the launch holds (1 or 2), the rows shared evenly.
 void launch_one_pass(fwi_ctx *c, const KArgs &a, int var, int tz, int depth) {
-    if (var >= 7 && c->ztab_ok && launch_step7(a, c->ztab, c->N, tz, var == 8, c->st)) return;
+    if (var >= 7 && c->ztab_ok && launch_step7(a, c->ztab, c->N, var == 9 ? -tz : tz, var >= 8, c->st)) return;
     launch_step(a, c->N, var >= 7 ? 32 : tz, depth, c->st);
 }
 
@@ -585,8 +605,13 @@ int autotune(fwi_ctx *c, int nb) {
     const bool order_forced = c->o.work_order >= 1 && c->o.work_order <= 4;
     // candidates (variant, rows per segment, prefetch depth); the one-pass step only where it applies
     std::vector<TuneVal> cand;
-    for (int var = 1; var <= 8; ++var) {
+    for (int var = 1; var <= 9; ++var) {
         if ((fv && var != fv) || (one_pass(var) && !c->fusable) || (var >= 5 && !c->ztab_ok)) continue;
+        if (var == 9) {   // balanced: one or two waves of resident blocks
+            for (int m : {1, 2})
+                if (c->o.tile_tz <= 0 || c->o.tile_tz == m) cand.push_back({9, m, 1, 2});
+            continue;
+        }
         if (var >= 6) {   // loads one row ahead, segments of 1, 2 (variant 6) or 4, 8 (variants 7, 8) blocks
             for (int m : var == 6 ? std::vector<int>{1, 2} : std::vector<int>{4, 8}) {
                 const int tz = v6_block(c->N, 1) * m;
@@ -665,6 +690,7 @@ int autotune(fwi_ctx *c, int nb) {
     for (int order = o_lo; order < o_hi; ++order) {
         a.order = order;
         for (const auto &vt : cand) {
+            if (vt.var == 9 && order != o_lo) continue;   // balanced launches ignore the work order
             run(vt);
             HIPCHK(hipEventRecord(e0, c->st));
             for (int rep = 0; rep < 2; ++rep) run(vt);
@@ -708,9 +734,9 @@ int run_forward(fwi_ctx *c, int s0, int nb, bool store, bool record, bool energy
     ChunkGraph cg(c);
     float *snap0[5] = {c->snap[0], c->snap[1], c->snap[2], c->snap[3], c->snap[4]};
     for (int it = 0; it < nt;) {
-        if (ckpt && it % c->S == 0) {
+        if (ckpt && it % c->S == 0 && c->spine_slot[it / c->S] >= 0) {
             Timer t(c, FWI_K_STORE);
-            RETIF(ckpt_copy(c, it / c->S, nb, w, true));
+            RETIF(ckpt_copy(c, c->spine_slot[it / c->S], nb, w, true));
         }
         if (graph && it + K <= nt) {
             KArgs ag = a;
@@ -774,6 +800,31 @@ void adj_step(fwi_ctx *c, KArgs &a, const Wave &w, const SrcArgs &sa, int it, si
     }
 }
 
+// The FP32 correlation sums go into their FP64 totals after the adjoint step of every time
+// index t with (nt - t) % kFlushSteps == 0 and once at the end: the same time steps whatever
+// the storage, so full storage, checkpoints and graph replay group the FP32 partial sums alike.
+constexpr int kFlushSteps = 64;
+static_assert(kGraphChunk == kFlushSteps, "a full-storage graph chunk ends on a flush step");
+
+bool flush_after(int nt, int it) { return (nt - it) % kFlushSteps == 0; }
+
+// the batch's FP32 correlation sums added into their FP64 totals and zeroed
+void flush_sums(fwi_ctx *c, int nb) {
+    Timer t(c, FWI_K_OTHER);
+    const size_t cells = (size_t)c->B * c->snz * c->snx;
+    float *g[5] = {c->gr[0], c->gr[1], c->gr[2], c->gx, c->gx + cells};
+    dev_flush_sums(c->gr64, g, c->gform ? 5 : 3, (size_t)nb * c->snz * c->snx, cells, c->st);
+}
+
+// the same inside a captured chunk of adjoint steps: step k of the chunk is time index
+// *si_base - k, known only at replay, so the kernel itself decides whether to flush
+void flush_sums_at(fwi_ctx *c, int nb, int k) {
+    const size_t cells = (size_t)c->B * c->snz * c->snx;
+    float *g[5] = {c->gr[0], c->gr[1], c->gr[2], c->gx, c->gx + cells};
+    dev_flush_sums_at(c->gr64, g, c->gform ? 5 : 3, (size_t)nb * c->snz * c->snx, cells, c->si_base, -1, k,
+                      c->g.nt, kFlushSteps, c->st);
+}
+
 // Adjoint pass (kernel_PSV with grad = true): residuals in res_* are injected at
 // the receivers in reversed time, each shot's scaled by its alpha (a power of two).
 int run_adjoint(fwi_ctx *c, int s0, int nb) {
@@ -784,6 +835,7 @@ int run_adjoint(fwi_ctx *c, int s0, int nb) {
     for (int i = 0; i < 3; ++i)
         HIPCHK(hipMemsetAsync(c->gr[i], 0, sizeof(float) * nb * c->snz * c->snx, c->st));
     if (c->gform) HIPCHK(hipMemsetAsync(c->gx, 0, sizeof(float) * 2 * c->B * c->snz * c->snx, c->st));
+    HIPCHK(hipMemsetAsync(c->gr64, 0, sizeof(double) * 5 * c->B * c->snz * c->snx, c->st));
     KArgs a = base_args(c, nb, w);
     a.We = c->f[8];
     a.energy = 1;
@@ -809,11 +861,14 @@ int run_adjoint(fwi_ctx *c, int s0, int nb) {
                     }
                 }));
                 jt += K;
+                flush_sums(c, nb);   // jt is a multiple of K = kFlushSteps
                 continue;
             }
             adj_step(c, a, w, sa, it, (size_t)(it / sdt) * slab);
+            if (flush_after(nt, it)) flush_sums(c, nb);
             ++jt;
         }
+        if (nt % kFlushSteps) flush_sums(c, nb);
     } else {
         const Wave rw = recompute_wave(c);
         RETIF(zero_state(c, nb, rw, false));   // cells outside the interior stay zero in both levels
@@ -823,29 +878,58 @@ int run_adjoint(fwi_ctx *c, int s0, int nb) {
         rg.si = c->si_rec;
         rg.si_base = c->si_base;
         rg.si_dir = 1;
-        ChunkGraph cgr(c);
+        ChunkGraph cgr(c), cgv(c);
         const bool graph = use_graphs(c) && c->S % 2 == 0;
         float *snap0[5] = {c->snap[0], c->snap[1], c->snap[2], c->snap[3], c->snap[4]};
-        for (int k = c->nseg - 1; k >= 0; --k) {
-            const int t0 = k * c->S, t1 = std::min(t0 + c->S, nt);
-            {
-                Timer t(c, FWI_K_STORE);
-                RETIF(ckpt_copy(c, k, nb, rw, false));
+        const int S = c->S, W = c->slots + 1;
+        // the recompute state at the start of a unit: zero at unit 0, else a slot's copy;
+        // the level order is reset first, so replayed graphs (captured in that order) and
+        // a short last segment that left it swapped agree
+        auto restore = [&](int hold) -> int {
+            if (rw.f[0] != c->rf_canon) rw.flip();
+            if (hold < 0) return zero_state(c, nb, rw, false);
+            Timer t(c, FWI_K_STORE);
+            return ckpt_copy(c, hold, nb, rw, false);
+        };
+        auto save = [&](int slot) -> int {
+            Timer t(c, FWI_K_STORE);
+            return ckpt_copy(c, slot, nb, rw, true);
+        };
+        // units [u0, u0 + k) forward from the state in rw, nothing stored (never the last unit)
+        auto advance = [&](int u0, int k) -> int {
+            for (int u = u0; u < u0 + k; ++u) {
+                const int t0 = u * S;
+                if (graph) {
+                    RETIF(cgv.run(t0, [&]() {
+                        for (int j = 0; j < S; ++j) {
+                            rg.si_k = j;
+                            time_step(c, rg, rw, fsa, t0 + j, nullptr);
+                        }
+                    }));
+                    continue;
+                }
+                for (int it = t0; it < t0 + S; ++it) time_step(c, r, rw, fsa, it, nullptr);
             }
-            if (graph && t1 - t0 == c->S) {
+            return FWI_OK;
+        };
+        // unit u recomputed from the state in rw into the segment buffer, then its adjoint steps
+        auto leaf = [&](int u) -> int {
+            const int t0 = u * S, t1 = std::min(t0 + S, nt);
+            if (graph && t1 - t0 == S) {
                 RETIF(cgr.run(t0, [&]() {
-                    for (int j = 0; j < c->S; ++j) {
+                    for (int j = 0; j < S; ++j) {
                         rg.si_k = j;
                         time_step(c, rg, rw, fsa, t0 + j, snap0);
                     }
                 }));
                 RETIF(cga.run(t1 - 1, [&]() {
-                    for (int j = 0; j < c->S; ++j) {
+                    for (int j = 0; j < S; ++j) {
                         ag.si_k = j;
                         adj_step(c, ag, w, sa, t1 - 1 - j, 0);
+                        flush_sums_at(c, nb, j);
                     }
                 }));
-                continue;
+                return FWI_OK;
             }
             for (int it = t0; it < t1; ++it) {
                 const size_t off = (size_t)((it - t0) / sdt) * slab;
@@ -853,8 +937,49 @@ int run_adjoint(fwi_ctx *c, int s0, int nb) {
                                  c->snap[4] + off};
                 time_step(c, r, rw, fsa, it, it % sdt == 0 ? dst : nullptr);
             }
-            for (int it = t1 - 1; it >= t0; --it) adj_step(c, a, w, sa, it, (size_t)((it - t0) / sdt) * slab);
-        }
+            for (int it = t1 - 1; it >= t0; --it) {
+                adj_step(c, a, w, sa, it, (size_t)((it - t0) / sdt) * slab);
+                if (flush_after(nt, it)) flush_sums(c, nb);
+            }
+            return FWI_OK;
+        };
+        // the leaves of units [a0, a0 + n) in reverse order; the state at a0 is in slot hold
+        // (-1: the zero state of unit 0) and slots [slots - s, slots) are free.  top() is the
+        // chain the forward pass walked: its split states are already saved.
+        std::function<int(int, int, int, int)> rev, top;
+        auto restart_each = [&](int a0, int n, int hold) -> int {   // no free slot
+            for (int j = n - 1; j >= 0; --j) {
+                RETIF(restore(hold));
+                RETIF(advance(a0, j));
+                RETIF(leaf(a0 + j));
+            }
+            return FWI_OK;
+        };
+        rev = [&](int a0, int n, int sfree, int hold) -> int {
+            if (n == 1) {
+                RETIF(restore(hold));
+                return leaf(a0);
+            }
+            if (sfree == 0) return restart_each(a0, n, hold);
+            const int m = c->ck_rev[(size_t)n * W + sfree], slot = c->slots - sfree;
+            RETIF(restore(hold));
+            RETIF(advance(a0, m));
+            RETIF(save(slot));
+            RETIF(rev(a0 + m, n - m, sfree - 1, slot));
+            return rev(a0, m, sfree, hold);
+        };
+        top = [&](int a0, int n, int sfree, int hold) -> int {
+            if (n == 1) {
+                RETIF(restore(hold));
+                return leaf(a0);
+            }
+            if (sfree == 0) return restart_each(a0, n, hold);
+            const int m = c->ck_top[(size_t)n * W + sfree], slot = c->slots - sfree;
+            RETIF(top(a0 + m, n - m, sfree - 1, slot));
+            return rev(a0, m, sfree, hold);
+        };
+        RETIF(top(0, c->nseg, c->slots, -1));
+        if (nt % kFlushSteps) flush_sums(c, nb);
     }
     c->last_s0 = s0;
     c->last_nb = nb;
@@ -894,7 +1019,10 @@ int batch_gradient(fwi_ctx *c, int s0, int nb, const double *adj_z, const double
     RETIF(run_forward(c, s0, nb, true, adj_z == nullptr, true));
     const int r0 = c->rec_off[s0], r1 = c->rec_off[s0 + nb];
     const size_t rows = (size_t)(r1 - r0) * c->g.nt, off = (size_t)r0 * c->g.nt;
-    if (adj_z) {
+    if (adj_z && c->adj_on_device) {   // the caller's device arrays, read by a kernel
+        dev_copy(c->res_z + off, adj_z, rows, c->st);
+        dev_copy(c->res_x + off, adj_x, rows, c->st);
+    } else if (adj_z) {
         HIPCHK(hipMemcpyAsync(c->res_z + off, adj_z, sizeof(double) * rows, hipMemcpyHostToDevice, c->st));
         HIPCHK(hipMemcpyAsync(c->res_x + off, adj_x, sizeof(double) * rows, hipMemcpyHostToDevice, c->st));
     } else {
@@ -927,7 +1055,7 @@ struct Need {
     size_t shared, per_shot;
 };
 
-Need need_bytes(const fwi_ctx *c, int storage, int S) {
+Need need_bytes(const fwi_ctx *c, int storage, int S, int slots) {
     const size_t nz = c->g.nz, nx = c->g.nx, nt = c->g.nt, fs = c->fs();
     const size_t snapcell = (size_t)c->snz * c->snx;
     Need n{};
@@ -935,7 +1063,7 @@ Need need_bytes(const fwi_ctx *c, int storage, int S) {
                (size_t)c->nsrc_tot * nt * 8 * 2 + (64 << 20);
     if (c->o.param == FWI_PARAM_VELOCITY) n.shared += 4 * nz * nx * 8;
     const size_t state_ck = 5 * fs * 4 + 4 * (c->psiz() + c->psix()) * 4;
-    n.per_shot = 9 * fs * 4 + 8 * (c->psiz() + c->psix()) * 4 + 5 * snapcell * 4;
+    n.per_shot = 9 * fs * 4 + 8 * (c->psiz() + c->psix()) * 4 + 5 * snapcell * 4 + 5 * snapcell * 8;
     if (c->plain) n.per_shot += 12 * fs * 4;
     // second time level of the one-pass step (5 fields, 4 stress-side psi)
     const size_t level2 = c->fusable ? (5 * fs + 2 * (c->psiz() + c->psix())) * 4 : 0;
@@ -943,50 +1071,120 @@ Need need_bytes(const fwi_ctx *c, int storage, int S) {
     if (storage == FWI_STORE_FULL) {
         n.per_shot += 5 * (size_t)c->snt * snapcell * 4;
     } else {
-        const size_t nseg = (nt + S - 1) / S;
-        n.per_shot += state_ck + level2 + nseg * state_ck + 5 * (size_t)(S / c->g.snap_dt) * snapcell * 4;
+        n.per_shot += state_ck + level2 + (size_t)slots * state_ck + 5 * (size_t)(S / c->g.snap_dt) * snapcell * 4;
     }
     return n;
 }
+
+// Binomial checkpointing over units of S time steps (Griewank's revolve, the splits
+// chosen by dynamic programming on a unit-advance cost).  rev(n, s): units advanced to
+// run the leaves of a chain of n units in reverse order with the state at its first
+// unit held and s free slots: a single unit needs none; with no free slot every leaf
+// restarts from the held state (n (n - 1) / 2); otherwise advance m units, save, reverse
+// the right part with s - 1 slots, then the left part with s.  top(n, s): the chain the
+// forward pass walks, whose advances are that pass itself.  Returns top(n, s) and fills
+// the best splits (single level, s >= n - 1: every boundary saved, nothing advanced).
+long long plan_schedule(int n, int s, std::vector<int> &top_m, std::vector<int> &rev_m) {
+    const int W = s + 1;
+    top_m.assign((size_t)(n + 1) * W, 1);
+    rev_m.assign((size_t)(n + 1) * W, 1);
+    if (s >= n - 1) return 0;
+    std::vector<long long> rev((size_t)(n + 1) * W, 0), top((size_t)(n + 1) * W, 0);
+    auto at = [W](int k, int j) { return (size_t)k * W + j; };
+    for (int k = 2; k <= n; ++k) {
+        rev[at(k, 0)] = top[at(k, 0)] = (long long)k * (k - 1) / 2;
+        for (int j = 1; j <= s; ++j) {
+            long long br = -1, bt = -1;
+            int mr = 1, mt = 1;
+            for (int m = 1; m < k; ++m) {
+                const long long r = m + rev[at(k - m, j - 1)] + rev[at(m, j)];
+                const long long t = top[at(k - m, j - 1)] + rev[at(m, j)];
+                if (br < 0 || r < br) br = r, mr = m;
+                if (bt < 0 || t < bt) bt = t, mt = m;
+            }
+            rev[at(k, j)] = br;
+            top[at(k, j)] = bt;
+            rev_m[at(k, j)] = mr;
+            top_m[at(k, j)] = mt;
+        }
+    }
+    return top[at(n, s)];
+}
+
+constexpr int kMaxUnits = 2048;   // the schedule's dynamic programme is O(units^2 slots)
 
 int plan(fwi_ctx *c) {
     size_t freeb = 0, total = 0;
     HIPCHK(hipMemGetInfo(&freeb, &total));
     const size_t budget = c->o.mem_budget > 0 ? (size_t)c->o.mem_budget : (size_t)(0.9 * (double)freeb);
-    const int sdt = c->g.snap_dt;
-    // segment length for checkpoints: minimise nseg * state + S/snap_dt * snapshot, multiple of snap_dt
+    const int sdt = c->g.snap_dt, nt = c->g.nt;
+    const bool even = c->o.use_graph != 0;   // replayed segments need an even step count
+    auto leaf_len = [&](int S_) {            // a multiple of snap_dt, at most the time axis rounded up
+        S_ = std::max(sdt, (S_ + sdt - 1) / sdt * sdt);
+        S_ = std::min(S_, (nt + sdt - 1) / sdt * sdt);
+        if (even && S_ % 2) S_ += sdt;       // (snap_dt is then odd)
+        return S_;
+    };
+    auto units_of = [&](int S_) { return (nt + S_ - 1) / S_; };
+    auto slots_cap = [&](int S_) {           // every unit boundary, or fewer as asked
+        const int full = units_of(S_) - 1;
+        return c->o.ckpt_slots > 0 ? std::min(full, c->o.ckpt_slots) : full;
+    };
+    // segment length: minimise nseg * state + S/snap_dt * snapshot
     int S = c->o.ckpt_interval;
     if (S <= 0) {
         const double state_ck = 5.0 * c->fs() + 4.0 * (c->psiz() + c->psix());
         const double snapb = 5.0 * c->snz * c->snx;
-        S = (int)std::lround(std::sqrt((double)c->g.nt * state_ck * sdt / snapb));
+        S = (int)std::lround(std::sqrt((double)nt * state_ck * sdt / snapb));
     }
-    S = std::max(sdt, (S + sdt - 1) / sdt * sdt);
-    S = std::min(S, (c->g.nt + sdt - 1) / sdt * sdt);
-    if (c->o.use_graph && S % 2) S += sdt;   // replayed segments need an even step count (sdt is then odd)
+    S = leaf_len(S);
+    int slots = slots_cap(S);
     int storage = c->o.storage;
-    auto fits = [&](int st_, int b) {
-        Need n = need_bytes(c, st_, S);
-        return n.shared + (size_t)b * n.per_shot <= budget;
-    };
     if (c->plain) {
         if (storage == FWI_STORE_CHECKPOINT) return c->fail(FWI_EINVAL, "the plain form stores full snapshots");
         storage = FWI_STORE_FULL;
     }
+    auto max_batch = [&](int st_, int S_, int sl_) {
+        Need n = need_bytes(c, st_, S_, sl_);
+        return n.shared + n.per_shot > budget ? 0 : (int)std::min<size_t>((budget - n.shared) / n.per_shot, c->nshot);
+    };
     if (storage == FWI_STORE_AUTO) {
         // Full storage moves 40 B per cell-step (store + load) against the ~100 B of a
         // recomputed step, but its footprint caps the batch; batching is worth more
         // than the recomputation, so full storage is kept only if it still allows
         // the batch checkpointing would (or every shot).
-        auto max_batch = [&](int st_) {
-            Need n = need_bytes(c, st_, S);
-            return n.shared + n.per_shot > budget ? 0 : (int)std::min<size_t>((budget - n.shared) / n.per_shot, c->nshot);
-        };
-        const int bf = max_batch(FWI_STORE_FULL), bc = max_batch(FWI_STORE_CHECKPOINT);
+        const int bf = max_batch(FWI_STORE_FULL, S, slots), bc = max_batch(FWI_STORE_CHECKPOINT, S, slots);
         const int want = c->o.batch > 0 ? std::min(c->o.batch, c->nshot) : c->nshot;
         storage = (bf >= std::min(want, bc) && bf > 0) ? FWI_STORE_FULL : FWI_STORE_CHECKPOINT;
     }
-    Need n = need_bytes(c, storage, S);
+    if (storage == FWI_STORE_CHECKPOINT && max_batch(storage, S, slots) == 0) {
+        // One shot with a checkpoint per segment does not fit: fewer slots and a binomial
+        // schedule, over the segment lengths that leave room for at least one slot, the
+        // one that recomputes the fewest steps.
+        const Need n0 = need_bytes(c, storage, S, 0), n1 = need_bytes(c, storage, S, 1);
+        const size_t state_ck = n1.per_shot - n0.per_shot;
+        long long best = -1;
+        int bS = 0, bsl = 0;
+        for (int div = 1; div <= 16; div *= 2) {
+            const int S_ = leaf_len(S / div);
+            const int n_ = units_of(S_);
+            const Need z = need_bytes(c, storage, S_, 0);
+            if (n_ > kMaxUnits || z.shared + z.per_shot > budget) continue;
+            const int sl = (int)std::min<size_t>((budget - z.shared - z.per_shot) / state_ck, (size_t)slots_cap(S_));
+            std::vector<int> tm, rm;
+            const long long cost = ((long long)n_ + plan_schedule(n_, sl, tm, rm)) * S_;
+            if (best < 0 || cost < best) best = cost, bS = S_, bsl = sl;
+            if (S_ == sdt) break;
+        }
+        if (best < 0) {
+            const Need n = need_bytes(c, storage, S, 0);
+            return c->fail(FWI_ENOMEM, "one shot needs %.2f GB even without checkpoints, budget %.2f GB",
+                           (n.shared + n.per_shot) / 1e9, budget / 1e9);
+        }
+        S = bS;
+        slots = bsl;
+    }
+    Need n = need_bytes(c, storage, S, slots);
     if (n.shared + n.per_shot > budget)
         return c->fail(FWI_ENOMEM, "one shot needs %.2f GB, budget %.2f GB", (n.shared + n.per_shot) / 1e9,
                        budget / 1e9);
@@ -998,7 +1196,22 @@ int plan(fwi_ctx *c) {
     c->B = (c->nshot + nbat - 1) / nbat;
     c->storage = storage;
     c->S = S;
-    c->nseg = (c->g.nt + S - 1) / S;
+    c->nseg = units_of(S);
+    c->slots = storage == FWI_STORE_CHECKPOINT ? slots : 0;
+    c->spine_slot.assign(c->nseg, -1);
+    if (storage == FWI_STORE_CHECKPOINT) {
+        if (c->nseg > kMaxUnits && c->slots < c->nseg - 1)
+            return c->fail(FWI_EINVAL, "%d segments with %d checkpoints: more than %d segments need one each",
+                           c->nseg, c->slots, kMaxUnits);
+        c->extra_units = plan_schedule(c->nseg, c->slots, c->ck_top, c->ck_rev);
+        // the states the forward pass saves: the splits along top()'s first chain
+        for (int a0 = 0, k = c->nseg, j = c->slots; k > 1 && j > 0; --j) {
+            const int m = c->slots >= c->nseg - 1 ? 1 : c->ck_top[(size_t)k * (c->slots + 1) + j];
+            c->spine_slot[a0 + m] = c->slots - j;
+            a0 += m;
+            k -= m;
+        }
+    }
     return FWI_OK;
 }
 
@@ -1042,6 +1255,7 @@ int allocate(fwi_ctx *c) {
     for (int i = 4; i < 8; ++i) c->psi[i] = F(B * c->psix());
     for (int i = 0; i < 3; ++i) c->gr[i] = F(B * snapcell);
     c->gx = F(2 * B * snapcell);
+    c->gr64 = D(5 * (size_t)B * snapcell);
     auto level2 = [&](float **f2, float **psi2) {
         for (int i = 0; i < 5; ++i) f2[i] = F(B * fs);
         for (int i = 0; i < 2; ++i) psi2[i] = F(B * c->psiz());
@@ -1057,7 +1271,8 @@ int allocate(fwi_ctx *c) {
         for (int i = 4; i < 8; ++i) c->rpsi[i] = F(B * c->psix());
         if (c->fusable) level2(c->rf2, c->rpsi2);
         c->ck_bytes_per_seg = sizeof(float) * B * (5 * fs + 4 * (c->psiz() + c->psix()));
-        c->ck = F(c->ck_bytes_per_seg / sizeof(float) * c->nseg);
+        c->ck = F(c->ck_bytes_per_seg / sizeof(float) * std::max(c->slots, 1));
+        c->rf_canon = c->rf[0];
     }
     for (int i = 0; i < 5; ++i) c->snap[i] = F(B * c->snap_stride);
     if (c->plain) {
@@ -1246,10 +1461,14 @@ int validate(fwi_ctx *c, const fwi_grid *g, const fwi_acq *q) {
         return c->fail(FWI_EINVAL,
                        "tile_tz %d (0, 8, 16, 32 or 64; variant 6: 0 or its row block times 1 or 2; "
                        "variants 7, 8: 0 or their row block times 4 or 8)", tt);
-    if (c->o.kernel_variant < 0 || c->o.kernel_variant > 8)
-        return c->fail(FWI_EINVAL, "kernel_variant %d (0..8)", c->o.kernel_variant);
+    if (c->o.kernel_variant < 0 || c->o.kernel_variant > 9)
+        return c->fail(FWI_EINVAL, "kernel_variant %d (0..9)", c->o.kernel_variant);
+    if (c->o.kernel_variant == 9 && tt > 2) return c->fail(FWI_EINVAL, "variant 9 takes tile_tz 0, 1 or 2");
     if (c->o.work_order < 0 || c->o.work_order > 4)
         return c->fail(FWI_EINVAL, "work_order %d (0..4)", c->o.work_order);
+    if (c->o.ckpt_slots < 0) return c->fail(FWI_EINVAL, "ckpt_slots %d < 0", c->o.ckpt_slots);
+    if (c->o.smooth_hfs != 0 && (c->o.smooth_hfs < 2 || c->o.smooth_hfs > 64))
+        return c->fail(FWI_EINVAL, "smooth_hfs %d (0, or 2..64)", c->o.smooth_hfs);
     if (c->o.param != FWI_PARAM_LAME && c->o.param != FWI_PARAM_VELOCITY)
         return c->fail(FWI_EINVAL, "param %d (0 lam/mu/rho, 1 cp/cs)", c->o.param);
     if (c->o.param == FWI_PARAM_VELOCITY) {
@@ -1416,12 +1635,33 @@ int32_t fwi_plan(const fwi_ctx *c, int32_t *batch, int32_t *storage, int32_t *ck
     return FWI_OK;
 }
 
+int32_t fwi_checkpoint_plan(const fwi_ctx *c, int32_t *units, int32_t *slots, double *recompute) {
+    if (!c) return FWI_EINVAL;
+    const bool ck = c->storage == FWI_STORE_CHECKPOINT;
+    if (units) *units = ck ? c->nseg : 0;
+    if (slots) *slots = ck ? c->slots : 0;
+    // the leaves recompute every step once; advances are whole segments
+    if (recompute) *recompute = ck ? ((double)c->g.nt + (double)c->extra_units * c->S) / c->g.nt : 0.0;
+    return FWI_OK;
+}
+
 int32_t fwi_set_model(fwi_ctx *c, const double *lam, const double *mu, const double *rho) {
     if (!c || !lam || !mu || !rho) return FWI_EINVAL;
     const size_t n = (size_t)c->g.nz * c->g.nx;
     RETIF(upload(c, c->m_lam, lam, n));
     RETIF(upload(c, c->m_mu, mu, n));
     RETIF(upload(c, c->m_rho, rho, n));
+    RETIF(refresh_media(c));
+    c->have_model = 1;
+    return FWI_OK;
+}
+
+int32_t fwi_set_model_device(fwi_ctx *c, const double *lam, const double *mu, const double *rho) {
+    if (!c || !lam || !mu || !rho) return FWI_EINVAL;
+    const size_t n = (size_t)c->g.nz * c->g.nx;
+    dev_copy(c->m_lam, lam, n, c->st);
+    dev_copy(c->m_mu, mu, n, c->st);
+    dev_copy(c->m_rho, rho, n, c->st);
     RETIF(refresh_media(c));
     c->have_model = 1;
     return FWI_OK;
@@ -1484,6 +1724,23 @@ int32_t fwi_forward(fwi_ctx *c, int32_t shot0, int32_t nshots, double *seis_z, d
     return FWI_OK;
 }
 
+int32_t fwi_forward_device(fwi_ctx *c, int32_t shot0, int32_t nshots, double *seis_z, double *seis_x) {
+    if (!c) return FWI_EINVAL;
+    if (!c->have_model) return c->fail(FWI_ESTATE, "fwi_forward_device before a model");
+    std::vector<std::pair<int, int>> bs;
+    if (batches(c, shot0, nshots, bs) != FWI_OK) return c->fail(FWI_EINVAL, "shot range");
+    const int nt = c->g.nt;
+    for (auto [s0, nb] : bs) {
+        RETIF(run_forward(c, s0, nb, false, true, true));
+        const int r0 = c->rec_off[s0], r1 = c->rec_off[s0 + nb], rb = c->rec_off[shot0];
+        const size_t n = (size_t)(r1 - r0) * nt;
+        if (seis_z) dev_copy(seis_z + (size_t)(r0 - rb) * nt, c->syn_z + (size_t)r0 * nt, n, c->st);
+        if (seis_x) dev_copy(seis_x + (size_t)(r0 - rb) * nt, c->syn_x + (size_t)r0 * nt, n, c->st);
+        RETIF(sync(c));
+    }
+    return FWI_OK;
+}
+
 int32_t fwi_get_wavefield(fwi_ctx *c, int32_t shot, int32_t field, double *out) {
     if (!c || !out || field < 0 || field > 8) return FWI_EINVAL;
     if (shot < c->last_s0 || shot >= c->last_s0 + c->last_nb)
@@ -1533,7 +1790,9 @@ int32_t fwi_gradient(fwi_ctx *c, int32_t shot0, int32_t nshots, double *misfit) 
         const size_t snapcell = (size_t)c->snz * c->snx;
         for (int s = 0; s < nb; ++s) {
             const double alpha = c->alpha[s0 + s];
-            const float *gs[3] = {c->gr[0] + s * snapcell, c->gr[1] + s * snapcell, c->gr[2] + s * snapcell};
+            const size_t cells = (size_t)c->B * snapcell;
+            const double *gs[3] = {c->gr64 + s * snapcell, c->gr64 + cells + s * snapcell,
+                                   c->gr64 + 2 * cells + s * snapcell};
             const double gsc[3] = {1.0 / (alpha * kGainLM), 1.0 / (alpha * kGainLM), 1.0 / alpha};
             dev_precondition(c->f[7] + s * c->fs(), c->f[8] + s * c->fs(), 1.0 / (alpha * alpha), gs, gsc,
                              c->G, c->C, c->tmp, c->red, nz, nx, c->P, c->g.snap_z1, c->g.snap_z2, c->g.snap_x1,
@@ -1554,14 +1813,15 @@ int32_t fwi_gradient_raw(fwi_ctx *c, int32_t shot0, int32_t nshots, const double
     double *outs[3] = {g_lam, g_mu, g_rho};
     for (int k = 0; k < 3; ++k)
         if (outs[k]) std::memset(outs[k], 0, snapcell * 8);
-    std::vector<float> h(snapcell * c->B);
+    std::vector<double> h(snapcell * c->B);
     const int nt = c->g.nt, rb = c->rec_off[shot0];
     for (auto [s0, nb] : bs) {
         const size_t off = (size_t)(c->rec_off[s0] - rb) * nt;
         RETIF(batch_gradient(c, s0, nb, adj_z + off, adj_x + off, nullptr));
         for (int k = 0; k < 3; ++k) {
             if (!outs[k]) continue;
-            HIPCHK(hipMemcpyAsync(h.data(), c->gr[k], snapcell * nb * 4, hipMemcpyDeviceToHost, c->st));
+            HIPCHK(hipMemcpyAsync(h.data(), c->gr64 + (size_t)k * c->B * snapcell, snapcell * nb * 8,
+                                  hipMemcpyDeviceToHost, c->st));
             RETIF(sync(c));
             for (int s = 0; s < nb; ++s) {
                 const double den = c->alpha[s0 + s] * (k < 2 ? kGainLM : 1.0);
@@ -1572,20 +1832,22 @@ int32_t fwi_gradient_raw(fwi_ctx *c, int32_t shot0, int32_t nshots, const double
     return FWI_OK;
 }
 
-int32_t fwi_misfit_gradient(fwi_ctx *c, int32_t shot0, int32_t nshots, const double *adj_z, const double *adj_x,
-                            double *d_lam, double *d_mu, double *d_rho) {
-    if (!c || !adj_z || !adj_x || !d_lam || !d_mu || !d_rho) return FWI_EINVAL;
-    if (!c->have_model) return c->fail(FWI_ESTATE, "fwi_misfit_gradient before fwi_set_model");
+// dJ/d(lam, mu, rho) into the context's scratch [3][nz][nx] (c->tmp holds 2 of them, the
+// third goes to the device array d3): adjoint sources adj_z / adj_x are device arrays
+// [rows][nt] (the batch's rows are copied into res_* by a kernel).
+int misfit_gradient_dev(fwi_ctx *c, int shot0, int nshots, const double *adj_z, const double *adj_x,
+                        double *d_lam, double *d_mu, double *d_rho) {
     if (c->plain) return c->fail(FWI_EINVAL, "the plain form computes the reference's sums only");
     if (c->g.snap_dz != 1 || c->g.snap_dx != 1)
         return c->fail(FWI_EINVAL, "the misfit derivative needs snap_dz = snap_dx = 1 (staggered chain rule)");
     std::vector<std::pair<int, int>> bs;
     if (batches(c, shot0, nshots, bs) != FWI_OK) return c->fail(FWI_EINVAL, "shot range");
-    const int snz = c->snz, snx = c->snx, z1 = c->g.snap_z1, x1 = c->g.snap_x1, nx = c->g.nx;
-    const size_t snapcell = (size_t)snz * snx;
-    // raw sums over the shots, FP64: s1, s1 + s2, rz, s3, rx
-    std::vector<double> raw(5 * snapcell, 0.0);
-    std::vector<float> h(snapcell * c->B);
+    const size_t snapcell = (size_t)c->snz * c->snx, cells = (size_t)c->B * snapcell;
+    if (!c->raw) {   // FP64 totals of the five sums over the shots, allocated on first use
+        c->raw = c->dalloc<double>(5 * snapcell);
+        if (!c->raw) return c->fail(FWI_ENOMEM, "misfit-derivative sums");
+    }
+    HIPCHK(hipMemsetAsync(c->raw, 0, sizeof(double) * 5 * snapcell, c->st));
     const int nt = c->g.nt, rb = c->rec_off[shot0];
     c->gform = 1;
     for (auto [s0, nb] : bs) {
@@ -1595,55 +1857,176 @@ int32_t fwi_misfit_gradient(fwi_ctx *c, int32_t shot0, int32_t nshots, const dou
             c->gform = 0;
             return r;
         }
-        for (int k = 0; k < 5; ++k) {
-            const float *src = k < 3 ? c->gr[k] : c->gx + (size_t)(k - 3) * c->B * snapcell;
-            HIPCHK(hipMemcpyAsync(h.data(), src, snapcell * nb * 4, hipMemcpyDeviceToHost, c->st));
-            RETIF(sync(c));
-            const bool gained = k == 0 || k == 1 || k == 3;   // the lam / mu terms carry kGainLM
-            for (int s = 0; s < nb; ++s) {
-                const double den = c->alpha[s0 + s] * (gained ? kGainLM : 1.0);
-                for (size_t i = 0; i < snapcell; ++i) raw[k * snapcell + i] += (double)h[s * snapcell + i] / den;
-            }
+        // per-shot totals of this batch (gr64 [k][B][cells]) into the sums over shots
+        for (int s = 0; s < nb; ++s) {
+            const double *g[5];
+            for (int k = 0; k < 5; ++k) g[k] = c->gr64 + k * cells + s * snapcell;
+            dev_acc_raw64(c->raw, g, snapcell, c->alpha_d + s0 + s, kGainLM, c->st);
         }
     }
     c->gform = 0;
     // The sums were taken with weight grad_sign * snap_dt * dt; with the CPU sign (-1)
     // they are +dJ/dm of J = sum 0.5 dt r^2 for adjoint sources r.
     const double sg = -1.0 / c->o.grad_sign;
-    std::vector<double> mu((size_t)c->g.nz * nx);
-    HIPCHK(hipMemcpyAsync(mu.data(), c->m_mu, mu.size() * 8, hipMemcpyDeviceToHost, c->st));
-    RETIF(sync(c));
-    const size_t n = (size_t)c->g.nz * nx;
-    std::memset(d_lam, 0, n * 8);
-    std::memset(d_mu, 0, n * 8);
-    std::memset(d_rho, 0, n * 8);
-    auto R = [&](int k, int jz, int jx) {   // raw sum k at box cell (jz, jx), 0 outside the box
-        return (jz < 0 || jx < 0) ? 0.0 : raw[k * snapcell + (size_t)jz * snx + jx];
+    dev_misfit_chain(c->raw, c->m_mu, d_lam, d_mu, d_rho, c->g.nz, c->g.nx, c->g.snap_z1, c->g.snap_x1, c->snz,
+                     c->snx, sg, c->st);
+    return sync(c);
+}
+
+// The exact derivative of J = sum 0.5 dt r^2 (adjoint sources r = adj_z / adj_x, host or
+// device arrays [rows][nt]) with respect to lam, mu, rho into device arrays [nz][nx] (zero
+// outside the snapshot box): the forward run storing its intermediates for every step, the
+// reverse sweep of psv_exact.hip, the media gradient carried through mat_av2.  Its buffers
+// are allocated for the call and freed before it returns; shots go in batches that fit.
+int exact_gradient(fwi_ctx *c, int shot0, int nshots, const double *adj_z, const double *adj_x, bool on_device,
+                   double *d_lam, double *d_mu, double *d_rho) {
+    if (shot0 < 0 || nshots <= 0 || shot0 + nshots > c->nshot) return c->fail(FWI_EINVAL, "shot range");
+    const int nz = c->g.nz, nx = c->g.nx, nt = c->g.nt;
+    const size_t fs = c->fs();
+    // per shot: 30 float fields (state, memories, adjoint state, carries, scratch), the FP64
+    // media gradient (5) and the intermediates of every step (5 nt)
+    const size_t per_shot = fs * 4 * (30 + 5 * (size_t)nt) + fs * 8 * 5;
+    size_t freeb = 0, total = 0;
+    HIPCHK(hipMemGetInfo(&freeb, &total));
+    const size_t avail = (size_t)(0.85 * (double)freeb);
+    const size_t g_all = fs * 8 * 5 * (size_t)nshots;   // the media gradients of every shot
+    if (per_shot + g_all > avail)
+        return c->fail(FWI_ENOMEM, "the exact adjoint keeps %.2f GB per shot (5 fields per step), %.2f GB free",
+                       per_shot / 1e9, avail / 1e9);
+    const int Bx = (int)std::min<size_t>((size_t)nshots, (avail - g_all) / per_shot);
+    std::vector<void *> mem;
+    auto A = [&](size_t bytes) -> void * {
+        void *p = nullptr;
+        if (hipMalloc(&p, bytes) != hipSuccess) return nullptr;
+        mem.push_back(p);
+        return p;
     };
-    for (int jz = 0; jz < snz; ++jz)
-        for (int jx = 0; jx < snx; ++jx) {
-            const int iz = z1 + jz, ix = x1 + jx;
-            const size_t o = (size_t)iz * nx + ix;
-            d_lam[o] = sg * R(0, jz, jx);
-            // mu_zx at (i, j) is the harmonic mean of mu at (i..i+1, j..j+1)
-            // (mat_av2, src/cpu/n_kernel_lib_PSV.cpp:875-880); the shear term of mu at
-            // (i, j) gathers the four mu_zx points it enters: d mu_zx / d mu = mu_zx^2 / (4 mu^2),
-            // with the s3 sum's own 1/mu^2 undone at each point
-            double shear = 0.0;
-            for (int dz = 0; dz <= 1; ++dz)
-                for (int dx = 0; dx <= 1; ++dx) {
-                    const int pz = iz - dz, px = ix - dx;
-                    if (jz - dz < 0 || jx - dx < 0) continue;
-                    const double mp = mu[(size_t)pz * nx + px];
-                    shear += R(3, jz - dz, jx - dx) * mp * mp;
-                }
-            const double mk = mu[o];
-            d_mu[o] = sg * (R(1, jz, jx) + (mk != 0.0 ? shear / (4.0 * mk * mk) : 0.0));
-            // rho_zp at (i, j) averages rho at (i, j) and (i+1, j), rho_xp at (i, j) and (i, j+1)
-            // (mat_av2, :886-887): each density gathers half of the two points of each kind
-            d_rho[o] = sg * 0.5 * (R(2, jz, jx) + R(2, jz - 1, jx) + R(4, jz, jx) + R(4, jz, jx - 1));
+    auto release = [&]() {
+        (void)hipStreamSynchronize(c->st);
+        for (void *p : mem) (void)hipFree(p);
+        mem.clear();
+    };
+    XArgs x{};
+    x.nz = nz; x.nx = nx; x.P = c->P; x.B = Bx; x.N = c->N;
+    x.pml_z = c->g.pml_z; x.pml_x = c->g.pml_x;
+    x.fs = fs;
+    x.dt = (float)c->g.dt;
+    for (int k = 0; k < 6; ++k) {
+        x.cz[k] = k < c->N ? (float)(c->g.hc[k + 1] / c->g.dz) : 0.f;
+        x.cx[k] = k < c->N ? (float)(c->g.hc[k + 1] / c->g.dx) : 0.f;
+    }
+    x.lam = c->lam; x.mu = c->mu; x.muzx = c->muzx; x.rhoz = c->rhoz; x.rhox = c->rhox;
+    x.zp = c->zp; x.xp = c->xp;
+    float *fl = static_cast<float *>(A(sizeof(float) * fs * Bx * 30));
+    float *hist = static_cast<float *>(A(sizeof(float) * fs * Bx * 5 * (size_t)nt));
+    double *G = static_cast<double *>(A(sizeof(double) * fs * 5 * nshots));
+    if (!fl || !hist || !G) {
+        release();
+        return c->fail(FWI_ENOMEM, "exact adjoint buffers");
+    }
+    float **slots[] = {&x.vz, &x.vx, &x.szz, &x.szx, &x.sxx, &x.pv[0], &x.pv[1], &x.pv[2], &x.pv[3],
+                       &x.ps[0], &x.ps[1], &x.ps[2], &x.ps[3], &x.bvz, &x.bvx, &x.bzz, &x.bzx, &x.bxx,
+                       &x.cv[0], &x.cv[1], &x.cv[2], &x.cv[3], &x.cs[0], &x.cs[1], &x.cs[2], &x.cs[3],
+                       &x.E[0], &x.E[1], &x.E[2], &x.E[3]};
+    for (int k = 0; k < 30; ++k) *slots[k] = fl + (size_t)k * fs * Bx;
+    for (int k = 0; k < 4; ++k) x.D[k] = x.Dv[k] = x.E[k];   // one scratch set serves all three
+    x.hist = hist;
+    const int rb = c->rec_off[shot0], rows = c->rec_off[shot0 + nshots] - rb;
+    const size_t nrow = (size_t)rows * nt, off = (size_t)rb * nt;
+    // the receivers' adjoint sources, in res_*: rc x reverse cumulative sum of dt adj
+    if (on_device) {
+        dev_copy(c->res_z + off, adj_z, nrow, c->st);
+        dev_copy(c->res_x + off, adj_x, nrow, c->st);
+    } else {
+        RETIF(upload(c, c->res_z + off, adj_z, nrow));
+        RETIF(upload(c, c->res_x + off, adj_x, nrow));
+    }
+    x_revcumsum(c->res_z + off, c->res_x + off, c->res_z + off, c->res_x + off, rows, nt, c->g.dt,
+                c->g.dt / (c->g.dz * c->g.dx), c->st);
+    int isurf[4];
+    for (int k = 0; k < 4; ++k) isurf[k] = c->g.isurf[k];
+    HIPCHK(hipMemsetAsync(G, 0, sizeof(double) * fs * 5 * nshots, c->st));
+    int r = FWI_OK;
+    for (int s0 = shot0; s0 < shot0 + nshots && r == FWI_OK; s0 += Bx) {
+        const int nb = std::min(Bx, shot0 + nshots - s0);
+        x.B = nb;
+        x.G = G + (size_t)(s0 - shot0) * 5 * fs;
+        (void)hipMemsetAsync(fl, 0, sizeof(float) * fs * Bx * 30, c->st);
+        KArgs ka{};   // the injection kernel's view of the forward / adjoint velocities
+        ka.P = c->P; ka.fs = fs; ka.dt = x.dt; ka.rhoz = c->rhoz; ka.rhox = c->rhox;
+        ka.vz = x.vz; ka.vx = x.vx;
+        const SrcArgs fsa = src_args(c, s0, nb);
+        for (int n = 0; n < nt; ++n) {
+            x_forward_step(x, n, isurf, c->st);
+            launch_inject(ka, fsa, n, c->st);
         }
-    return FWI_OK;
+        SrcArgs asa = adj_args(c, s0, nb);   // receiver groups, rows of res_*
+        asa.type = 1;                        // v' += A (the running sum's transpose)
+        asa.scale_shot = nullptr;
+        KArgs kb = ka;
+        kb.vz = x.bvz;
+        kb.vx = x.bvx;
+        const SrcArgs *t0 = c->stf_type == 0 ? &fsa : nullptr;
+        for (int n = nt - 1; n >= 0; --n) {
+            launch_inject(kb, asa, n, c->st);
+            x_adjoint_step(x, n, isurf, t0, c->st);
+        }
+        if (hipGetLastError() != hipSuccess) r = c->fail(FWI_EHIP, "exact adjoint launch");
+    }
+    if (r == FWI_OK) {
+        x.G = G;
+        x_media_chain(x, nshots, c->m_mu, c->m_rho, c->g.snap_z1, c->g.snap_z2, c->g.snap_x1, c->g.snap_x2, d_lam,
+                      d_mu, d_rho, c->st);
+        const hipError_t e = hipStreamSynchronize(c->st);
+        if (e != hipSuccess) r = c->fail(FWI_EHIP, "exact adjoint: %s", hipGetErrorString(e));
+    }
+    release();
+    return r;
+}
+
+int32_t fwi_misfit_gradient_exact(fwi_ctx *c, int32_t shot0, int32_t nshots, const double *adj_z,
+                                  const double *adj_x, double *d_lam, double *d_mu, double *d_rho,
+                                  int32_t on_device) {
+    if (!c || !adj_z || !adj_x || !d_lam || !d_mu || !d_rho) return FWI_EINVAL;
+    if (!c->have_model) return c->fail(FWI_ESTATE, "fwi_misfit_gradient_exact before a model");
+    if (on_device) return exact_gradient(c, shot0, nshots, adj_z, adj_x, true, d_lam, d_mu, d_rho);
+    const size_t n = (size_t)c->g.nz * c->g.nx;
+    if (!c->mg_out) {
+        c->mg_out = c->dalloc<double>(3 * n);
+        if (!c->mg_out) return c->fail(FWI_ENOMEM, "misfit-derivative output");
+    }
+    RETIF(exact_gradient(c, shot0, nshots, adj_z, adj_x, false, c->mg_out, c->mg_out + n, c->mg_out + 2 * n));
+    double *outs[3] = {d_lam, d_mu, d_rho};
+    for (int k = 0; k < 3; ++k)
+        HIPCHK(hipMemcpyAsync(outs[k], c->mg_out + k * n, n * 8, hipMemcpyDeviceToHost, c->st));
+    return sync(c);
+}
+
+int32_t fwi_misfit_gradient(fwi_ctx *c, int32_t shot0, int32_t nshots, const double *adj_z, const double *adj_x,
+                            double *d_lam, double *d_mu, double *d_rho) {
+    if (!c || !adj_z || !adj_x || !d_lam || !d_mu || !d_rho) return FWI_EINVAL;
+    if (!c->have_model) return c->fail(FWI_ESTATE, "fwi_misfit_gradient before fwi_set_model");
+    // host arrays: through the context's own device buffers
+    const size_t n = (size_t)c->g.nz * c->g.nx;
+    if (!c->mg_out) {
+        c->mg_out = c->dalloc<double>(3 * n);
+        if (!c->mg_out) return c->fail(FWI_ENOMEM, "misfit-derivative output");
+    }
+    RETIF(misfit_gradient_dev(c, shot0, nshots, adj_z, adj_x, c->mg_out, c->mg_out + n, c->mg_out + 2 * n));
+    double *outs[3] = {d_lam, d_mu, d_rho};
+    for (int k = 0; k < 3; ++k)
+        HIPCHK(hipMemcpyAsync(outs[k], c->mg_out + k * n, n * 8, hipMemcpyDeviceToHost, c->st));
+    return sync(c);
+}
+
+int32_t fwi_misfit_gradient_device(fwi_ctx *c, int32_t shot0, int32_t nshots, const double *adj_z,
+                                   const double *adj_x, double *d_lam, double *d_mu, double *d_rho) {
+    if (!c || !adj_z || !adj_x || !d_lam || !d_mu || !d_rho) return FWI_EINVAL;
+    if (!c->have_model) return c->fail(FWI_ESTATE, "fwi_misfit_gradient_device before a model");
+    c->adj_on_device = 1;
+    const int r = misfit_gradient_dev(c, shot0, nshots, adj_z, adj_x, d_lam, d_mu, d_rho);
+    c->adj_on_device = 0;
+    return r;
 }
 
 int32_t fwi_zero_gradient(fwi_ctx *c) {
@@ -1676,6 +2059,48 @@ int32_t fwi_taper_gradient(fwi_ctx *c) {
     for (int i = 0; i < 3; ++i)
         dev_taper(c->G[i], c->g.nz, c->g.nx, c->g.snap_z1, c->g.snap_z2, c->g.snap_x1, c->g.snap_x2, c->g.taper,
                   c->st);
+    return sync(c);
+}
+
+int32_t fwi_pcg_direction(fwi_ctx *c) {
+    if (!c) return FWI_EINVAL;
+    const size_t n = (size_t)c->g.nz * c->g.nx;
+    if (!c->pcg) {
+        c->pcg = c->dalloc<double>(6 * n);
+        if (!c->pcg) return c->fail(FWI_ENOMEM, "conjugate-gradient state");
+        HIPCHK(hipMemsetAsync(c->pcg, 0, sizeof(double) * 6 * n, c->st));
+    }
+    for (int p = 0; p < 3; ++p) dev_pcg(c->G[p], c->pcg + 2 * p * n, c->pcg + (2 * p + 1) * n, n, c->red, c->st);
+    return sync(c);
+}
+
+int32_t fwi_smooth_gradient(fwi_ctx *c, int32_t hfs) {
+    if (!c) return FWI_EINVAL;
+    if (hfs < 2 || hfs > 64) return c->fail(FWI_EINVAL, "smoothing half width %d (2..64)", hfs);
+    const int w = 2 * hfs + 1;
+    if (c->gfilt_hfs != hfs) {   // gauss_filter_kernel (src/cpu/n_kernel_lib_PSV.cpp:1012-1049)
+        if (!c->gfilt) {
+            c->gfilt = c->dalloc<double>(129 * 129);
+            if (!c->gfilt) return c->fail(FWI_ENOMEM, "Gaussian filter");
+        }
+        std::vector<double> k((size_t)w * w);
+        const double sigma = hfs / 2, q = 2.0 * sigma * sigma;
+        double sum = 0.0;
+        for (int ii = -hfs; ii <= hfs; ++ii)
+            for (int jj = -hfs; jj <= hfs; ++jj) {
+                const double v = std::exp(-((ii * ii) / q) - ((jj * jj) / q));
+                k[(size_t)(jj + hfs) * w + ii + hfs] = v;
+                sum += v;
+            }
+        for (double &v : k) v /= sum;
+        RETIF(upload(c, c->gfilt, k.data(), k.size()));
+        c->gfilt_hfs = hfs;
+    }
+    const size_t n = (size_t)c->g.nz * c->g.nx;
+    for (int p = 0; p < 3; ++p) {
+        HIPCHK(hipMemcpyAsync(c->tmp, c->G[p], n * 8, hipMemcpyDeviceToDevice, c->st));
+        dev_gauss(c->tmp, c->G[p], c->gfilt, c->g.nz, c->g.nx, hfs, c->st);
+    }
     return sync(c);
 }
 
@@ -1748,6 +2173,14 @@ int32_t fwi_update_model(fwi_ctx *c, double eps, double rho_factor, double rho_m
     return refresh_media(c);
 }
 
+// The smallest trial step whose model change the FP32 media resolve: a step eps moves a
+// parameter by at most eps times the larger of its largest value and its clip, i.e. by
+// ~eps of the values themselves, and below 2^-20 (~16 FP32 ulps) a trial's misfit differs
+// from the current one by rounding only.  Such a trial never counts as a decrease (the
+// FP64 reference sees the true sign there: along an ascent direction it halves on to its
+// "Steplength estimation failed" exit, and so does this search).
+constexpr double kResolvedStep = 0x1p-20;
+
 // step_length_PSV (src/cpu/n_step_length_PSV.cpp:13-209) with update_param 0
 int32_t fwi_step_length(fwi_ctx *c, double est, double L2_0, double *step_out, int32_t *ntrials) {
     if (!c || !step_out) return FWI_EINVAL;
@@ -1765,7 +2198,8 @@ int32_t fwi_step_length(fwi_ctx *c, double est, double L2_0, double *step_out, i
             sl[itest] = est;
             sl[0] = 0.0;
         }
-        if (step1 == 0 && L2t[0] <= L2t[1]) {
+        const bool lower = L2t[0] > L2t[1] && sl[1] >= kResolvedStep;   // the first trial's test
+        if (step1 == 0 && !lower) {
             est = est / scalefac;
             countstep++;
         }
@@ -1778,7 +2212,7 @@ int32_t fwi_step_length(fwi_ctx *c, double est, double L2_0, double *step_out, i
             est = est + est / scalefac;
             countstep++;
         }
-        if (step1 == 0 && L2t[0] > L2t[1]) {
+        if (step1 == 0 && lower) {
             sl[1] = est;
             step1 = 1;
             iteste = itests = 2;
@@ -1813,7 +2247,11 @@ int32_t fwi_iteration(fwi_ctx *c, double *L2_out, double *step_out) {
     RETIF(fwi_gradient(c, 0, c->nshot, m.data()));
     double L2 = 0;
     for (double v : m) L2 += v;
+    // the reference's order (src/cpu/n_simulate_PSV.cpp:431-455): conjugate direction, taper,
+    // smoothing -- the first and last disabled there, options here
+    if (c->o.pcg) RETIF(fwi_pcg_direction(c));
     RETIF(fwi_taper_gradient(c));
+    if (c->o.smooth_hfs) RETIF(fwi_smooth_gradient(c, c->o.smooth_hfs));
     double step = 0;
     RETIF(fwi_step_length(c, c->step_est, L2, &step, nullptr));
     RETIF(fwi_update_model(c, step, 0.5, 1.25));

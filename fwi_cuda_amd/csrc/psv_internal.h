@@ -175,6 +175,38 @@ void plain_step(const KArgs &a, const PlainBufs &p, int N, const int isurf[4], i
                 hipStream_t st);
 void plain_grad(const KArgs &a, hipStream_t st);
 
+// ---- exact discrete adjoint (psv_exact.hip) -------------------------------------------
+// One batch of B shots, every array [B][nz][P] unless noted.  The forward kernels keep per
+// step and cell e_zz, e_xx, e_zx (strain rates of the stress update) and S_z, S_x (stress
+// divergences of the velocity update) in hist; the reverse kernels apply the transposed
+// step and add the media gradient (lam, mu, mu_zx, rho_zp, rho_xp) into G in FP64.
+struct XArgs {
+    int nz, nx, P, B, N, pml_z, pml_x;
+    size_t fs;                             // nz * P
+    float dt;
+    float cz[6], cx[6];                    // hc[k] / dz, hc[k] / dx
+    const float *lam, *mu, *muzx, *rhoz, *rhox;   // [nz][P]
+    const CpmlPoint *zp, *xp;
+    float *vz, *vx, *szz, *szx, *sxx;      // forward state
+    float *pv[4], *ps[4];                  // forward CPML memories, stress / velocity pass (slots
+                                           // d_z vz|szz, d_z vx|szx, d_x vz|szx, d_x vx|sxx)
+    float *hist;                           // [nt][B][5][nz][P]
+    float *bvz, *bvx, *bzz, *bzx, *bxx;    // adjoint state
+    float *cv[4], *cs[4];                  // adjoint CPML carries, stress / velocity pass
+    float *E[4], *D[4], *Dv[4];            // scratch (may be one set): forward strain rates of a
+                                           // step; velocity-pass and stress-pass derivative adjoints
+    double *G;                             // [B][5][nz][P]
+};
+void x_forward_step(const XArgs &x, int n, const int isurf[4], hipStream_t st);
+// type0: the shots' sources when they are type 0 (their amplitude carries rho^-1), else null
+void x_adjoint_step(const XArgs &x, int n, const int isurf[4], const SrcArgs *type0, hipStream_t st);
+// A = rc x the reverse cumulative sum of dt x adj, per receiver row [rows][nt] (may be in place)
+void x_revcumsum(const double *adj_z, const double *adj_x, double *A_z, double *A_x, int rows, int nt, double dt,
+                 double rc, hipStream_t st);
+// G summed over nb shots and carried to lam, mu, rho on the box [z1, z2] x [x1, x2] (0 elsewhere)
+void x_media_chain(const XArgs &x, int nb, const double *mu, const double *rho, int z1, int z2, int x1, int x2,
+                   double *d_lam, double *d_mu, double *d_rho, hipStream_t st);
+
 // ---- small kernels on the model and gradient (psv_misc.hip) ------------------------
 // mat_av2: float lam, mu and averaged media [nz][P] from the FP64 model; the means of lam,
 // mu, rho over (nz - 1)(nx - 1) cells into means[3], summed in a fixed order from
@@ -192,7 +224,7 @@ void dev_residual(const double *seis_z, const double *seis_x, const double *obs_
 void dev_absmax(const double *x, size_t n, double *out, hipStream_t st);
 void dev_shot_absmax(const double *a, const double *b, const int *rec_off_local, int nshots, int nt, double *out,
                      hipStream_t st);
-void dev_precondition(const float *We, const float *Wa, double wa_scale, const float *const g[3],
+void dev_precondition(const float *We, const float *Wa, double wa_scale, const double *const g[3],
                       const double g_scale[3], double *const G[3], const double C[3], double *tmp,
                       double *wmax, int nz, int nx, int P, int z1, int z2, int x1, int x2, int sdz,
                       int sdx, int snz, int snx, hipStream_t st);
@@ -209,6 +241,22 @@ void dev_velocity_gradient(const double *G_lam, const double *G_mu, const double
 void dev_velocity_update(double *lam, double *mu, double *rho, const double *rho_old, const double *cp_old,
                          const double *cs_old, const double *g_cp, const double *g_cs, double f_cp, double f_cs,
                          const double bounds[4], size_t n, hipStream_t st);
+// misfit derivative: one shot's five FP64 totals added into the sums over shots raw[5][snz][snx],
+// divided by the shot's alpha (and the lam / mu terms by gain); then the staggered chain rule
+// to dJ/d(lam, mu, rho) [nz][nx] with sign sg
+void dev_acc_raw64(double *raw, const double *const g[5], size_t cells, const double *alpha, double gain,
+                   hipStream_t st);
+// the FP32 correlation sums g32[0..nk) ([n] active cells, stride cells in g64) into g64, zeroed
+void dev_flush_sums(double *g64, float *const g32[5], int nk, size_t n, size_t cells, hipStream_t st);
+void dev_flush_sums_at(double *g64, float *const g32[5], int nk, size_t n, size_t cells, const int *base, int dir,
+                       int k, int nt, int every, hipStream_t st);
+void dev_misfit_chain(const double *raw, const double *mu, double *d_lam, double *d_mu, double *d_rho, int nz,
+                      int nx, int z1, int x1, int snz, int snx, double sg, hipStream_t st);
+void dev_copy(double *dst, const double *src, size_t n, hipStream_t st);   // by a kernel
+// apply_gauss_filter of src into dst with the host-built filter [2 hfs + 1]^2 (device)
+void dev_gauss(const double *src, double *dst, const double *filt, int nz, int nx, int hfs, hipStream_t st);
+// PCG_PSV on one parameter: beta from sum g^2 / sum old^2 (scratch: 1 + 2 x 512 doubles)
+void dev_pcg(double *g, double *dir, double *old, size_t n, double *scratch, hipStream_t st);
 void dev_count_nonfinite(const float *x, size_t n, int *out, hipStream_t st);
 void dev_fill(float *p, float v, size_t n, hipStream_t st);
 

@@ -162,12 +162,12 @@ __global__ void k_energy_add(double *Wc, const double *wmax, int nx, int z1, int
 }
 
 // interpol_grad2 phase 1: snap points onto a zeroed full grid
-__global__ void k_interp_points(const float *g, double g_scale, double *T, int nx, int z1, int x1, int sdz, int sdx,
-                                int snz, int snx) {
+__global__ void k_interp_points(const double *g, double g_scale, double *T, int nx, int z1, int x1, int sdz,
+                                int sdx, int snz, int snx) {
     const int q = blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= snz * snx) return;
     const int jz = q / snx, jx = q - jz * snx;
-    T[(size_t)(z1 + jz * sdz) * nx + x1 + jx * sdx] = (double)g[q] * g_scale;
+    T[(size_t)(z1 + jz * sdz) * nx + x1 + jx * sdx] = g[q] * g_scale;
 }
 
 // phase 2: along x on snap rows iz < z2 (one thread per row segment)
@@ -268,6 +268,143 @@ __global__ void k_vel_update(double *lam, double *mu, double *rho, const double 
     }
 }
 
+// One shot's five FP64 correlation totals (s1, s1 + s2, dt vz'.Vz, s3, dt vx'.Vx on the
+// snapshot box) added into the sums over shots: raw[k] += g_k / (alpha * gain_k), the lam / mu
+// terms carrying the gain.
+__global__ void k_acc_raw64(double *raw, const double *g0, const double *g1, const double *g2, const double *g3,
+                            const double *g4, size_t cells, const double *alpha, double gain) {
+    const double *g[5] = {g0, g1, g2, g3, g4};
+    const double al = *alpha;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < cells; i += (size_t)gridDim.x * blockDim.x)
+        for (int k = 0; k < 5; ++k) {
+            const bool gained = k == 0 || k == 1 || k == 3;
+            raw[k * cells + i] += g[k][i] / (al * (gained ? gain : 1.0));
+        }
+}
+
+// FP32 correlation sums added into their FP64 totals and zeroed: g64[k][i] += g32[k][i]
+__global__ void k_flush_sums(double *g64, float *g0, float *g1, float *g2, float *g3, float *g4, int nk, size_t n,
+                             size_t cells) {
+    float *g[5] = {g0, g1, g2, g3, g4};
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        for (int k = 0; k < nk; ++k) {
+            g64[k * cells + i] += (double)g[k][i];
+            g[k][i] = 0.f;
+        }
+}
+
+// The same flush inside a replayed graph, taken only after the adjoint step whose time
+// index t = *base + dir * k leaves a multiple of `every` steps behind it ((nt - t) % every == 0)
+__global__ void k_flush_sums_at(double *g64, float *g0, float *g1, float *g2, float *g3, float *g4, int nk, size_t n,
+                                size_t cells, const int *base, int dir, int k, int nt, int every) {
+    const int t = *base + dir * k;
+    if ((nt - t) % every != 0) return;
+    float *g[5] = {g0, g1, g2, g3, g4};
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        for (int q = 0; q < nk; ++q) {
+            g64[q * cells + i] += (double)g[q][i];
+            g[q][i] = 0.f;
+        }
+}
+
+// The staggered chain rule from the five sums to dJ/d(lam, mu, rho) on [nz][nx] (zero outside
+// the box): mu_zx at (i, j) is the harmonic mean of mu on (i..i+1, j..j+1) (mat_av2,
+// src/cpu/n_kernel_lib_PSV.cpp:875-880), so the shear sum of each of the four mu_zx points a
+// cell enters, times mu_zx^2 / (4 mu^2) with the sum's own 1 / mu^2 undone; rho_zp and
+// rho_xp average rho over (i, i+1) and (j, j+1) (:886-887), each density gathering half of
+// its two points of each kind.
+__global__ void k_misfit_chain(const double *raw, const double *mu, double *d_lam, double *d_mu, double *d_rho,
+                               int nz, int nx, int z1, int x1, int snz, int snx, double sg) {
+    const size_t n = (size_t)nz * nx, cells = (size_t)snz * snx;
+    for (size_t o = (size_t)blockIdx.x * blockDim.x + threadIdx.x; o < n; o += (size_t)gridDim.x * blockDim.x) {
+        const int iz = (int)(o / nx), ix = (int)(o - (size_t)iz * nx);
+        const int jz = iz - z1, jx = ix - x1;
+        if (jz < 0 || jz >= snz || jx < 0 || jx >= snx) {
+            d_lam[o] = 0.0;
+            d_mu[o] = 0.0;
+            d_rho[o] = 0.0;
+            continue;
+        }
+        auto R = [&](int k, int qz, int qx) { return (qz < 0 || qx < 0) ? 0.0 : raw[k * cells + (size_t)qz * snx + qx]; };
+        d_lam[o] = sg * R(0, jz, jx);
+        double shear = 0.0;
+        for (int dz = 0; dz <= 1; ++dz)
+            for (int dx = 0; dx <= 1; ++dx) {
+                if (jz - dz < 0 || jx - dx < 0) continue;
+                const double mp = mu[(size_t)(iz - dz) * nx + (ix - dx)];
+                shear += R(3, jz - dz, jx - dx) * mp * mp;
+            }
+        const double mk = mu[o];
+        d_mu[o] = sg * (R(1, jz, jx) + (mk != 0.0 ? shear / (4.0 * mk * mk) : 0.0));
+        d_rho[o] = sg * 0.5 * (R(2, jz, jx) + R(2, jz - 1, jx) + R(4, jz, jx) + R(4, jz, jx - 1));
+    }
+}
+
+// a copy between device arrays that may come from the caller (no runtime copy engine involved)
+__global__ void k_copy_d(double *dst, const double *src, size_t n) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
+// apply_gauss_filter (src/cpu/n_kernel_lib_PSV.cpp:1051-1084): the (2 hfs + 1)^2 filter over
+// cells [hfs, nz - hfs - 1) x [hfs, nx - hfs - 1) of the copy src into dst, other cells copied;
+// accumulated in double (the reference's accumulator is an int)
+__global__ void k_gauss(const double *src, double *dst, const double *filt, int nz, int nx, int hfs) {
+    const size_t n = (size_t)nz * nx;
+    const int w = 2 * hfs + 1;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const int iz = (int)(i / nx), ix = (int)(i - (size_t)iz * nx);
+        if (iz < hfs || iz >= nz - hfs - 1 || ix < hfs || ix >= nx - hfs - 1) {
+            dst[i] = src[i];
+            continue;
+        }
+        double conv = 0.0;
+        for (int jj = 0; jj < w; ++jj)
+            for (int ii = 0; ii < w; ++ii)
+                conv += src[(size_t)(iz + jj - hfs) * nx + ix + ii - hfs] * filt[jj * w + ii];
+        dst[i] = conv;
+    }
+}
+
+// block partials of sum g^2 and sum old^2 (PCG_PSV's Fletcher-Reeves beta), then old = -g
+__global__ void k_pcg_sums(const double *g, double *old, size_t n, double *part) {
+    __shared__ double sh[kRB];
+    double a = 0, b = 0;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        a += g[i] * g[i];
+        b += old[i] * old[i];
+        old[i] = -g[i];
+    }
+    a = block_reduce(a, OpSum(), sh);
+    b = block_reduce(b, OpSum(), sh);
+    if (threadIdx.x == 0) {
+        part[2 * blockIdx.x] = a;
+        part[2 * blockIdx.x + 1] = b;
+    }
+}
+
+// beta = sum g^2 / sum old^2 from the partials, clipped at 0 (0 when old was 0: the first call)
+__global__ void k_pcg_beta(const double *part, int nparts, double *beta) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    double a = 0, b = 0;
+    for (int i = 0; i < nparts; ++i) {
+        a += part[2 * i];
+        b += part[2 * i + 1];
+    }
+    const double v = b > 0.0 ? a / b : 0.0;
+    *beta = v > 0 ? v : 0.0;
+}
+
+// dir = -g + beta dir; g = dir
+__global__ void k_pcg_dir(double *g, double *dir, const double *beta, size_t n) {
+    const double bt = *beta;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const double d = -g[i] + bt * dir[i];
+        dir[i] = d;
+        g[i] = d;
+    }
+}
+
 __global__ void k_nonfinite(const float *x, size_t n, int *out) {
     int c = 0;
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
@@ -322,7 +459,7 @@ void dev_shot_absmax(const double *a, const double *b, const int *rec_off_local,
     hipLaunchKernelGGL(k_shot_absmax, dim3(nshots), dim3(kRB), 0, st, a, b, rec_off_local, nt, out);
 }
 
-void dev_precondition(const float *We, const float *Wa, double wa_scale, const float *const g[3],
+void dev_precondition(const float *We, const float *Wa, double wa_scale, const double *const g[3],
                       const double g_scale[3], double *const G[3], const double C[3], double *tmp, double *wmax, int nz, int nx, int P,
                       int z1, int z2, int x1, int x2, int sdz, int sdx, int snz, int snx, hipStream_t st) {
     double *Wc = tmp, *T = tmp + (size_t)nz * nx;
@@ -374,6 +511,45 @@ void dev_velocity_update(double *lam, double *mu, double *rho, const double *rho
                          const double bounds[4], size_t n, hipStream_t st) {
     hipLaunchKernelGGL(k_vel_update, dim3(nblocks(n)), dim3(kRB), 0, st, lam, mu, rho, rho_old, cp_old, cs_old, g_cp,
                        g_cs, f_cp, f_cs, bounds[0], bounds[1], bounds[2], bounds[3], n);
+}
+
+void dev_acc_raw64(double *raw, const double *const g[5], size_t cells, const double *alpha, double gain,
+                   hipStream_t st) {
+    hipLaunchKernelGGL(k_acc_raw64, dim3(nblocks(cells)), dim3(kRB), 0, st, raw, g[0], g[1], g[2], g[3], g[4], cells,
+                       alpha, gain);
+}
+
+void dev_flush_sums(double *g64, float *const g32[5], int nk, size_t n, size_t cells, hipStream_t st) {
+    hipLaunchKernelGGL(k_flush_sums, dim3(nblocks(n)), dim3(kRB), 0, st, g64, g32[0], g32[1], g32[2], g32[3], g32[4],
+                       nk, n, cells);
+}
+
+void dev_flush_sums_at(double *g64, float *const g32[5], int nk, size_t n, size_t cells, const int *base, int dir,
+                       int k, int nt, int every, hipStream_t st) {
+    hipLaunchKernelGGL(k_flush_sums_at, dim3(nblocks(n)), dim3(kRB), 0, st, g64, g32[0], g32[1], g32[2], g32[3],
+                       g32[4], nk, n, cells, base, dir, k, nt, every);
+}
+
+void dev_misfit_chain(const double *raw, const double *mu, double *d_lam, double *d_mu, double *d_rho, int nz,
+                      int nx, int z1, int x1, int snz, int snx, double sg, hipStream_t st) {
+    hipLaunchKernelGGL(k_misfit_chain, dim3(nblocks((size_t)nz * nx)), dim3(kRB), 0, st, raw, mu, d_lam, d_mu,
+                       d_rho, nz, nx, z1, x1, snz, snx, sg);
+}
+
+void dev_copy(double *dst, const double *src, size_t n, hipStream_t st) {
+    hipLaunchKernelGGL(k_copy_d, dim3(nblocks(n)), dim3(kRB), 0, st, dst, src, n);
+}
+
+void dev_gauss(const double *src, double *dst, const double *filt, int nz, int nx, int hfs, hipStream_t st) {
+    hipLaunchKernelGGL(k_gauss, dim3(nblocks((size_t)nz * nx, 4096)), dim3(kRB), 0, st, src, dst, filt, nz, nx, hfs);
+}
+
+void dev_pcg(double *g, double *dir, double *old, size_t n, double *scratch, hipStream_t st) {
+    const int nb = nblocks(n, 512);
+    double *beta = scratch, *part = scratch + 1;
+    hipLaunchKernelGGL(k_pcg_sums, dim3(nb), dim3(kRB), 0, st, g, old, n, part);
+    hipLaunchKernelGGL(k_pcg_beta, dim3(1), dim3(1), 0, st, part, nb, beta);
+    hipLaunchKernelGGL(k_pcg_dir, dim3(nblocks(n)), dim3(kRB), 0, st, g, dir, beta, n);
 }
 
 void dev_count_nonfinite(const float *x, size_t n, int *out, hipStream_t st) {

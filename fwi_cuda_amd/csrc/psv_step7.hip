@@ -31,24 +31,23 @@
 //            181-205); surf_mirror's top row (:297-301)
 //   velocity sdiff2 + pml_diff2 + update_v2 (:207-270), fwi_grad2 (:439-484) on correlating
 //            steps, gard_fwd_storage2 (:398-436) on storing steps
+#include <algorithm>
+
 #include "psv_device.h"
 
 namespace psv {
 namespace {
 
-template <int N, int M, int MODE, bool GRAD, bool XW>
-__device__ __forceinline__ void step7_rows(const KArgs &a, const ZTab &zt, const StepView &sv) {
-    constexpr int R = 2 * N + 2, RS = N + 1, TZ = R * M, OUTW = kBX - 4 * N;
+// Rows [z0, z1) of shot s for the wavefront's column tile tx (z0 >= N, z1 <= nz - N).
+template <int N, int MODE, bool GRAD>
+__device__ __forceinline__ void step7_rows(const KArgs &a, const ZTab &zt, const StepView &sv, int s, int tx, int z0,
+                                           int z1) {
+    constexpr int R = 2 * N + 2, RS = N + 1, OUTW = kBX - 4 * N;
     constexpr bool adj = (MODE & kVelAdjoint) != 0, energy = (MODE & kVelEnergy) != 0;
-    int tbx, tby, s;
-    tile_of<TZ, OUTW, XW>(a, N, tbx, tby, s);
-    const int lane = threadIdx.x, wy = wave_uniform((int)threadIdx.y);
-    const int tx = XW ? tbx * kBY + wy : tbx;   // the wavefront's column tile
+    const int lane = threadIdx.x;
     const int x = tx * OUTW + lane - N;         // lane 2N holds the first owned column N + tx * OUTW
-    const int z0 = N + (XW ? tby : tby * kBY + wy) * TZ;
     const int nzi = a.nz - N;
-    if (z0 >= nzi || tx * OUTW >= a.nx - 2 * N) return;
-    const int z1 = min(z0 + TZ, nzi);
+    if (z0 >= z1 || tx * OUTW >= a.nx - 2 * N) return;
     const int cx = min(max(x, 0), a.nx - 1);   // lanes off the grid read a valid column
     // stresses are updated where the lane has its x taps and the column is interior;
     // lanes [2N, 64 - 2N) own their column
@@ -326,35 +325,108 @@ __device__ __forceinline__ void step7_rows(const KArgs &a, const ZTab &zt, const
         });
 }
 
-// kVelAdjoint launches branch once, on the step's correlation switch.  KArgs stays the
-// first kernel argument: the CPML blocks read its pointers back at segment offset 0.
-template <int N, int M, int MODE, bool XW>
-__global__ __launch_bounds__(kBX *kBY) void k_step7(const KArgs a, const ZTab zt) {
-    const StepView sv = step_view(a, 0);
+// One wavefront's rows, with the correlation switch of kVelAdjoint launches taken once.
+template <int N, int MODE>
+__device__ __forceinline__ void step7_run(const KArgs &a, const ZTab &zt, const StepView &sv, int s, int tx, int z0,
+                                          int z1) {
     if constexpr ((MODE & kVelAdjoint) != 0) {
         if (sv.grad) {
-            step7_rows<N, M, MODE, true, XW>(a, zt, sv);
+            step7_rows<N, MODE, true>(a, zt, sv, s, tx, z0, z1);
             return;
         }
     }
-    step7_rows<N, M, MODE, false, XW>(a, zt, sv);
+    step7_rows<N, MODE, false>(a, zt, sv, s, tx, z0, z1);
+}
+
+// Tile mode (rows == 0): a block is kBY segments of TZ = (2N + 2) M rows of one column tile,
+// or (XW) kBY adjacent column tiles of one segment.  Balanced mode (rows > 0, XW only): the
+// launch holds as many blocks as the GPU keeps resident, and block k takes rows
+// [k rows, (k + 1) rows) of the sequence of every interior row of every block column
+// (kBY tiles wide) of every shot, x fastest, then shots; so every block gets the same
+// number of rows, in at most two pieces, and no partly filled last wave of blocks is left.
+// KArgs stays the first kernel argument: the CPML blocks read its pointers back at segment
+// offset 0.
+template <int N, int M, int MODE, bool XW>
+__global__ __launch_bounds__(kBX *kBY) void k_step7(const KArgs a, const ZTab zt, const int rows) {
+    constexpr int TZ = (2 * N + 2) * M, OUTW = kBX - 4 * N;
+    const StepView sv = step_view(a, 0);
+    const int wy = wave_uniform((int)threadIdx.y);
+    if (XW && rows > 0) {
+        const int gx = ((a.nx - 2 * N + OUTW - 1) / OUTW + kBY - 1) / kBY;
+        const int nzi = a.nz - 2 * N;
+        const int n = gridDim.x, L = blockIdx.x, xcd = L & 7, loc = L >> 3, q = n >> 3, r = n & 7;
+        const long long W = xcd * q + min(xcd, r) + loc;   // each XCD a contiguous range of blocks
+        const long long total = (long long)a.B * gx * nzi;
+        long long r0 = W * rows;
+        const long long r1 = min(r0 + rows, total);
+        while (r0 < r1) {
+            const int u = (int)(r0 / nzi), zs = (int)(r0 - (long long)u * nzi);
+            const int ze = (int)min((long long)nzi, r1 - (long long)u * nzi);
+            const int s = u / gx, bx = u - s * gx;
+            step7_run<N, MODE>(a, zt, sv, s, bx * kBY + wy, N + zs, N + ze);
+            r0 = (long long)u * nzi + ze;
+        }
+        return;
+    }
+    int tbx, tby, s;
+    tile_of<TZ, OUTW, XW>(a, N, tbx, tby, s);
+    const int tx = XW ? tbx * kBY + wy : tbx;   // the wavefront's column tile
+    const int z0 = N + (XW ? tby : tby * kBY + wy) * TZ;
+    step7_run<N, MODE>(a, zt, sv, s, tx, z0, min(z0 + TZ, a.nz - N));
+}
+
+int compute_units() {
+    static int cus = 0;
+    if (!cus) {
+        int dev = 0, v = 0;
+        if (hipGetDevice(&dev) == hipSuccess &&
+            hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess)
+            cus = v;
+        if (cus <= 0) cus = 256;
+    }
+    return cus;
+}
+
+// Blocks of an instance the GPU keeps resident at once (balanced mode's launch size),
+// from its register and LDS use; asked once per instance.
+template <int N, int M, int MODE, bool XW>
+int resident_blocks() {
+    static int per_cu = 0;
+    if (!per_cu) {
+        int v = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&v, k_step7<N, M, MODE, XW>, kBX * kBY, 0) != hipSuccess ||
+            v <= 0)
+            v = 1;
+        per_cu = v;
+    }
+    return compute_units() * per_cu;
+}
+
+template <int N, int M, int MODE, bool XW>
+void step7_go(const KArgs &a, const ZTab &zt, dim3 grid, int mult, hipStream_t st) {
+    auto *k = k_step7<N, M, MODE, XW>;
+    int rows = 0;
+    if (XW && mult > 0) {   // balanced: mult x the resident blocks, the rows shared evenly
+        constexpr int OUTW = kBX - 4 * N;
+        const int gx = ((a.nx - 2 * N + OUTW - 1) / OUTW + kBY - 1) / kBY;
+        const long long total = (long long)a.B * gx * (a.nz - 2 * N);
+        const long long nblk = std::min<long long>((long long)mult * resident_blocks<N, M, MODE, XW>(), total);
+        rows = (int)((total + nblk - 1) / nblk);
+        grid = dim3((unsigned)((total + rows - 1) / rows));
+    }
+    hipLaunchKernelGGL(k, grid, dim3(kBX, kBY), 0, st, a, zt, rows);
 }
 
 template <int N, int M, bool XW>
-bool step7_m(const KArgs &a, const ZTab &zt, dim3 grid, dim3 block, hipStream_t st) {
+bool step7_m(const KArgs &a, const ZTab &zt, dim3 grid, int mult, hipStream_t st) {
     switch (vel_mode(a)) {
-        case 0: hipLaunchKernelGGL((k_step7<N, M, 0, XW>), grid, block, 0, st, a, zt); return true;
-        case kVelEnergy: hipLaunchKernelGGL((k_step7<N, M, kVelEnergy, XW>), grid, block, 0, st, a, zt); return true;
-        case kVelStore: hipLaunchKernelGGL((k_step7<N, M, kVelStore, XW>), grid, block, 0, st, a, zt); return true;
-        case kVelEnergy | kVelStore:
-            hipLaunchKernelGGL((k_step7<N, M, kVelEnergy | kVelStore, XW>), grid, block, 0, st, a, zt);
-            return true;
-        case kVelEnergy | kVelAdjoint:
-            hipLaunchKernelGGL((k_step7<N, M, kVelEnergy | kVelAdjoint, XW>), grid, block, 0, st, a, zt);
-            return true;
+        case 0: step7_go<N, M, 0, XW>(a, zt, grid, mult, st); return true;
+        case kVelEnergy: step7_go<N, M, kVelEnergy, XW>(a, zt, grid, mult, st); return true;
+        case kVelStore: step7_go<N, M, kVelStore, XW>(a, zt, grid, mult, st); return true;
+        case kVelEnergy | kVelStore: step7_go<N, M, kVelEnergy | kVelStore, XW>(a, zt, grid, mult, st); return true;
+        case kVelEnergy | kVelAdjoint: step7_go<N, M, kVelEnergy | kVelAdjoint, XW>(a, zt, grid, mult, st); return true;
         case kVelEnergy | kVelAdjoint | kVelMisfit:
-            hipLaunchKernelGGL((k_step7<N, M, kVelEnergy | kVelAdjoint | kVelMisfit, XW>), grid, block, 0, st, a,
-                               zt);
+            step7_go<N, M, kVelEnergy | kVelAdjoint | kVelMisfit, XW>(a, zt, grid, mult, st);
             return true;
         default: return false;
     }
@@ -363,27 +435,31 @@ bool step7_m(const KArgs &a, const ZTab &zt, dim3 grid, dim3 block, hipStream_t 
 // Segments of four or eight blocks: every segment recomputes 2N halo rows of stresses,
 // so one block (2N + 2 rows) alone would spend almost half its stress work on halos.
 template <int N>
-bool step7_n(const KArgs &a, const ZTab &zt, int M, bool xw, dim3 grid, dim3 block, hipStream_t st) {
-    if (xw) return M == 8 ? step7_m<N, 8, true>(a, zt, grid, block, st) : step7_m<N, 4, true>(a, zt, grid, block, st);
-    return M == 8 ? step7_m<N, 8, false>(a, zt, grid, block, st) : step7_m<N, 4, false>(a, zt, grid, block, st);
+bool step7_n(const KArgs &a, const ZTab &zt, int M, bool xw, dim3 grid, int mult, hipStream_t st) {
+    if (xw)
+        return M == 8 ? step7_m<N, 8, true>(a, zt, grid, mult, st) : step7_m<N, 4, true>(a, zt, grid, mult, st);
+    return M == 8 ? step7_m<N, 8, false>(a, zt, grid, mult, st) : step7_m<N, 4, false>(a, zt, grid, mult, st);
 }
 
 }  // namespace
 
 bool launch_step7(const KArgs &a, const ZTab &zt, int N, int tz, bool xw, hipStream_t st) {
+    // tz < 0: balanced mode (variant 8's layout) with -tz times the resident blocks
+    const int mult = tz < 0 ? -tz : 0;
+    if (mult) xw = true;
     const int R = 2 * N + 2, M = tz >= 8 * R ? 8 : 4, TZ = R * M;
     const int nxi = a.nx - 2 * N, nzi = a.nz - 2 * N, outw = kBX - 4 * N;
     const int segs = (nzi + TZ - 1) / TZ, tiles = (nxi + outw - 1) / outw;
     // grid as tile_of<TZ, OUTW, XW> counts it
     const int nblk = xw ? ((tiles + kBY - 1) / kBY) * segs : tiles * ((segs + kBY - 1) / kBY);
-    const dim3 grid(nblk * a.B), block(kBX, kBY);
+    const dim3 grid(nblk * a.B);
     switch (N) {
-        case 1: return step7_n<1>(a, zt, M, xw, grid, block, st);
-        case 2: return step7_n<2>(a, zt, M, xw, grid, block, st);
-        case 3: return step7_n<3>(a, zt, M, xw, grid, block, st);
-        case 4: return step7_n<4>(a, zt, M, xw, grid, block, st);
-        case 5: return step7_n<5>(a, zt, M, xw, grid, block, st);
-        default: return step7_n<6>(a, zt, M, xw, grid, block, st);
+        case 1: return step7_n<1>(a, zt, M, xw, grid, mult, st);
+        case 2: return step7_n<2>(a, zt, M, xw, grid, mult, st);
+        case 3: return step7_n<3>(a, zt, M, xw, grid, mult, st);
+        case 4: return step7_n<4>(a, zt, M, xw, grid, mult, st);
+        case 5: return step7_n<5>(a, zt, M, xw, grid, mult, st);
+        default: return step7_n<6>(a, zt, M, xw, grid, mult, st);
     }
 }
 

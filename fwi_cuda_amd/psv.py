@@ -151,7 +151,8 @@ class Simulation:
 
     def __init__(self, case: dict, *, batch=0, storage=_lib.STORE_AUTO, ckpt_interval=0, mem_budget=0,
                  form=_lib.FORM_FUSED, grad_sign=-1, tile_tz=0, device=0, check_finite=1, kernel_variant=0,
-                 work_order=0, use_graph=0, param=_lib.PARAM_LAME, vel_bounds=None):
+                 work_order=0, use_graph=0, param=_lib.PARAM_LAME, vel_bounds=None, ckpt_slots=0, pcg=0,
+                 smooth_hfs=0):
         lib = _lib.load()
         self.lib = lib
         self.case = case
@@ -187,6 +188,8 @@ class Simulation:
         o.kernel_variant, o.work_order = int(kernel_variant), int(work_order)
         o.use_graph = int(use_graph)
         o.param = int(param)
+        o.ckpt_slots = int(ckpt_slots)
+        o.pcg, o.smooth_hfs = int(pcg), int(smooth_hfs)
         if vel_bounds is not None:
             o.vel_bounds[:] = [float(v) for v in vel_bounds]
         h = C.c_void_p()
@@ -225,13 +228,21 @@ class Simulation:
         b, s, k = C.c_int32(), C.c_int32(), C.c_int32()
         n = C.c_int64()
         self._chk(self.lib.fwi_plan(self.ctx, C.byref(b), C.byref(s), C.byref(k), C.byref(n)))
+        u, sl, rf = C.c_int32(), C.c_int32(), C.c_double()
+        self._chk(self.lib.fwi_checkpoint_plan(self.ctx, C.byref(u), C.byref(sl), C.byref(rf)))
         return dict(batch=b.value, storage={1: "full", 2: "checkpoint"}.get(s.value, s.value),
-                    ckpt_interval=k.value, device_bytes=n.value)
+                    ckpt_interval=k.value, device_bytes=n.value, ckpt_units=u.value, ckpt_slots=sl.value,
+                    recompute=round(rf.value, 4))
 
     # -- model (mat_av2) --------------------------------------------------------
     def set_model(self, lam, mu, rho):
         lam, mu, rho = (np.ascontiguousarray(a, np.float64) for a in (lam, mu, rho))
         self._chk(self.lib.fwi_set_model(self.ctx, dptr(lam), dptr(mu), dptr(rho)))
+
+    def set_model_device(self, lam_ptr: int, mu_ptr: int, rho_ptr: int):
+        """Model from device arrays [nz][nx] FP64 (raw HIP pointers, e.g. tensor.data_ptr())."""
+        self._chk(self.lib.fwi_set_model_device(self.ctx, C.c_void_p(lam_ptr), C.c_void_p(mu_ptr),
+                                                C.c_void_p(rho_ptr)))
 
     def get_model(self):
         out = [np.zeros((self.nz, self.nx)) for _ in range(3)]
@@ -262,6 +273,12 @@ class Simulation:
         uz, ux = np.zeros((n, self.nt)), np.zeros((n, self.nt))
         self._chk(self.lib.fwi_forward(self.ctx, shot0, nshots, dptr(uz), dptr(ux)))
         return uz, ux
+
+    def forward_device(self, seis_z_ptr: int, seis_x_ptr: int, shot0=0, nshots=None):
+        """forward() into device arrays [rows][nt] FP64 (raw HIP pointers)."""
+        nshots = self.nshot - shot0 if nshots is None else nshots
+        self._chk(self.lib.fwi_forward_device(self.ctx, shot0, nshots, C.c_void_p(seis_z_ptr),
+                                              C.c_void_p(seis_x_ptr)))
 
     def split(self, stacked, shot0=0):
         """Stacked CSR rows -> list of per-shot [nrec][nt] arrays."""
@@ -307,6 +324,39 @@ class Simulation:
         self._chk(self.lib.fwi_misfit_gradient(self.ctx, shot0, nshots, dptr(adj_z), dptr(adj_x),
                                                *[dptr(a) for a in out]))
         return tuple(out)
+
+    def misfit_gradient_exact(self, adj_z, adj_x, shot0=0, nshots=None):
+        """The exact derivative of J = sum 0.5 dt r^2 for adjoint sources r (the discrete
+        adjoint of the time step), dJ/d(lam, mu, rho) [nz][nx]."""
+        nshots = self.nshot - shot0 if nshots is None else nshots
+        adj_z, adj_x = (np.ascontiguousarray(a, np.float64) for a in (adj_z, adj_x))
+        out = [np.zeros((self.nz, self.nx)) for _ in range(3)]
+        self._chk(self.lib.fwi_misfit_gradient_exact(self.ctx, shot0, nshots, adj_z.ctypes.data, adj_x.ctypes.data,
+                                                     *[a.ctypes.data for a in out], 0))
+        return tuple(out)
+
+    def misfit_gradient_exact_device(self, adj_z_ptr: int, adj_x_ptr: int, d_lam_ptr: int, d_mu_ptr: int,
+                                     d_rho_ptr: int, shot0=0, nshots=None):
+        """misfit_gradient_exact() on device arrays (raw HIP pointers)."""
+        nshots = self.nshot - shot0 if nshots is None else nshots
+        self._chk(self.lib.fwi_misfit_gradient_exact(self.ctx, shot0, nshots, *[C.c_void_p(p) for p in (
+            adj_z_ptr, adj_x_ptr, d_lam_ptr, d_mu_ptr, d_rho_ptr)], 1))
+
+    def PCG_PSV(self):
+        """The conjugate-gradient direction in place of the global gradient (PCG_PSV)."""
+        self._chk(self.lib.fwi_pcg_direction(self.ctx))
+
+    def apply_gauss_filter(self, hfs):
+        """Gaussian smoothing of the global gradient (apply_gauss_filter, FP64 accumulator)."""
+        self._chk(self.lib.fwi_smooth_gradient(self.ctx, int(hfs)))
+
+    def misfit_gradient_device(self, adj_z_ptr: int, adj_x_ptr: int, d_lam_ptr: int, d_mu_ptr: int, d_rho_ptr: int,
+                               shot0=0, nshots=None):
+        """misfit_gradient() on device arrays (raw HIP pointers): adjoint sources [rows][nt]
+        in, dJ/d(lam, mu, rho) [nz][nx] out, all FP64."""
+        nshots = self.nshot - shot0 if nshots is None else nshots
+        self._chk(self.lib.fwi_misfit_gradient_device(self.ctx, shot0, nshots, *[C.c_void_p(p) for p in (
+            adj_z_ptr, adj_x_ptr, d_lam_ptr, d_mu_ptr, d_rho_ptr)]))
 
     def get_gradient(self):
         out = [np.zeros((self.nz, self.nx)) for _ in range(3)]

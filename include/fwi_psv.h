@@ -52,7 +52,7 @@
 extern "C" {
 #endif
 
-#define FWI_PSV_ABI_VERSION 2
+#define FWI_PSV_ABI_VERSION 3
 
 enum fwi_status {
     FWI_OK = 0,
@@ -157,6 +157,14 @@ typedef struct {
     double vel_bounds[4];   /* FWI_PARAM_VELOCITY: cp_min, cp_max, cs_min, cs_max in m/s, the
                                clip bounds and update scales of update_mat2 (all 0: 1000, 6000,
                                500, 3500) */
+    int32_t pcg;            /* fwi_iteration: replace the global gradient by its conjugate-gradient
+                               direction (fwi_pcg_direction) before the taper */
+    int32_t smooth_hfs;     /* fwi_iteration: Gaussian smoothing of half width smooth_hfs (>= 2;
+                               0 = none) after the taper (fwi_smooth_gradient) */
+    int32_t ckpt_slots;     /* CHECKPOINT storage: at most this many checkpoint buffers (0 = as many
+                               as the budget allows).  With fewer than the ceil(nt / ckpt_interval) - 1
+                               segment boundaries, the reversal recomputes segments from held
+                               checkpoints on a binomial (revolve-style) schedule */
 } fwi_opts;
 
 /* Per-kernel timing collected while profiling is on (HIP events on the
@@ -202,6 +210,11 @@ void fwi_destroy(fwi_ctx *ctx);
 int32_t fwi_plan(const fwi_ctx *ctx, int32_t *batch, int32_t *storage, int32_t *ckpt_interval,
                  int64_t *device_bytes);
 
+/* Checkpoint schedule of CHECKPOINT storage: units = segments of ckpt_interval steps,
+ * slots = checkpoint buffers, recompute = forward steps run again per time step of the
+ * adjoint (1 when every segment boundary is kept: each segment recomputed once). */
+int32_t fwi_checkpoint_plan(const fwi_ctx *ctx, int32_t *units, int32_t *slots, double *recompute);
+
 /* Model lam, mu, rho [nz][nx]; derives mu_zx, rho_zp, rho_xp and the scalar means
  * (mat_av2) on the device. */
 int32_t fwi_set_model(fwi_ctx *ctx, const double *lam, const double *mu, const double *rho);
@@ -237,14 +250,50 @@ int32_t fwi_gradient_raw(fwi_ctx *ctx, int32_t shot0, int32_t nshots, const doub
  * single reference function: it is fwi_grad2's correlation
  * (src/cpu/n_kernel_lib_PSV.cpp:439-484) with the density term taken as the adjoint
  * velocity times the forward velocity, and the shear and density terms carried
- * through mat_av2's staggered averages (:853-922) by the chain rule, so that it
- * matches finite differences of J.  Needs snap_dz = snap_dx = 1 and the fused form. */
+ * through mat_av2's staggered averages (:853-922) by the chain rule on the device.
+ * Needs snap_dz = snap_dx = 1 and the fused form. */
 int32_t fwi_misfit_gradient(fwi_ctx *ctx, int32_t shot0, int32_t nshots, const double *adj_z,
                             const double *adj_x, double *d_lam, double *d_mu, double *d_rho);
+
+/* The exact derivative of the same J (no reference counterpart: the reference's gradient is
+ * the correlation of fwi_grad2): the discrete adjoint of the time step -- every operation of
+ * the forward step (stencils, CPML recursions, free surface, sources, the receivers'
+ * running sum) transposed and applied in reverse order, the media gradient taken from the
+ * forward's stored strain rates and stress divergences and carried to lam, mu, rho through
+ * mat_av2's averages.  Zero outside the snapshot box.  It keeps five fields per time step
+ * and shot (FP32) for the call, in batches of shots that fit; on_device: adj_* and the
+ * outputs are device arrays (as in the _device calls below). */
+int32_t fwi_misfit_gradient_exact(fwi_ctx *ctx, int32_t shot0, int32_t nshots, const double *adj_z,
+                                  const double *adj_x, double *d_lam, double *d_mu, double *d_rho,
+                                  int32_t on_device);
+
+/* The same three calls with device arrays (HIP pointers of the runtime the library runs
+ * on, e.g. torch tensors in a process that loaded torch first): model lam, mu, rho
+ * [nz][nx] in, seismograms [rows][nt] out, adjoint sources in and the derivative out, all
+ * FP64.  Their contents are read and written by kernels on the context's stream; each call
+ * returns when its outputs are complete, and the caller makes its own writes to the inputs
+ * complete before the call (e.g. a stream synchronisation).  The reference's entry points
+ * take host arrays only (include/cpu/n_kernel_PSV.hpp:8-42). */
+int32_t fwi_set_model_device(fwi_ctx *ctx, const double *lam, const double *mu, const double *rho);
+int32_t fwi_forward_device(fwi_ctx *ctx, int32_t shot0, int32_t nshots, double *seis_z, double *seis_x);
+int32_t fwi_misfit_gradient_device(fwi_ctx *ctx, int32_t shot0, int32_t nshots, const double *adj_z,
+                                   const double *adj_x, double *d_lam, double *d_mu, double *d_rho);
 int32_t fwi_zero_gradient(fwi_ctx *ctx);
 int32_t fwi_get_gradient(fwi_ctx *ctx, double *g_lam, double *g_mu, double *g_rho);
 int32_t fwi_set_gradient(fwi_ctx *ctx, const double *g_lam, const double *g_mu, const double *g_rho);
 int32_t fwi_taper_gradient(fwi_ctx *ctx);
+/* PCG_PSV on the three global gradients (src/cpu/n_PCG_PSV.cpp:13-44, disabled in the
+ * reference's loop, src/cpu/n_simulate_PSV.cpp:431-435): Fletcher-Reeves beta = sum g^2 /
+ * sum old^2 clipped at 0, dir = -g + beta dir, old = -g, g = dir; the context keeps dir and
+ * old between calls.  The first call (old = 0) takes beta = 0 where the reference divides
+ * by zero. */
+int32_t fwi_pcg_direction(fwi_ctx *ctx);
+/* apply_gauss_filter with gauss_filter_kernel on the three global gradients
+ * (src/cpu/n_kernel_lib_PSV.cpp:1012-1084, disabled in the reference's loop): a
+ * (2 hfs + 1)^2 Gaussian with sigma = hfs / 2 (integer division) over the cells
+ * [hfs, n - hfs - 1) of each axis, accumulated in FP64 (the reference's accumulator is an
+ * int, which truncates every partial sum).  hfs >= 2 (hfs = 1 makes sigma 0). */
+int32_t fwi_smooth_gradient(fwi_ctx *ctx, int32_t hfs);
 
 /* Sum of the shots' L2 misfits for the current model (forward only). */
 int32_t fwi_misfit(fwi_ctx *ctx, int32_t shot0, int32_t nshots, double *L2);
@@ -253,6 +302,10 @@ int32_t fwi_save_model(fwi_ctx *ctx);
  * FWI_PARAM_VELOCITY: cp, cs with eps (f from the cp / cs gradient and vel_bounds),
  * rho = rho_old; rho_factor and rho_min are not used. */
 int32_t fwi_update_model(fwi_ctx *ctx, double eps, double rho_factor, double rho_min);
+/* step_length_PSV's bracketing from est (halve until a trial lowers the misfit, then grow)
+ * and calc_opt_step's parabola vertex (src/cpu/n_step_length_PSV.cpp:13-209).  A trial
+ * step below 2^-20 (a model change the FP32 media do not resolve) never counts as lowering
+ * the misfit.  FWI_ESTEP where the reference prints "Steplength estimation failed!". */
 int32_t fwi_step_length(fwi_ctx *ctx, double est, double L2_0, double *step_out, int32_t *ntrials);
 /* One simulate_fwi_PSV iteration over all shots; returns its misfit and step. */
 int32_t fwi_iteration(fwi_ctx *ctx, double *L2, double *step);

@@ -1,9 +1,11 @@
 """Checkpoint-budget sweep of one gradient, through the C ABI.
 
-    python measure/ckpt_sweep.py --config C4 --budgets 240,120,60,30
+    python measure/ckpt_sweep.py --config C4 --budgets 240,120,60,30 [--slots 0,8,3]
 
-For each device-memory budget (GB) the planner picks the batch of shots and
-the checkpoint interval that fit; the same gradient (observed data modelled
+For each device-memory budget (GB), and each cap on the checkpoint buffers
+(--slots; 0 = one per segment boundary, else the binomial reversal over that
+many held states), the planner picks the batch of shots and the checkpoint
+interval that fit; the same gradient (observed data modelled
 from the true model, start model = the true model smoothed) is timed and
 checked against the first budget's result, which it must equal bit for bit
 (checkpointed recomputation replays the forward exactly).  One line per
@@ -27,6 +29,8 @@ def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
     ap.add_argument("--config", default="C4", choices=tuple(configs.BUILDERS))
     ap.add_argument("--budgets", default="240,120,60,30", help="device-memory budgets in GB")
+    ap.add_argument("--slots", default="0", help="checkpoint-buffer caps (0 = one per segment)")
+    ap.add_argument("--batch", type=int, default=0, help="shots per launch (0 = the planner's)")
     ap.add_argument("--nt", type=int, default=0)
     ap.add_argument("--nshot", type=int, default=0)
     ap.add_argument("--json", default="")
@@ -49,11 +53,13 @@ def main(argv=None):
     print(f"observed data: {time.perf_counter() - t0:.1f} s", flush=True)
 
     rows, ref = [], None
-    for gb in (float(v) for v in args.budgets.split(",")):
+    runs = [(float(b), int(s)) for b in args.budgets.split(",") for s in args.slots.split(",")]
+    for gb, slots in runs:
         try:
-            sim = psv.Simulation(c, storage=_lib.STORE_CHECKPOINT, mem_budget=int(gb * 1e9))
+            sim = psv.Simulation(c, storage=_lib.STORE_CHECKPOINT, mem_budget=int(gb * 1e9), ckpt_slots=slots,
+                                 **({"batch": args.batch} if args.batch else {}))
         except _lib.FwiError as e:   # the budget holds less than one shot's arrays
-            row = {"budget_GB": gb, "refused": str(e)}
+            row = {"budget_GB": gb, "slots_cap": slots, "refused": str(e)}
             rows.append(row)
             print(json.dumps(row), flush=True)
             continue
@@ -71,7 +77,7 @@ def main(argv=None):
             ref = (m, G)
         else:
             same = bool(np.array_equal(m, ref[0]) and all(np.array_equal(a, b) for a, b in zip(G, ref[1])))
-        row = {"budget_GB": gb, **plan, "seconds": round(wall, 2), "Gcell_per_s": round(useful / wall / 1e9, 3),
+        row = {"budget_GB": gb, "slots_cap": slots, **plan, "seconds": round(wall, 2), "Gcell_per_s": round(useful / wall / 1e9, 3),
                "identical_to_first": same}
         rows.append(row)
         print(json.dumps(row), flush=True)

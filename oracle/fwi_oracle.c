@@ -723,3 +723,339 @@ double or_calc_opt_step(const double L2[3], const double sl[3])
     if (L2[1] < L2[0] && L2[2] < L2[1]) opteps = sl[2];
     return opteps;
 }
+
+/* ========================================================================== */
+/* Exact discrete adjoint (reverse mode through or_step)                        */
+/* ========================================================================== */
+
+/* Scatter of the transposed staggered derivatives.  vdiff forms, per interior
+ * cell, dz_z = sum_k cz_k (vz[i+k-1] - vz[i-k]), dx_z = sum cz_k (vx[i+k] - vx[i-k+1]),
+ * dz_x = sum cx_k (vz[j+k] - vz[j-k+1]), dx_x = sum cx_k (vx[j+k-1] - vx[j-k]); the
+ * transpose adds each coefficient times the derivative's adjoint back onto the cells
+ * it read.  sdiff (second function) uses the mirrored offsets. */
+static void vdiff_T(const or_grid *g, const double *D[4], double *bvz, double *bvx)
+{
+    const int nx = g->nx, N = g->fdorder / 2;
+    const int nz1 = N, nz2 = g->nz - N, nx1 = N, nx2 = g->nx - N;
+    const double dxi = 1.0 / g->dx, dzi = 1.0 / g->dz;
+    for (int iz = nz1; iz < nz2; iz++)
+        for (int ix = nx1; ix < nx2; ix++) {
+            const double a = AT(D[0], iz, ix), b = AT(D[1], iz, ix), c = AT(D[2], iz, ix), d = AT(D[3], iz, ix);
+            for (int k = 1; k <= N; k++) {
+                const double cz = dzi * g->hc[k], cx = dxi * g->hc[k];
+                AT(bvz, iz + k - 1, ix) += cz * a;
+                AT(bvz, iz - k, ix) -= cz * a;
+                AT(bvx, iz + k, ix) += cz * b;
+                AT(bvx, iz - k + 1, ix) -= cz * b;
+                AT(bvz, iz, ix + k) += cx * c;
+                AT(bvz, iz, ix - k + 1) -= cx * c;
+                AT(bvx, iz, ix + k - 1) += cx * d;
+                AT(bvx, iz, ix - k) -= cx * d;
+            }
+        }
+}
+
+static void sdiff_T(const or_grid *g, const double *D[4], double *bzz, double *bzx, double *bxx)
+{
+    const int nx = g->nx, N = g->fdorder / 2;
+    const int nz1 = N, nz2 = g->nz - N, nx1 = N, nx2 = g->nx - N;
+    const double dxi = 1.0 / g->dx, dzi = 1.0 / g->dz;
+    for (int iz = nz1; iz < nz2; iz++)
+        for (int ix = nx1; ix < nx2; ix++) {
+            const double a = AT(D[0], iz, ix), b = AT(D[1], iz, ix), c = AT(D[2], iz, ix), d = AT(D[3], iz, ix);
+            for (int k = 1; k <= N; k++) {
+                const double cz = dzi * g->hc[k], cx = dxi * g->hc[k];
+                AT(bzz, iz + k, ix) += cz * a;
+                AT(bzz, iz - k + 1, ix) -= cz * a;
+                AT(bzx, iz + k - 1, ix) += cz * b;
+                AT(bzx, iz - k, ix) -= cz * b;
+                AT(bzx, iz, ix + k - 1) += cx * c;
+                AT(bzx, iz, ix - k) -= cx * c;
+                AT(bxx, iz, ix + k) += cx * d;
+                AT(bxx, iz, ix - k + 1) -= cx * d;
+            }
+        }
+}
+
+/* pml_diff reversed: the forward psi' = b psi + a d, d~ = d / K + psi' over the
+ * interior (slots 0, 1 with the z profiles (integer, half), 2, 3 with the x profiles
+ * (half, integer)).  In reverse, P = carry + adj(d~), adj(d) = adj(d~) / K + a P and the
+ * carry passed to the previous step is b P. */
+static void pml_T(const or_grid *g, double *D[4], double *carry[4])
+{
+    const int nx = g->nx, N = g->fdorder / 2;
+    const int nz1 = N, nz2 = g->nz - N, nx1 = N, nx2 = g->nx - N;
+    for (int iz = nz1; iz < nz2; iz++)
+        for (int ix = nx1; ix < nx2; ix++) {
+            if (g->pml_z) {
+                double P = AT(carry[0], iz, ix) + AT(D[0], iz, ix);
+                AT(D[0], iz, ix) = AT(D[0], iz, ix) / g->K_z[iz] + g->a_z[iz] * P;
+                AT(carry[0], iz, ix) = g->b_z[iz] * P;
+                P = AT(carry[1], iz, ix) + AT(D[1], iz, ix);
+                AT(D[1], iz, ix) = AT(D[1], iz, ix) / g->K_half_z[iz] + g->a_half_z[iz] * P;
+                AT(carry[1], iz, ix) = g->b_half_z[iz] * P;
+            }
+            if (g->pml_x) {
+                double P = AT(carry[3], iz, ix) + AT(D[3], iz, ix);
+                AT(D[3], iz, ix) = AT(D[3], iz, ix) / g->K_x[ix] + g->a_x[ix] * P;
+                AT(carry[3], iz, ix) = g->b_x[ix] * P;
+                P = AT(carry[2], iz, ix) + AT(D[2], iz, ix);
+                AT(D[2], iz, ix) = AT(D[2], iz, ix) / g->K_half_x[ix] + g->a_half_x[ix] * P;
+                AT(carry[2], iz, ix) = g->b_half_x[ix] * P;
+            }
+        }
+}
+
+/* surf_mirror reversed, sides in reverse order (right, left, bottom, top); per line the
+ * images are undone, then the assignments: the set stress s = 4 dt d (lam mu + mu^2) /
+ * (lam + 2 mu) sends its adjoint to the tangential strain d (bD) and to lam, mu. */
+static void surf_T(const or_grid *g, const or_media *m, const double *Ev[4], double *bzz, double *bzx,
+                   double *bxx, double *bD[4], double *gl, double *gm)
+{
+    const int nx = g->nx, N = g->fdorder / 2;
+    const int nz1 = N, nz2 = g->nz - N, nx1 = N, nx2 = g->nx - N;
+    const double dt = g->dt;
+    for (int side = 3; side >= 0; side--) {
+        const int is = g->isurf[side];
+        if (is <= 0) continue;
+        const int n_lines = side < 2 ? nx2 - nx1 : nz2 - nz1;
+        for (int l = 0; l < n_lines; l++) {
+            int iz = side < 2 ? is : nz1 + l, ix = side < 2 ? nx1 + l : is;
+            if (side == 0)
+                for (int k = 1; k < is - nz1 + 1; k++) {
+                    AT(bzx, is + k, ix) -= AT(bzx, is - k, ix); AT(bzx, is - k, ix) = 0.0;
+                    AT(bzz, is + k, ix) -= AT(bzz, is - k, ix); AT(bzz, is - k, ix) = 0.0;
+                }
+            else if (side == 1)
+                for (int k = 1; k <= nz2 - is; k++) {
+                    AT(bzx, is - k, ix) -= AT(bzx, is + k, ix); AT(bzx, is + k, ix) = 0.0;
+                    AT(bzz, is - k, ix) -= AT(bzz, is + k, ix); AT(bzz, is + k, ix) = 0.0;
+                }
+            else if (side == 2)
+                for (int k = 1; k < is - nx1 + 1; k++) {
+                    AT(bzx, iz, is + k) -= AT(bzx, iz, is - k); AT(bzx, iz, is - k) = 0.0;
+                    AT(bxx, iz, is + k) -= AT(bxx, iz, is - k); AT(bxx, iz, is - k) = 0.0;
+                }
+            else
+                for (int k = 1; k <= nx2 - is; k++) {
+                    AT(bzx, iz, is - k) -= AT(bzx, iz, is + k); AT(bzx, iz, is + k) = 0.0;
+                    AT(bxx, iz, is - k) -= AT(bxx, iz, is + k); AT(bxx, iz, is + k) = 0.0;
+                }
+            const double l_ = AT(m->lam, iz, ix), u = AT(m->mu, iz, ix), den = l_ + 2.0 * u;
+            const double f = (l_ * u + u * u) / den;
+            const double f_l = u * u / (den * den), f_u = (l_ * l_ + 2.0 * l_ * u + 2.0 * u * u) / (den * den);
+            /* top / bottom set sxx from dx_x (slot 3); left / right set szz from dz_z (slot 0) */
+            double *bs = side < 2 ? bxx : bzz;
+            const int slot = side < 2 ? 3 : 0;
+            const double a = AT(bs, iz, ix), d = AT(Ev[slot], iz, ix);
+            AT(bD[slot], iz, ix) += a * 4.0 * dt * f;
+            AT(gl, iz, ix) += a * 4.0 * dt * d * f_l;
+            AT(gm, iz, ix) += a * 4.0 * dt * d * f_u;
+            AT(bzz, iz, ix) = 0.0;
+            AT(bzx, iz, ix) = 0.0;
+            AT(bxx, iz, ix) = 0.0;
+        }
+    }
+}
+
+void or_adjoint_exact(const or_grid *g, const or_media *m, int32_t ishot, const or_sources *src,
+                      int32_t nrec, const int32_t *z_rec, const int32_t *x_rec, const double *cot_uz,
+                      const double *cot_ux, double *g_lam, double *g_mu, double *g_muzx, double *g_rhoz,
+                      double *g_rhox)
+{
+    const int nz = g->nz, nx = g->nx, nt = g->nt, N = g->fdorder / 2;
+    const int nz1 = N, nz2 = nz - N, nx1 = N, nx2 = nx - N;
+    const size_t n = (size_t)nz * nx, fb = sizeof(double) * n;
+    const double dt = g->dt, rc = dt / (g->dz * g->dx);
+    /* forward state at the start of every step: vz vx szz szx sxx and the 8 memories */
+    double *hist = malloc(fb * 13 * (size_t)nt);
+    double *buf = calloc(n * 20, sizeof(double));
+    or_state s = {buf, buf + n, buf + 2 * n, buf + 3 * n, buf + 4 * n, buf + 5 * n, buf + 6 * n, buf + 7 * n,
+                  buf + 8 * n, buf + 9 * n, buf + 10 * n, buf + 11 * n,
+                  buf + 12 * n, buf + 13 * n, buf + 14 * n, buf + 15 * n,
+                  buf + 16 * n, buf + 17 * n, buf + 18 * n, buf + 19 * n};
+    double **keep[13] = {&s.vz, &s.vx, &s.szz, &s.szx, &s.sxx, &s.mem_vz_z, &s.mem_vx_z, &s.mem_szz_z,
+                         &s.mem_szx_z, &s.mem_vz_x, &s.mem_vx_x, &s.mem_szx_x, &s.mem_sxx_x};
+    or_reset_state(g, &s);
+    memset(s.mem_vz_z, 0, fb * 8);
+    for (int it = 0; it < nt; it++) {
+        for (int k = 0; k < 13; k++) memcpy(hist + ((size_t)it * 13 + k) * n, *keep[k], fb);
+        or_step(g, m, &s, it, ishot, src, NULL, 0, 0, NULL, 0, NULL, NULL, NULL);
+    }
+    /* adjoint state and per-step intermediates */
+    double *abuf = calloc(n * 25, sizeof(double));
+    double *bvz = abuf, *bvx = abuf + n, *bzz = abuf + 2 * n, *bzx = abuf + 3 * n, *bxx = abuf + 4 * n;
+    double *cv[4] = {abuf + 5 * n, abuf + 6 * n, abuf + 7 * n, abuf + 8 * n};     /* stress-pass carries */
+    double *cs[4] = {abuf + 9 * n, abuf + 10 * n, abuf + 11 * n, abuf + 12 * n};  /* velocity-pass carries */
+    double *Ev[4] = {abuf + 13 * n, abuf + 14 * n, abuf + 15 * n, abuf + 16 * n}; /* forward strains */
+    double *Sz = abuf + 17 * n, *Sx = abuf + 18 * n;
+    double *bD[4] = {abuf + 19 * n, abuf + 20 * n, abuf + 21 * n, abuf + 22 * n};
+    double *acc = calloc(2 * (size_t)(nrec > 0 ? nrec : 1), sizeof(double));
+    memset(g_lam, 0, fb); memset(g_mu, 0, fb); memset(g_muzx, 0, fb); memset(g_rhoz, 0, fb); memset(g_rhox, 0, fb);
+    for (int it = nt - 1; it >= 0; it--) {
+        /* replay step it up to its intermediates */
+        for (int k = 0; k < 13; k++) memcpy(*keep[k], hist + ((size_t)it * 13 + k) * n, fb);
+        or_vdiff(g, &s);
+        if (g->pml_z || g->pml_x) pml_diff(g, &s, s.mem_vz_z, s.mem_vx_z, s.mem_vz_x, s.mem_vx_x);
+        memcpy(Ev[0], s.dz_z, fb); memcpy(Ev[1], s.dx_z, fb); memcpy(Ev[2], s.dz_x, fb); memcpy(Ev[3], s.dx_x, fb);
+        update_s(g, m, &s);
+        if (g->surf) surf_mirror(g, m, &s);
+        or_sdiff(g, &s);
+        if (g->pml_z || g->pml_x) pml_diff(g, &s, s.mem_szz_z, s.mem_szx_z, s.mem_szx_x, s.mem_sxx_x);
+        for (size_t i = 0; i < n; i++) {
+            Sz[i] = s.dz_x[i] + s.dz_z[i];
+            Sx[i] = s.dx_x[i] + s.dx_z[i];
+        }
+        /* urec: d[it] = d[it-1] + rc v; the running sum's adjoint carries sum_{t >= it} cot[t] */
+        for (int ir = 0; ir < nrec; ir++) {
+            acc[2 * ir] += cot_uz[(size_t)ir * nt + it];
+            acc[2 * ir + 1] += cot_ux[(size_t)ir * nt + it];
+            AT(bvz, z_rec[ir], x_rec[ir]) += rc * acc[2 * ir];
+            AT(bvx, z_rec[ir], x_rec[ir]) += rc * acc[2 * ir + 1];
+        }
+        /* vsrc: type 0 sources scale by the inverse densities of their cells */
+        if (src && src->stf_type == 0)
+            for (int is = 0; is < src->nsrc; is++) {
+                if (src->shot_to_fire[is] != ishot) continue;
+                const int zs = src->z_src[is], xs = src->x_src[is];
+                AT(g_rhoz, zs, xs) += AT(bvz, zs, xs) * dt * src->stf_z[(size_t)is * nt + it] / (g->dz * g->dx);
+                AT(g_rhox, zs, xs) += AT(bvx, zs, xs) * dt * src->stf_x[(size_t)is * nt + it] / (g->dz * g->dx);
+            }
+        /* update_v: v += dt rho^-1 S */
+        memset(bD[0], 0, fb * 4);
+        for (int iz = nz1; iz < nz2; iz++)
+            for (int ix = nx1; ix < nx2; ix++) {
+                const double az = AT(bvz, iz, ix), ax = AT(bvx, iz, ix);
+                AT(g_rhoz, iz, ix) += az * dt * AT(Sz, iz, ix);
+                AT(g_rhox, iz, ix) += ax * dt * AT(Sx, iz, ix);
+                const double sz = dt * AT(m->rho_zp, iz, ix) * az, sx = dt * AT(m->rho_xp, iz, ix) * ax;
+                AT(bD[0], iz, ix) = sz;   /* szz_z */
+                AT(bD[2], iz, ix) = sz;   /* szx_x */
+                AT(bD[1], iz, ix) = sx;   /* szx_z */
+                AT(bD[3], iz, ix) = sx;   /* sxx_x */
+            }
+        if (g->pml_z || g->pml_x) pml_T(g, bD, cs);
+        sdiff_T(g, (const double **)bD, bzz, bzx, bxx);
+        /* surf_mirror, then update_s */
+        memset(bD[0], 0, fb * 4);
+        if (g->surf) surf_T(g, m, (const double **)Ev, bzz, bzx, bxx, bD, g_lam, g_mu);
+        for (int iz = nz1; iz < nz2; iz++)
+            for (int ix = nx1; ix < nx2; ix++) {
+                const double zz = AT(bzz, iz, ix), zx = AT(bzx, iz, ix), xx = AT(bxx, iz, ix);
+                const double lam = AT(m->lam, iz, ix), mu = AT(m->mu, iz, ix), mzx = AT(m->mu_zx, iz, ix);
+                const double e_zz = AT(Ev[0], iz, ix), e_xx = AT(Ev[3], iz, ix), e_zx = AT(Ev[2], iz, ix) + AT(Ev[1], iz, ix);
+                AT(g_lam, iz, ix) += dt * (zz + xx) * (e_xx + e_zz);
+                AT(g_mu, iz, ix) += dt * 2.0 * (zz * e_zz + xx * e_xx);
+                AT(g_muzx, iz, ix) += dt * zx * e_zx;
+                AT(bD[0], iz, ix) += dt * ((lam + 2.0 * mu) * zz + lam * xx);
+                AT(bD[3], iz, ix) += dt * (lam * zz + (lam + 2.0 * mu) * xx);
+                AT(bD[1], iz, ix) += dt * mzx * zx;
+                AT(bD[2], iz, ix) += dt * mzx * zx;
+            }
+        if (g->pml_z || g->pml_x) pml_T(g, bD, cv);
+        vdiff_T(g, (const double **)bD, bvz, bvx);
+        /* variables outside the interior are constants of the forward: no adjoint */
+        for (int iz = 0; iz < nz; iz++)
+            for (int ix = 0; ix < nx; ix++)
+                if (iz < nz1 || iz >= nz2 || ix < nx1 || ix >= nx2) {
+                    AT(bvz, iz, ix) = AT(bvx, iz, ix) = 0.0;
+                    AT(bzz, iz, ix) = AT(bzx, iz, ix) = AT(bxx, iz, ix) = 0.0;
+                }
+    }
+    free(acc);
+    free(abuf);
+    free(buf);
+    free(hist);
+}
+
+/* The media gradient carried to lam, mu, rho through mat_av2 (src/cpu/n_kernel_lib_PSV.cpp:
+ * 853-922): mu_zx[i][j] = 4 / sum of 1/mu over (i..i+1, j..j+1), so d mu_zx / d mu = mu_zx^2 /
+ * (4 mu^2); rho_zp[i][j] = 1 / (0.5 (rho[i][j] + rho[i+1][j])), so d rho_zp / d rho =
+ * -0.5 rho_zp^2 for both, and rho_xp over (j, j+1) likewise; the last row and column of the
+ * averaged media are not defined (0). */
+void or_media_chain(const double *g_lam, const double *g_mu, const double *g_muzx, const double *g_rhoz,
+                    const double *g_rhox, const double *mu, const double *mu_zx, const double *rho_zp,
+                    const double *rho_xp, int32_t nz, int32_t nx, double *d_lam, double *d_mu, double *d_rho)
+{
+    for (int iz = 0; iz < nz; iz++)
+        for (int ix = 0; ix < nx; ix++) {
+            AT(d_lam, iz, ix) = AT(g_lam, iz, ix);
+            double dm = AT(g_mu, iz, ix), dr = 0.0;
+            const double u = AT(mu, iz, ix);
+            for (int a = 0; a <= 1; a++)
+                for (int b = 0; b <= 1; b++) {
+                    const int pz = iz - a, px = ix - b;
+                    if (pz < 0 || px < 0 || pz >= nz - 1 || px >= nx - 1 || u == 0.0) continue;
+                    const double q = AT(mu_zx, pz, px);
+                    dm += AT(g_muzx, pz, px) * q * q / (4.0 * u * u);
+                }
+            for (int a = 0; a <= 1; a++) {
+                const int pz = iz - a;
+                if (pz >= 0 && pz < nz - 1 && ix < nx - 1) dr -= 0.5 * AT(g_rhoz, pz, ix) * AT(rho_zp, pz, ix) * AT(rho_zp, pz, ix);
+                const int px = ix - a;
+                if (px >= 0 && px < nx - 1 && iz < nz - 1) dr -= 0.5 * AT(g_rhox, iz, px) * AT(rho_xp, iz, px) * AT(rho_xp, iz, px);
+            }
+            AT(d_mu, iz, ix) = dm;
+            AT(d_rho, iz, ix) = dr;
+        }
+}
+
+/* ========================================================================== */
+/* Conjugate-gradient direction and Gaussian gradient smoothing (disabled in    */
+/* the reference's loop, src/cpu/n_simulate_PSV.cpp:431-455)                    */
+/* ========================================================================== */
+
+/* PCG_PSV (src/cpu/n_PCG_PSV.cpp:13-44): Fletcher-Reeves beta = sum g^2 / sum old^2 with
+ * old = -g of the previous call, clipped at 0; dir = -g + beta dir; old = -g; g = dir.  The
+ * first call has old = 0, where the reference divides by zero; here beta is 0 then. */
+void or_pcg_psv(double *dir, double *old, double *grad, int32_t nz, int32_t nx)
+{
+    const size_t n = (size_t)nz * nx;
+    double bi = 0.0, bj = 0.0;
+    for (size_t i = 0; i < n; i++) {
+        bi += grad[i] * grad[i];
+        bj += old[i] * old[i];
+        old[i] = -grad[i];
+    }
+    double beta = bj > 0.0 ? bi / bj : 0.0;
+    beta = beta > 0 ? beta : 0.0;
+    for (size_t i = 0; i < n; i++) {
+        dir[i] = -grad[i] + beta * dir[i];
+        grad[i] = dir[i];
+    }
+}
+
+/* gauss_filter_kernel (src/cpu/n_kernel_lib_PSV.cpp:1012-1049): sigma = hfs / 2 in integer
+ * division, exp(-(i^2 + j^2) / (2 sigma^2)) over [-hfs, hfs]^2, normalised to sum 1. */
+void or_gauss_filter_kernel(double *kernel, int32_t hfs)
+{
+    const int w = 2 * hfs + 1;
+    const double sigma = hfs / 2, q = 2.0 * sigma * sigma;
+    double sum = 0.0;
+    for (int ii = -hfs; ii <= hfs; ii++)
+        for (int jj = -hfs; jj <= hfs; jj++) {
+            const double v = exp(-((ii * ii) / q) - ((jj * jj) / q));
+            kernel[(jj + hfs) * w + ii + hfs] = v;
+            sum += v;
+        }
+    for (int i = 0; i < w * w; i++) kernel[i] /= sum;
+}
+
+/* apply_gauss_filter (:1051-1084): the filter over cells [hfs, nz - hfs - 1) x [hfs, nx - hfs - 1)
+ * of a copy, other cells kept.  The reference accumulates in an int (every partial sum
+ * truncated); the restatement accumulates in double, as its `conv = 0.0` start intends. */
+void or_apply_gauss_filter(double *mat, const double *filter, int32_t nz, int32_t nx, int32_t hfs)
+{
+    const int w = 2 * hfs + 1;
+    double *tmp = malloc(sizeof(double) * (size_t)nz * nx);
+    memcpy(tmp, mat, sizeof(double) * (size_t)nz * nx);
+    for (int iz = hfs; iz < nz - hfs - 1; iz++)
+        for (int ix = hfs; ix < nx - hfs - 1; ix++) {
+            double conv = 0.0;
+            for (int jj = 0; jj < w; jj++)
+                for (int ii = 0; ii < w; ii++)
+                    conv += tmp[(size_t)(iz + jj - hfs) * nx + ix + ii - hfs] * filter[jj * w + ii];
+            mat[(size_t)iz * nx + ix] = conv;
+        }
+    free(tmp);
+}

@@ -118,6 +118,25 @@ void or_scale_grad_E2(double *grad, const double *grad_shot, double mat_av, cons
 void or_taper2(double *A, int32_t nz, int32_t nx, int32_t snap_z1, int32_t snap_z2, int32_t snap_x1,
                int32_t snap_x2, const int32_t taper[8]);
 
+/* ---- exact discrete adjoint --------------------------------------------------------- */
+/* Reverse mode through or_step for one shot: cot_uz / cot_ux [nrec][nt] are the
+ * cotangents of the receivers' displacement seismograms (dL/d seismogram); out the
+ * gradient of L with respect to the media lam, mu, mu_zx, rho_zp, rho_xp [nz][nx].  The
+ * forward states of every step are kept (5 fields and 8 memories per step). */
+void or_adjoint_exact(const or_grid *g, const or_media *m, int32_t ishot, const or_sources *src,
+                      int32_t nrec, const int32_t *z_rec, const int32_t *x_rec, const double *cot_uz,
+                      const double *cot_ux, double *g_lam, double *g_mu, double *g_muzx, double *g_rhoz,
+                      double *g_rhox);
+/* The media gradient carried to lam, mu, rho through mat_av2's averages. */
+void or_media_chain(const double *g_lam, const double *g_mu, const double *g_muzx, const double *g_rhoz,
+                    const double *g_rhox, const double *mu, const double *mu_zx, const double *rho_zp,
+                    const double *rho_xp, int32_t nz, int32_t nx, double *d_lam, double *d_mu, double *d_rho);
+
+/* ---- conjugate-gradient direction, Gaussian smoothing --------------------------------- */
+void or_pcg_psv(double *dir, double *old, double *grad, int32_t nz, int32_t nx);
+void or_gauss_filter_kernel(double *kernel, int32_t hfs);
+void or_apply_gauss_filter(double *mat, const double *filter, int32_t nz, int32_t nx, int32_t hfs);
+
 /* ---- step length --------------------------------------------------------------- */
 double or_calc_opt_step(const double L2[3], const double sl[3]);
 

@@ -291,6 +291,69 @@ def kernel_psv(c, media, ishot, *, grad=False, accu=True, state=None, snaps=None
     return out
 
 
+def adjoint_exact(c, media, ishot, cot_uz, cot_ux):
+    """The exact discrete adjoint of kernel_psv's forward for one shot: the gradient of a
+    loss L with respect to the media (lam, mu, mu_zx, rho_zp, rho_xp) given the cotangents
+    dL/d(seismogram) of the receivers' displacement seismograms [nrec][nt] (reverse mode
+    through every operation of the time step, FP64, the forward states of every step kept)."""
+    lib = load("oracle")
+    keep = _Keep()
+    g = _grid(c, keep)
+    m = OrMedia(keep.d(media["lam"]), keep.d(media["mu"]), keep.d(media["mu_zx"]),
+                keep.d(media["rho_zp"]), keep.d(media["rho_xp"]))
+    src = OrSources(len(c["z_src"]), int(c["stf_type"]), keep.d(c["stf_z"]), keep.d(c["stf_x"]),
+                    keep.i(c["z_src"]), keep.i(c["x_src"]), keep.i(c["shot_to_fire"]))
+    nz, nx = int(c["nz"]), int(c["nx"])
+    out = [np.zeros((nz, nx)) for _ in range(5)]
+    zr, xr = np.ascontiguousarray(c["z_rec"], np.int32), np.ascontiguousarray(c["x_rec"], np.int32)
+    cz = np.ascontiguousarray(cot_uz, np.float64)
+    cx = np.ascontiguousarray(cot_ux, np.float64)
+    lib.or_adjoint_exact(C.byref(g), C.byref(m), C.c_int32(ishot), C.byref(src), C.c_int32(len(zr)), _i(zr), _i(xr),
+                         _d(cz), _d(cx), *[_d(a) for a in out])
+    return dict(zip(("lam", "mu", "mu_zx", "rho_zp", "rho_xp"), out))
+
+
+def media_chain(grads, media):
+    """The media gradient carried to (lam, mu, rho) through mat_av2's averages."""
+    lib = load("oracle")
+    nz, nx = media["mu"].shape
+    out = [np.zeros((nz, nx)) for _ in range(3)]
+    g = [np.ascontiguousarray(grads[k], np.float64) for k in ("lam", "mu", "mu_zx", "rho_zp", "rho_xp")]
+    lib.or_media_chain(*[_d(a) for a in g], _d(media["mu"]), _d(media["mu_zx"]), _d(media["rho_zp"]),
+                       _d(media["rho_xp"]), C.c_int32(nz), C.c_int32(nx), *[_d(a) for a in out])
+    return tuple(out)
+
+
+def pcg_psv(dir_, old, grad, which="oracle"):
+    """PCG_PSV (src/cpu/n_PCG_PSV.cpp:13-44) on one parameter, in place on all three arrays."""
+    lib = load(which)
+    nz, nx = grad.shape
+    fn = lib.or_pcg_psv if which == "oracle" else lib.ref_pcg_psv
+    fn(_d(dir_), _d(old), _d(grad), C.c_int32(nz), C.c_int32(nx))
+    return dir_, old, grad
+
+
+def gauss_filter_kernel(hfs, which="oracle"):
+    """gauss_filter_kernel (src/cpu/n_kernel_lib_PSV.cpp:1012-1049): [2 hfs + 1]^2."""
+    lib = load(which)
+    k = np.zeros((2 * hfs + 1, 2 * hfs + 1))
+    (lib.or_gauss_filter_kernel if which == "oracle" else lib.ref_gauss_filter_kernel)(_d(k), C.c_int32(hfs))
+    return k
+
+
+def apply_gauss_filter(mat, hfs, which="oracle"):
+    """apply_gauss_filter (:1051-1084) in place; which="ref" runs the reference as written."""
+    lib = load(which)
+    nz, nx = mat.shape
+    filt = gauss_filter_kernel(hfs, which=which)
+    if which == "oracle":
+        lib.or_apply_gauss_filter(_d(mat), _d(filt), C.c_int32(nz), C.c_int32(nx), C.c_int32(hfs))
+    else:
+        tmp = np.zeros_like(mat)
+        lib.ref_apply_gauss_filter(_d(mat), _d(tmp), _d(filt), C.c_int32(nz), C.c_int32(nx), C.c_int32(hfs))
+    return mat
+
+
 def adjsrc2(mod_uz, mod_ux, true_uz, true_ux, dt, which="oracle"):
     """adjsrc2: residuals (returned) and the L2 misfit."""
     lib = load(which)

@@ -14,6 +14,7 @@
 #include "n_kernel_PSV.hpp"
 #include "n_kernel_lib_PSV.hpp"
 #include "n_step_length_PSV.hpp"
+#include "n_PCG_PSV.hpp"
 #include "h_preproc_PSV.hpp"
 #include "h_holbergcoeff.hpp"
 
@@ -189,6 +190,24 @@ double ref_step_length_psv(double est, double L2_0, int nshot, const or_grid *g,
                            z_src, x_src, fire, rec->nrec, rec->rtf_type, rtf_uz.ptr, rtf_ux.ptr, z_rec, x_rec,
                            tuz.ptr, tux.ptr, true, avz.ptr, avx.ptr, aszz.ptr, aszx.ptr, asxx.ptr, g->snap_z1,
                            g->snap_z2, g->snap_x1, g->snap_x2, g->snap_dt, g->snap_dz, g->snap_dx, 0);
+}
+
+// PCG_PSV (src/cpu/n_PCG_PSV.cpp:13-44) on one parameter's arrays [nz][nx]
+void ref_pcg_psv(double *dir, double *old, double *grad, int nz, int nx) {
+    Rows d(dir, nz, nx), o(old, nz, nx), g(grad, nz, nx);
+    PCG_PSV(d.ptr, o.ptr, g.ptr, nz, nx);
+}
+
+// gauss_filter_kernel (src/cpu/n_kernel_lib_PSV.cpp:1012-1049): [2 hfs + 1][2 hfs + 1]
+void ref_gauss_filter_kernel(double *kernel, int hfs) {
+    Rows k(kernel, 2 * hfs + 1, 2 * hfs + 1);
+    gauss_filter_kernel(k.ptr, hfs);
+}
+
+// apply_gauss_filter (:1051-1084) as written (its accumulator is an int)
+void ref_apply_gauss_filter(double *mat, double *mat_temp, double *filter, int nz, int nx, int hfs) {
+    Rows m(mat, nz, nx), t(mat_temp, nz, nx), f(filter, 2 * hfs + 1, 2 * hfs + 1);
+    apply_gauss_filter(m.ptr, t.ptr, f.ptr, nz, nx, hfs);
 }
 
 }  // extern "C"

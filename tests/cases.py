@@ -64,6 +64,37 @@ def small_case(O, *, nz=72, nx=96, nt=400, fdorder=8, nshot=3, surface=True, see
     return c
 
 
+def adjoint_case(O, fdorder, surf, stf_type, nz=40, nx=52, nt=160, nshot=1, uneven=False):
+    """A small case for the exact adjoint: a layered model with an inclusion, CPML or a free
+    surface ("none", "top", or "four" sides with image rows and columns), shots 9 cells
+    apart, five receivers (shot 1 of an uneven pair keeps three of them)."""
+    N = fdorder // 2
+    lam, mu, rho = layered_model(nz, nx, 11)
+    hc = O.holberg(fdorder)
+    npml = 8
+    dt = 0.5 * O.dtstab(lam, mu, rho, hc, 1.0, npml)
+    if surf == "none":
+        isurf, pml = (0, 0, 0, 0), (npml, npml, npml, npml)
+    elif surf == "top":
+        isurf, pml = (N, 0, 0, 0), (0, npml, npml, npml)
+    else:   # all four sides, with image rows / columns
+        isurf, pml = (N + 2, nz - N - 3, N + 1, nx - N - 2), (0, 0, 0, 0)
+    top = max(isurf[0], pml[0], N)
+    zr = [top + 1, top + 2, nz // 2, top + 1, nz - 12]
+    xr = [12, 20, 30, 40, 26]
+    extra = {}
+    if nshot > 1 and uneven:
+        zr, xr = zr + zr[:3], xr + xr[:3]
+        extra["rec_off"] = np.array([0, 5, 8], np.int32)
+    c = O.make_case(nt=nt, nz=nz, nx=nx, dt=dt, dz=1.0, dx=1.0, fdorder=fdorder, lam=lam, mu=mu, rho=rho,
+                    z_src=[top + 5 + s for s in range(nshot)], x_src=[nx // 2 - 3 + 9 * s for s in range(nshot)],
+                    shot_to_fire=list(range(nshot)), fc=350.0, amp_z=1.0, amp_x=0.6,
+                    stf_type=stf_type, npml=pml, isurf=isurf, z_rec=zr, x_rec=xr,
+                    snap=(top, nz - 10, 10, nx - 10, 1, 1, 1))
+    c.update(extra)
+    return c
+
+
 def start_model(c, seed=7):
     """The background without the inclusion (for gradient tests)."""
     return layered_model(int(c["nz"]), int(c["nx"]), seed, inclusion=False)
@@ -82,6 +113,30 @@ def per_shot_obs(c, O, media):
         f = O.kernel_psv(cs, media, s, accu=False)
         out.append((f["uz"], f["ux"]))
     return out
+
+
+def matched_obs(c, O, media0, syn, obs):
+    """Observed data for the oracle that gives it the GPU's residual.
+
+    The adjoint source is d_syn - d_obs.  Where d_obs is FP64 and the residual is a
+    small part of the data, the FP32 error of d_syn (rel ~1e-6) is a large part of
+    the residual, and no FP32 forward can do better.  With
+    d_obs' = d_syn_oracle + d_obs - d_syn_gpu the oracle's residual equals the
+    GPU's, so the gradients compare the adjoint pass and the correlation alone.
+    syn: per-shot (uz, ux) of the GPU at the start model media0.
+    """
+    out = []
+    for s in range(int(c["nshot"])):
+        f = O.kernel_psv(shot_case(c, s), media0, s, accu=False)
+        out.append((f["uz"] + obs[s][0] - syn[s][0], f["ux"] + obs[s][1] - syn[s][1]))
+    return out
+
+
+def residual_fraction(syn, obs):
+    """|d_syn - d_obs| / |d_obs| over every shot."""
+    r = sum(np.sum((a - b) ** 2) for p, q in zip(syn, obs) for a, b in zip(p, q))
+    d = sum(np.sum(b ** 2) for q in obs for b in q)
+    return float(np.sqrt(r / d))
 
 
 def shot_case(c, s):

@@ -85,7 +85,7 @@ def test_shot_gradient_matches_reference_golden(psv, name):
 
 
 @pytest.mark.parametrize("surface", [True, False])
-@pytest.mark.parametrize("variant", [1, 2, 4, 5, 6, 7, 8])
+@pytest.mark.parametrize("variant", [1, 2, 4, 5, 6, 7, 8, 9])
 def test_order8_forward_vs_oracle(psv, O, surface, variant):
     c = cases.small_case(O, surface=surface, nshot=2)
     med = O.mat_av2(c)
@@ -100,7 +100,7 @@ def test_order8_forward_vs_oracle(psv, O, surface, variant):
         assert e < 1e-4
 
 
-@pytest.mark.parametrize("variant", [1, 2, 4, 5, 6, 7, 8])
+@pytest.mark.parametrize("variant", [1, 2, 4, 5, 6, 7, 8, 9])
 def test_order8_gradient_vs_oracle(psv, O, variant):
     c = cases.small_case(O, nshot=2, nt=450)
     med_true = O.mat_av2(c)
@@ -126,7 +126,7 @@ def test_order8_gradient_vs_oracle(psv, O, variant):
             assert e < 2e-3
 
 
-@pytest.mark.parametrize("variant", [1, 4, 5, 6, 7, 8])
+@pytest.mark.parametrize("variant", [1, 4, 5, 6, 7, 8, 9])
 def test_checkpointed_equals_full_storage(psv, O, variant):
     c = cases.small_case(O, nshot=2, nt=300, snap_dt=2)
     med_true = O.mat_av2(c)
@@ -146,6 +146,40 @@ def test_checkpointed_equals_full_storage(psv, O, variant):
         np.testing.assert_array_equal(m, ref[0])
         for i in range(3):
             np.testing.assert_array_equal(G[i], ref[1][i], err_msg=str(key))
+
+
+@pytest.mark.parametrize("variant,graph", [(1, 0), (7, 0), (8, 1), (5, 1), (9, 1)])
+def test_binomial_checkpoints_equal_full_storage(psv, O, variant, graph):
+    """Fewer checkpoint buffers than segment boundaries (ckpt_slots 1, 2, 5 for 22
+    segments of 14 steps, the last one short): the reversal advances from held
+    checkpoints on the binomial schedule and recomputes more (plan's recompute > 1),
+    and the misfits and gradient still equal full snapshot storage bit for bit
+    (the same kernels replay the same steps), with and without graph replay."""
+    c = cases.small_case(O, nshot=3, nt=302, snap_dt=2, uneven=True)
+    med_true = O.mat_av2(c)
+    obs = cases.per_shot_obs(c, O, med_true)
+    lam0, mu0, rho0 = cases.start_model(c)
+    res = {}
+    for storage, slots in (("full", 0), ("ckpt", 0), ("ckpt", 5), ("ckpt", 2), ("ckpt", 1)):
+        st = psv._lib.STORE_FULL if storage == "full" else psv._lib.STORE_CHECKPOINT
+        with psv.Simulation(c, storage=st, ckpt_interval=14, ckpt_slots=slots, kernel_variant=variant,
+                            use_graph=graph, batch=2) as sim:
+            plan = sim.plan()
+            if storage == "ckpt":
+                assert plan["ckpt_units"] == 22
+                assert plan["ckpt_slots"] == (21 if slots == 0 else slots)
+                assert (plan["recompute"] > 1.0) == (slots > 0)
+            _print(f"variant {variant} graph {graph} {storage} slots {slots}: {plan}")
+            sim.set_model(lam0, mu0, rho0)
+            sim.set_observed([o[0] for o in obs], [o[1] for o in obs])
+            m = sim.gradient()
+            res[(storage, slots)] = (m, sim.get_gradient(), plan["recompute"])
+    ref = res[("full", 0)]
+    for key, (m, G, _) in res.items():
+        np.testing.assert_array_equal(m, ref[0])
+        for i in range(3):
+            np.testing.assert_array_equal(G[i], ref[1][i], err_msg=str(key))
+    assert res[("ckpt", 1)][2] > res[("ckpt", 2)][2] > res[("ckpt", 5)][2] > res[("ckpt", 0)][2] == 1.0
 
 
 def test_c4_window_checkpointed_equals_full_storage(psv, O):
@@ -178,7 +212,7 @@ def test_c4_window_checkpointed_equals_full_storage(psv, O):
             np.testing.assert_array_equal(G[i], ref[1][i], err_msg=str(key))
 
 
-@pytest.mark.parametrize("variant", [1, 4, 5, 6, 7, 8])
+@pytest.mark.parametrize("variant", [1, 4, 5, 6, 7, 8, 9])
 def test_batched_uneven_equals_per_shot(psv, O, variant):
     c = cases.small_case(O, nshot=4, nt=260, uneven=True)
     med_true = O.mat_av2(c)
@@ -395,7 +429,7 @@ def test_work_orders_agree_bitwise(psv, O):
             np.testing.assert_array_equal(ux, ref[2])
 
 
-@pytest.mark.parametrize("variant", [1, 4, 5, 6, 7, 8])
+@pytest.mark.parametrize("variant", [1, 4, 5, 6, 7, 8, 9])
 @pytest.mark.parametrize("storage", ["full", "ckpt"])
 def test_graph_replay_equals_eager(psv, O, variant, storage):
     """use_graph: chunks of time steps captured once per propagation and replayed
@@ -424,7 +458,7 @@ def test_graph_replay_equals_eager(psv, O, variant, storage):
         np.testing.assert_array_equal(out[1][3][i], out[0][3][i])
 
 
-@pytest.mark.parametrize("variant", [4, 7, 8])
+@pytest.mark.parametrize("variant", [4, 7, 8, 9])
 def test_one_pass_step_refused_where_it_does_not_apply(psv, O, variant):
     c = cases.small_case(O, nshot=1, nt=20)
     c["isurf"] = np.array([int(c["fdorder"]) // 2, int(c["nz"]) - 20, 0, 0], np.int32)   # a bottom surface too

@@ -69,8 +69,9 @@ def test_default_options(lib):
     lib.fwi_default_opts(C.byref(o))
     assert (o.storage, o.form, o.grad_sign, o.check_finite, o.kernel_variant, o.work_order) == \
         (_lib.STORE_AUTO, _lib.FORM_FUSED, -1, 1, 0, 0)
-    assert lib.fwi_abi_version() == _lib.ABI_VERSION == 2
-    assert o.param == _lib.PARAM_LAME and list(o.vel_bounds) == [0.0] * 4
+    assert lib.fwi_abi_version() == _lib.ABI_VERSION == 3
+    assert (o.pcg, o.smooth_hfs) == (0, 0)
+    assert o.param == _lib.PARAM_LAME and list(o.vel_bounds) == [0.0] * 4 and o.ckpt_slots == 0
     assert lib.fwi_strerror(-8)
 
 
@@ -101,7 +102,21 @@ def test_stability_limit_closed_form():
     lam[5, 7] = rho[5, 7] * 4000.0 ** 2 - 2 * mu[5, 7]    # the fastest cell sets the limit
     hc = psv.holbergcoeff(8)
     dts = psv.checkfd_dtstab(lam, mu, rho, hc, 1.0, 0)
-    assert dts == pytest.approx(1.0 / (np.sqrt(2.0) * np.abs(hc[1:5]).sum() * 4000.0), rel=1e-12)
+    want = 1.0 / (np.sqrt(2.0) * np.abs(hc[1:5]).sum() * 4000.0)
+    assert dts == pytest.approx(want, rel=1e-12)
+    # the oracle's restatement: the same closed form; cells in the skipped first row and
+    # column, and inside the fw-cell margin, do not set the limit
+    import sys
+    sys.path.insert(0, str(ROOT))
+    from oracle import oracle as O
+    O.load("oracle")
+    assert O.dtstab(lam, mu, rho, hc, 1.0, 0) == pytest.approx(want, rel=1e-12)
+    lam2 = lam.copy()
+    lam2[0, 3] = rho[0, 3] * 5000.0 ** 2 - 2 * mu[0, 3]     # row 0 is outside the scan
+    lam2[8, 28] = rho[8, 28] * 6000.0 ** 2 - 2 * mu[8, 28]  # column nx - 2, past nx - fw for fw = 3
+    assert O.dtstab(lam2, mu, rho, hc, 1.0, 3) == pytest.approx(want, rel=1e-12)
+    assert O.dtstab(lam2, mu, rho, hc, 1.0, 0) == pytest.approx(want * 4000.0 / 6000.0, rel=1e-12)
+    assert O.dtstab(lam, mu, rho, hc, 0.5, 0) == pytest.approx(want / 2, rel=1e-12)
 
 
 @pytest.mark.skipif(gpu_available(), reason="checks the behaviour without a device")
@@ -128,7 +143,8 @@ def test_invalid_options_are_rejected_before_any_device_work():
     from fwi_cuda_amd import _lib, psv
     O.load("oracle")
     c = cases.small_case(O, nshot=1, nt=10)
-    for kw in ({"tile_tz": 5}, {"kernel_variant": 9}, {"work_order": 5}):
+    for kw in ({"tile_tz": 5}, {"kernel_variant": 10}, {"work_order": 5}, {"kernel_variant": 9, "tile_tz": 3},
+               {"ckpt_slots": -1}, {"smooth_hfs": 1}):
         with pytest.raises(_lib.FwiError) as e:
             psv.Simulation(c, **kw)
         assert e.value.status == -1, kw
@@ -208,6 +224,7 @@ def test_bench_names_the_tuned_instance_and_finds_its_traffic(tmp_path):
     log = "chosen o2/v8/tz40/d1; o2/v7/tz40/d1 4.880; "
     assert bench.kernel_instantiation("step", log, 4) == "k_step7<4, 4, *, true>"
     assert bench.kernel_instantiation("step", "chosen o2/v7/tz80/d1; ", 4) == "k_step7<4, 8, *, false>"
+    assert bench.kernel_instantiation("step", "chosen o1/v9/tz2/d2; ", 4) == "k_step7<4, 4, *, true>"
     assert bench.kernel_instantiation("step", "chosen o0/v4/tz32/d1; ", 4) == "k_step<4, 32, "
     assert bench.kernel_instantiation("velocity", "chosen o1/v6/tz20/d1; ", 4) == "k_velocity6<4, 1, 2, "
     assert bench.kernel_instantiation("stress", "chosen o2/v2/tz16/d1; ", 4) == "k_stress2<4, 16, 1>"

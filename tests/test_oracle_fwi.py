@@ -127,3 +127,63 @@ def test_fwi_iteration_on_the_reference_fixture(oracle_mod):
         np.testing.assert_allclose(G[i], want, rtol=1e-12, atol=1e-12 * np.abs(want).max())
         alone = O.taper2(c, own[i].copy())
         assert np.linalg.norm(G[i] - alone) > 1e-3 * np.linalg.norm(alone)
+
+
+def test_pcg_matches_the_reference(oracle_mod):
+    """PCG_PSV (src/cpu/n_PCG_PSV.cpp:13-44): with a previous direction in place, the
+    restatement and the compiled reference agree bit for bit over two calls; on the first
+    call (old = 0) the reference divides by zero and returns NaN, the restatement takes
+    beta = 0 (the steepest-descent direction -g)."""
+    O = oracle_mod
+    if not O.ref_available():
+        pytest.skip("oracle/_ref is not built (make -C oracle ref)")
+    rng = np.random.default_rng(31)
+    shape = (7, 9)
+    d0, o0 = rng.standard_normal(shape), rng.standard_normal(shape)
+    gs = [rng.standard_normal(shape) for _ in range(2)]
+    outs = {}
+    for which in ("oracle", "ref"):
+        d, o = d0.copy(), o0.copy()
+        got = []
+        for g0 in gs:
+            g = g0.copy()
+            O.pcg_psv(d, o, g, which=which)
+            got.append((d.copy(), o.copy(), g.copy()))
+        outs[which] = got
+    for a, b in zip(outs["oracle"], outs["ref"]):
+        for x, y in zip(a, b):
+            np.testing.assert_array_equal(x, y)
+    g = rng.standard_normal(shape)
+    d, o = np.zeros(shape), np.zeros(shape)
+    O.pcg_psv(d, o, g.copy(), which="ref")
+    assert np.isnan(d).all()
+    d, o, gg = np.zeros(shape), np.zeros(shape), g.copy()
+    O.pcg_psv(d, o, gg)
+    np.testing.assert_array_equal(gg, -g)
+    np.testing.assert_array_equal(o, -g)
+
+
+@pytest.mark.parametrize("hfs", [2, 3, 5])
+def test_gauss_filter_kernel_and_smoothing(oracle_mod, hfs):
+    """gauss_filter_kernel against the compiled reference (bit for bit; sigma = hfs / 2 in
+    integer division, normalised); apply_gauss_filter against a direct numpy correlation
+    over the reference's index range.  The reference's own apply_gauss_filter accumulates in
+    an int, so a gradient-sized field (|values| < 1) comes out 0 wherever it filters."""
+    O = oracle_mod
+    if not O.ref_available():
+        pytest.skip("oracle/_ref is not built (make -C oracle ref)")
+    k = O.gauss_filter_kernel(hfs)
+    np.testing.assert_array_equal(k, O.gauss_filter_kernel(hfs, which="ref"))
+    assert k.sum() == pytest.approx(1.0, rel=1e-14)
+    np.testing.assert_array_equal(k, k.T)
+    rng = np.random.default_rng(hfs)
+    A = rng.standard_normal((24, 31)) * 1e-3
+    got = O.apply_gauss_filter(A.copy(), hfs)
+    want = A.copy()
+    for iz in range(hfs, 24 - hfs - 1):
+        for ix in range(hfs, 31 - hfs - 1):
+            want[iz, ix] = np.sum(A[iz - hfs:iz + hfs + 1, ix - hfs:ix + hfs + 1] * k)
+    np.testing.assert_allclose(got, want, rtol=1e-12, atol=1e-18)
+    ref = O.apply_gauss_filter(A.copy(), hfs, which="ref")
+    assert np.all(ref[hfs:24 - hfs - 1, hfs:31 - hfs - 1] == 0.0)
+    np.testing.assert_array_equal(ref[:hfs], A[:hfs])

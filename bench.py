@@ -104,7 +104,8 @@ def shot_groups(nshot, per_step):
 
 def kernel_instantiation(kind, tune_log, N):
     """The template instance the autotuner picked for a stencil kernel kind, as the
-    profiler names it (without the argument list)."""
+    profiler names it (without the argument list); a launch-mode template argument
+    (forward, recomputation, adjoint) shows as *."""
     import re
     m = re.search(r"chosen o\d/v(\d)/tz(\d+)", tune_log or "")
     if not m:
@@ -115,15 +116,15 @@ def kernel_instantiation(kind, tune_log, N):
             return f"k_step7<{N}, 4, *, true>"
         if var in (7, 8):   # variant 8: the block's wavefronts side by side in x
             return f"k_step7<{N}, {max(1, tz // (2 * N + 2))}, *, {'true' if var == 8 else 'false'}>"
-        return f"k_step<{N}, {tz}, "
+        return f"k_step<{N}, {tz}, *>"
     base = {"stress": "k_stress", "velocity": "k_velocity"}[kind]
     if var == 1:
         return f"{base}<{N}, {tz}>"
-    if var == 5:   # the velocity kernel's launch modes share the prefix
-        return f"{base}5<{N}, {tz}>" if kind == "stress" else f"{base}5<{N}, {tz}, "
+    if var == 5:
+        return f"{base}5<{N}, {tz}>" if kind == "stress" else f"{base}5<{N}, {tz}, *>"
     if var == 6:   # segments of tz = M blocks of 2N + 2 rows, loads one row ahead
         m = max(1, tz // (2 * N + 2))
-        return f"{base}6<{N}, 1, {m}>" if kind == "stress" else f"{base}6<{N}, 1, {m}, "
+        return f"{base}6<{N}, 1, {m}>" if kind == "stress" else f"{base}6<{N}, 1, {m}, *>"
     return f"{base}2<{N}, {tz}, {1 if var == 2 else 6}>"
 
 
@@ -554,11 +555,12 @@ def main():
         "dtype": "f32",
         "data": "synthetic (seeded Gaussian-random-field model, observed data modelled on the GPU)",
         "config": {"workload": f"{args.config}: {nz}x{nx} grid, dx {c['dx']} m, nt {nt}, {nshot} shots, "
-                               f"uneven receivers, order {int(c['fdorder'])}",
+                               f"uneven receivers, order {int(c['fdorder'])}, gradient correlated every "
+                               f"{sdt} steps (the reference's snap_dt)",
                    "step": f"full gradient of {sps} shots (one launch batch); {len(groups)} steps = one "
                            f"gradient over all {nshot} shots",
                    "shots_per_step": sps, "batch": plan["batch"], "storage": plan["storage"],
-                   "ckpt_interval": plan["ckpt_interval"], "tile_rows": tz,
+                   "ckpt_interval": plan["ckpt_interval"], "snap_dt": sdt, "tile_rows": tz,
                    "parallelism": "single GPU, shots batched per launch"},
         "shots_per_s": round(shots_per_s, 3),
         "hbm_frac": round(hbm_frac, 4),

@@ -9,7 +9,9 @@ C5  1001 x 2001, 8 layers + 3 ellipses, 32 shots, 150 receivers, 150 Hz, nt 4000
 Common set-up (order-8 Holberg stencil, 20-cell CPML left / right / bottom, a
 free surface at the top row of the interior (isurf = fpad), vertical-force
 sources two cells below the surface, receivers on the surface row, dt = 0.5
-of the stencil's stability limit).  ``nt`` / ``nshot`` can be reduced for
+of the stencil's stability limit, snapshots and the gradient's correlation on
+every 3rd time step as the reference's FWI set-up has them,
+scripts/pre_proc_fwi.py:68).  ``nt`` / ``nshot`` / ``snap_dt`` can be changed for
 tests; every model is reproducible from its seed.
 """
 from __future__ import annotations
@@ -20,6 +22,7 @@ from . import psv
 
 NPML = 20
 FDORDER = 8
+SNAP_DT = 3   # scripts/pre_proc_fwi.py:68
 
 
 def gardner(cp):
@@ -52,7 +55,7 @@ def layers(nz, nx, n, lo, hi, seed, monotone=True):
     return cp
 
 
-def _case(nz, nx, dx, nt, cp, cs, rho, src_x, rec, fc, seed, snap_dt=1):
+def _case(nz, nx, dx, nt, cp, cs, rho, src_x, rec, fc, seed, snap_dt=SNAP_DT):
     lam, mu, rho = lame(cp, cs, rho)
     N = FDORDER // 2
     hc = psv.holbergcoeff(FDORDER)
@@ -73,7 +76,7 @@ def _case(nz, nx, dx, nt, cp, cs, rho, src_x, rec, fc, seed, snap_dt=1):
     return c
 
 
-def c1(nt=2000, seed=1, true_model=True):
+def c1(nt=2000, seed=1, true_model=True, snap_dt=SNAP_DT):
     nz, nx = 201, 401
     rng = np.random.default_rng(seed)
     cp = np.full((nz, nx), 3000.0)
@@ -87,15 +90,15 @@ def c1(nt=2000, seed=1, true_model=True):
         cp[m] *= 1.1
         cs[m] *= 1.1
     rec = np.linspace(NPML, nx - NPML - 1, 40).astype(np.int32)
-    return _case(nz, nx, 1.0, nt, cp, cs, np.full((nz, nx), 2200.0), [nx // 2], rec, 100.0, seed)
+    return _case(nz, nx, 1.0, nt, cp, cs, np.full((nz, nx), 2200.0), [nx // 2], rec, 100.0, seed, snap_dt)
 
 
-def c2(nt=3000, seed=2, nshot=8):
+def c2(nt=3000, seed=2, nshot=8, snap_dt=SNAP_DT):
     nz, nx = 301, 601
     cp = layers(nz, nx, 10, 2000.0, 4500.0, seed, monotone=False)
     src = np.linspace(NPML + 10, nx - NPML - 11, nshot).astype(np.int32)
     rec = np.linspace(NPML, nx - NPML - 1, 120).astype(np.int32)
-    return _case(nz, nx, 1.0, nt, cp, cp / np.sqrt(3.0), gardner(cp), src, rec, 250.0, seed)
+    return _case(nz, nx, 1.0, nt, cp, cp / np.sqrt(3.0), gardner(cp), src, rec, 250.0, seed, snap_dt)
 
 
 def uneven_receivers(nx, nshot, src_x, seed, lo=20, hi=200):
@@ -115,15 +118,15 @@ def uneven_receivers(nx, nshot, src_x, seed, lo=20, hi=200):
     return np.asarray(zr, np.int32), np.asarray(xr, np.int32), np.asarray(off, np.int32)
 
 
-def c3(nt=4000, seed=3, nshot=64, nz=1001, nx=2001):
+def c3(nt=4000, seed=3, nshot=64, nz=1001, nx=2001, snap_dt=SNAP_DT):
     cp = grf(nz, nx, 30.0, 3500.0, 300.0, seed)
     rng = np.random.default_rng(seed)
     src = np.sort(rng.integers(NPML + 10, nx - NPML - 10, nshot)).astype(np.int32)
     rec = uneven_receivers(nx, nshot, src, seed)
-    return _case(nz, nx, 0.5, nt, cp, cp / 1.8, gardner(cp), src, rec, 100.0, seed)
+    return _case(nz, nx, 0.5, nt, cp, cp / 1.8, gardner(cp), src, rec, 100.0, seed, snap_dt)
 
 
-def c4(nt=8000, seed=4, nshot=16, nz=2001, nx=4001):
+def c4(nt=8000, seed=4, nshot=16, nz=2001, nx=4001, snap_dt=SNAP_DT):
     cp = layers(nz, nx, 15, 1800.0, 5000.0, seed)
     cs = cp / np.sqrt(3.0)
     rho = gardner(cp)
@@ -135,10 +138,10 @@ def c4(nt=8000, seed=4, nshot=16, nz=2001, nx=4001):
     cp[m], cs[m], rho[m] = 5500.0, 3100.0, 2900.0
     src = np.linspace(NPML + 10, nx - NPML - 11, nshot).astype(np.int32)
     rec = np.linspace(NPML, nx - NPML - 1, 300).astype(np.int32)
-    return _case(nz, nx, 0.5, nt, cp, cs, rho, src, rec, 100.0, seed)
+    return _case(nz, nx, 0.5, nt, cp, cs, rho, src, rec, 100.0, seed, snap_dt)
 
 
-def c5(nt=4000, seed=5, nshot=32, true_model=True, nz=1001, nx=2001, nrec=150):
+def c5(nt=4000, seed=5, nshot=32, true_model=True, nz=1001, nx=2001, nrec=150, snap_dt=SNAP_DT):
     """C5; a smaller nz x nx keeps the layout with the anomalies and the start model's
     smoothing scaled by nz / 1001 (the seeded draws are the same at full size)."""
     from scipy.ndimage import gaussian_filter
@@ -159,7 +162,7 @@ def c5(nt=4000, seed=5, nshot=32, true_model=True, nz=1001, nx=2001, nrec=150):
         cs = gaussian_filter(cs, 40.0 * f_sz, mode="nearest")
     src = np.linspace(NPML + 10, nx - NPML - 11, nshot).astype(np.int32)
     rec = np.linspace(NPML, nx - NPML - 1, nrec).astype(np.int32)
-    return _case(nz, nx, 0.5, nt, cp, cs, gardner(cp), src, rec, 150.0, seed)
+    return _case(nz, nx, 0.5, nt, cp, cs, gardner(cp), src, rec, 150.0, seed, snap_dt)
 
 
 BUILDERS = {"C1": c1, "C2": c2, "C3": c3, "C4": c4, "C5": c5}

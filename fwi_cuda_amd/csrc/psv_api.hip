@@ -657,10 +657,13 @@ int autotune(fwi_ctx *c, int nb) {
     hipEvent_t e0, e1;
     HIPCHK(hipEventCreate(&e0));
     HIPCHK(hipEventCreate(&e1));
-    // A gradient's mix of launches: a forward step, a recomputed step that stores its
-    // snapshot and an adjoint step that correlates with one (slab 0 of the snapshot
-    // buffers; the state, sums and slab are overwritten or zeroed before any pass).
-    KArgs as = a, ag = a;
+    // A gradient's mix of launches over snap_dt steps (at most 4): snap_dt forward steps,
+    // snap_dt recomputed steps of which one stores its snapshot and snap_dt adjoint steps
+    // of which one correlates with it (slab 0 of the snapshot buffers; the state, sums
+    // and slab are overwritten or zeroed before any pass).
+    const int kmix = std::min(std::max(c->g.snap_dt, 1), 4);
+    KArgs ar = a, as = a, ap = a, ag = a;
+    ar.energy = 0;
     as.energy = 0;
     as.store = 1;
     as.st_vz = c->snap[0]; as.st_vx = c->snap[1]; as.st_szz = c->snap[2]; as.st_szx = c->snap[3];
@@ -672,8 +675,14 @@ int autotune(fwi_ctx *c, int nb) {
     ag.sn_vz = c->snap[0]; ag.sn_vx = c->snap[1]; ag.sn_szz = c->snap[2]; ag.sn_szx = c->snap[3];
     ag.sn_sxx = c->snap[4];
     ag.sn_stride = c->snap_stride;
+    ap.We = c->f[8];
+    ap.displ = 1;
+    std::vector<KArgs *> mix;
+    for (int k = 0; k < kmix; ++k) mix.push_back(&a);
+    for (int k = 0; k < kmix; ++k) mix.push_back(k ? &ar : &as);
+    for (int k = 0; k < kmix; ++k) mix.push_back(k ? &ap : &ag);
     auto run = [&](const TuneVal &v) {
-        for (KArgs *x : {&a, &as, &ag}) {
+        for (KArgs *x : mix) {
             x->order = a.order;
             if (one_pass(v.var)) {
                 launch_one_pass(c, *x, v.var, v.tz, v.depth);

@@ -1,13 +1,15 @@
 """Multi-iteration FWI on a benchmark workload, through the C ABI.
 
-    python -m fwi_cuda_amd.run_fwi --config C5 --iters 10
+    python -m fwi_cuda_amd.run_fwi --config C5 --iters 10 [--param lame] [--pcg] [--smooth HFS]
 
 The observed data are modelled from the workload's true model; the inversion
 starts from its smoothed start model (C5: the true model smoothed by a 40-cell
 Gaussian) and runs simulate_fwi_PSV's iteration (src/cpu/n_simulate_PSV.cpp:249-582):
 per-shot gradients preconditioned by the energy weights and summed, the taper,
-the parabolic step-length search and the model update.  The gradient sign is
-the GPU twin's (d_kernel_lib_PSV.cu:838-840), the one under which the loop
+the parabolic step-length search and the model update.  The unknowns are cp and cs
+with rho held fixed (the lam/mu/rho gradient carried to them by the chain rule on the
+device), or lam, mu, rho as the reference updates them (--param lame).  The gradient
+sign is the GPU twin's (d_kernel_lib_PSV.cu:838-840), the one under which the loop
 descends.  One line per iteration: misfit, step, wall time, and the model's
 relative L2 distance to the true P and S velocities.
 """
@@ -40,6 +42,9 @@ def main(argv=None):
     ap.add_argument("--iters", type=int, default=10)
     ap.add_argument("--nt", type=int, default=0, help="shorter time axis")
     ap.add_argument("--nshot", type=int, default=0, help="fewer shots")
+    ap.add_argument("--param", choices=("velocity", "lame"), default="velocity")
+    ap.add_argument("--pcg", action="store_true", help="conjugate-gradient directions (PCG_PSV)")
+    ap.add_argument("--smooth", type=int, default=0, metavar="HFS", help="Gaussian smoothing half width")
     ap.add_argument("--json", default="", help="also write the history as JSON to this file")
     args = ap.parse_args(argv)
 
@@ -60,7 +65,8 @@ def main(argv=None):
     print(f"{args.config}: {true['nz']}x{true['nx']}, nt {true['nt']}, {true['nshot']} shots", flush=True)
 
     hist = []
-    with psv.Simulation(true, grad_sign=1) as sim:
+    param = psv._lib.PARAM_VELOCITY if args.param == "velocity" else psv._lib.PARAM_LAME
+    with psv.Simulation(true, grad_sign=1, param=param, pcg=int(args.pcg), smooth_hfs=args.smooth) as sim:
         t0 = time.perf_counter()
         uz, ux = sim.forward()
         sim.set_observed(uz, ux)
@@ -87,7 +93,8 @@ def main(argv=None):
                   f"cp rel-L2 {rec['cp_rel_l2']:.5f}, cs rel-L2 {rec['cs_rel_l2']:.5f}", flush=True)
     if args.json:
         with open(args.json, "w") as f:
-            json.dump({"config": args.config, "start": {"cp_rel_l2": e0[0], "cs_rel_l2": e0[1]},
+            json.dump({"config": args.config, "param": args.param, "pcg": args.pcg, "smooth": args.smooth,
+                       "start": {"cp_rel_l2": e0[0], "cs_rel_l2": e0[1]},
                        "iterations": hist}, f, indent=1)
     return 0 if hist else 1
 

@@ -1,6 +1,6 @@
 """The benchmark configurations themselves against the FP64 oracle (the CPU path restated).
 
-  C1  full size (201 x 401, one shot, 40 receivers, nt 2000, snap_dt 1): forward
+  C1  full size (201 x 401, one shot, 40 receivers, nt 2000, snap_dt 3): forward
       seismograms, the misfit and preconditioned gradient, and one FWI iteration
       (gradient, taper, step-length search, update) with the GPU twin's sign
   C2  one shot of the C2 grid (301 x 601, 120 receivers, nt 3000, snapshots every 4th

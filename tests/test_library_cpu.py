@@ -225,8 +225,9 @@ def test_bench_names_the_tuned_instance_and_finds_its_traffic(tmp_path):
     assert bench.kernel_instantiation("step", log, 4) == "k_step7<4, 4, *, true>"
     assert bench.kernel_instantiation("step", "chosen o2/v7/tz80/d1; ", 4) == "k_step7<4, 8, *, false>"
     assert bench.kernel_instantiation("step", "chosen o1/v9/tz2/d2; ", 4) == "k_step7<4, 4, *, true>"
-    assert bench.kernel_instantiation("step", "chosen o0/v4/tz32/d1; ", 4) == "k_step<4, 32, "
-    assert bench.kernel_instantiation("velocity", "chosen o1/v6/tz20/d1; ", 4) == "k_velocity6<4, 1, 2, "
+    assert bench.kernel_instantiation("step", "chosen o0/v4/tz32/d1; ", 4) == "k_step<4, 32, *>"
+    assert bench.kernel_instantiation("velocity", "chosen o1/v6/tz20/d1; ", 4) == "k_velocity6<4, 1, 2, *>"
+    assert bench.kernel_instantiation("velocity", "chosen o1/v5/tz16/d1; ", 4) == "k_velocity5<4, 16, *>"
     assert bench.kernel_instantiation("stress", "chosen o2/v2/tz16/d1; ", 4) == "k_stress2<4, 16, 1>"
     f = tmp_path / "traffic.txt"
     f.write_text("header\n"

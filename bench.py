@@ -107,7 +107,7 @@ def kernel_instantiation(kind, tune_log, N):
     profiler names it (without the argument list); a launch-mode template argument
     (forward, recomputation, adjoint) shows as *."""
     import re
-    m = re.search(r"chosen o\d/v(\d)/tz(\d+)", tune_log or "")
+    m = re.search(r"chosen o\d/v(\d+)/tz(\d+)", tune_log or "")
     if not m:
         return None
     var, tz = int(m.group(1)), int(m.group(2))
@@ -120,8 +120,8 @@ def kernel_instantiation(kind, tune_log, N):
     base = {"stress": "k_stress", "velocity": "k_velocity"}[kind]
     if var == 1:
         return f"{base}<{N}, {tz}>"
-    if var == 5:
-        return f"{base}5<{N}, {tz}>" if kind == "stress" else f"{base}5<{N}, {tz}, *>"
+    if var in (5, 10):   # 10: variant 5's kernels on full 64-column tiles
+        return f"{base}{var}<{N}, {tz}>" if kind == "stress" else f"{base}{var}<{N}, {tz}, *>"
     if var == 6:   # segments of tz = M blocks of 2N + 2 rows, loads one row ahead
         m = max(1, tz // (2 * N + 2))
         return f"{base}6<{N}, 1, {m}>" if kind == "stress" else f"{base}6<{N}, 1, {m}, *>"

@@ -605,14 +605,14 @@ int autotune(fwi_ctx *c, int nb) {
     const bool order_forced = c->o.work_order >= 1 && c->o.work_order <= 4;
     // candidates (variant, rows per segment, prefetch depth); the one-pass step only where it applies
     std::vector<TuneVal> cand;
-    for (int var = 1; var <= 9; ++var) {
+    for (int var = 1; var <= 10; ++var) {
         if ((fv && var != fv) || (one_pass(var) && !c->fusable) || (var >= 5 && !c->ztab_ok)) continue;
         if (var == 9) {   // balanced: one or two waves of resident blocks
             for (int m : {1, 2})
                 if (c->o.tile_tz <= 0 || c->o.tile_tz == m) cand.push_back({9, m, 1, 2});
             continue;
         }
-        if (var >= 6) {   // loads one row ahead, segments of 1, 2 (variant 6) or 4, 8 (variants 7, 8) blocks
+        if (var >= 6 && var <= 8) {   // loads one row ahead, segments of 1, 2 (variant 6) or 4, 8 (variants 7, 8) blocks
             for (int m : var == 6 ? std::vector<int>{1, 2} : std::vector<int>{4, 8}) {
                 const int tz = v6_block(c->N, 1) * m;
                 if (c->o.tile_tz <= 0 || c->o.tile_tz == tz) cand.push_back({var, tz, 1, 0});
@@ -620,7 +620,7 @@ int autotune(fwi_ctx *c, int nb) {
             continue;
         }
         const std::vector<int> tzs = var == 4   ? std::vector<int>{32, 64}
-                                     : var == 5 ? std::vector<int>{16, 32}
+                                     : var == 5 || var == 10 ? std::vector<int>{16, 32}
                                                 : std::vector<int>{8, 16, 32, 64};
         for (int tz : tzs)
             if (c->o.tile_tz <= 0 || c->o.tile_tz == tz) cand.push_back({var, tz, 1, 0});
@@ -629,11 +629,12 @@ int autotune(fwi_ctx *c, int nb) {
         return c->fail(FWI_EINVAL, "no stencil kernel for kernel_variant %d, tile_tz %d%s", fv, c->o.tile_tz,
                        fv == 4   ? " (the one-pass step takes tile_tz 32 or 64, no free surface other than "
                                    "the top one at row fdorder/2, and sources and receivers inside the interior)"
-                       : fv == 5 ? " (variant 5 takes tile_tz 16 or 32, and z CPML strips of at most 64 rows "
-                                   "at the top and bottom of the grid)"
+                       : fv == 5 || fv == 10
+                           ? " (variants 5 and 10 take tile_tz 16 or 32, and z CPML strips of at most 64 rows "
+                             "at the top and bottom of the grid)"
                        : fv == 6 ? " (variant 6 takes tile_tz 0, fdorder + 2 or 2 fdorder + 4, and z CPML "
                                    "strips of at most 64 rows at the top and bottom of the grid)"
-                       : fv >= 7 ? " (variants 7 and 8 take tile_tz 0, 4 fdorder + 8 or 8 fdorder + 16, "
+                       : fv == 7 || fv == 8 ? " (variants 7 and 8 take tile_tz 0, 4 fdorder + 8 or 8 fdorder + 16, "
                                    "variant 4's conditions and variant 6's z CPML strips)"
                                  : "");
     if (cand.size() == 1 && order_forced) {   // nothing left to choose
@@ -1470,8 +1471,8 @@ int validate(fwi_ctx *c, const fwi_grid *g, const fwi_acq *q) {
         return c->fail(FWI_EINVAL,
                        "tile_tz %d (0, 8, 16, 32 or 64; variant 6: 0 or its row block times 1 or 2; "
                        "variants 7, 8: 0 or their row block times 4 or 8)", tt);
-    if (c->o.kernel_variant < 0 || c->o.kernel_variant > 9)
-        return c->fail(FWI_EINVAL, "kernel_variant %d (0..9)", c->o.kernel_variant);
+    if (c->o.kernel_variant < 0 || c->o.kernel_variant > 10)
+        return c->fail(FWI_EINVAL, "kernel_variant %d (0..10)", c->o.kernel_variant);
     if (c->o.kernel_variant == 9 && tt > 2) return c->fail(FWI_EINVAL, "variant 9 takes tile_tz 0, 1 or 2");
     if (c->o.work_order < 0 || c->o.work_order > 4)
         return c->fail(FWI_EINVAL, "work_order %d (0..4)", c->o.work_order);

@@ -49,9 +49,12 @@ __device__ __forceinline__ StepView step_view(const KArgs &a, int it) {
 // With XW the kBY wavefronts of a block sit side by side in x (a tile is kBY * OUTW
 // columns of one z segment), otherwise one above the other in z (kBY segments of one
 // OUTW-column tile).
-template <int TZ, int OUTW = kBX, bool XW = false>
+// FULL: tiles of OUTW = kBX columns starting at column 0 (aligned rows), covering the
+// interior columns [N, nx - N); otherwise tiles of OUTW output columns from column N.
+template <int TZ, int OUTW = kBX, bool XW = false, bool FULL = false>
 __device__ __forceinline__ void tile_of(const KArgs &a, int N, int &bx, int &by, int &s) {
-    const int gx = XW ? ((a.nx - 2 * N + OUTW - 1) / OUTW + kBY - 1) / kBY : (a.nx - 2 * N + OUTW - 1) / OUTW;
+    const int xspan = FULL ? a.nx - N : a.nx - 2 * N;
+    const int gx = XW ? ((xspan + OUTW - 1) / OUTW + kBY - 1) / kBY : (xspan + OUTW - 1) / OUTW;
     const int gy = XW ? (a.nz - 2 * N + TZ - 1) / TZ : ((a.nz - 2 * N + TZ - 1) / TZ + kBY - 1) / kBY;
     const int n = gx * gy * a.B;
     const int L = blockIdx.x, xcd = L & 7, loc = L >> 3;
@@ -83,6 +86,24 @@ __device__ __forceinline__ void tile_of(const KArgs &a, int N, int &bx, int &by,
 // value of v in lane (lane + d); d is a compile-time constant after unrolling
 __device__ __forceinline__ float lane_off(float v, int lane, int d) {
     return d == 0 ? v : __shfl(v, lane + d);
+}
+
+// Column (lane + d) of a full kBX-column tile, |d| <= the halo width H: v holds the
+// lane's own column; h holds the halo, the H columns right of the tile in lanes
+// 0..H-1 and the H columns left of it in lanes kBX-H..kBX-1.  One rotation by d: the
+// source lanes that only wrapped destinations read offer their halo value instead.
+__device__ __forceinline__ float lane_off_h(float v, float h, int lane, int d) {
+    if (d == 0) return v;
+    const bool from_h = d > 0 ? lane < d : lane >= kBX + d;
+    return __shfl(from_h ? h : v, (lane + d) & (kBX - 1));
+}
+
+// The column a lane loads into the halo register of a full tile starting at column
+// x0 (see lane_off_h), clamped into [0, nx): lanes H..kBX-H-1 repeat lane 0's column
+// so that the load touches only the halo's two lines.
+__device__ __forceinline__ int halo_col(int x0, int lane, int H, int nx) {
+    const int c = lane < H ? x0 + kBX + lane : lane >= kBX - H ? x0 - kBX + lane : x0 + kBX;
+    return min(max(c, 0), nx - 1);
 }
 
 // Wave-uniform base pointer plus a 32-bit byte offset per lane: the global

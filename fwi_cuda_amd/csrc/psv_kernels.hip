@@ -509,20 +509,25 @@ __global__ __launch_bounds__(kBX *kBY, MINW) void k_velocity2(const KArgs a) {
 // only owned lanes store.  The velocity kernel's launch modes are template flags.
 // ---------------------------------------------------------------------------
 
-template <int N, int TZ>
-__global__ __launch_bounds__(kBX *kBY) void k_stress5(const KArgs a, const ZTab zt) {
-    constexpr int OUTW = kBX - 2 * N;
+// FULL (variant 10): every lane of a 64-column tile starting at column 64 k owns its
+// column (rows start on 256-byte boundaries, no lane is spent on the halo); the N halo
+// columns each side that the x taps need come from one extra load per field and row
+// into a halo register, read through lane_off_h.
+template <int N, int TZ, bool FULL>
+__device__ __forceinline__ void stress5_body(const KArgs &a, const ZTab &zt) {
+    constexpr int OUTW = FULL ? kBX : kBX - 2 * N;
     int tbx, tby, s;
-    tile_of<TZ, OUTW>(a, N, tbx, tby, s);
+    tile_of<TZ, OUTW, false, FULL>(a, N, tbx, tby, s);
     const int lane = threadIdx.x;
     const int z0 = N + (tby * kBY + wave_uniform((int)threadIdx.y)) * TZ;
     const int nzi = a.nz - N;
     if (z0 >= nzi) return;
     const int zend = min(z0 + TZ, nzi);
     const int ix = tbx * OUTW + lane;
-    const bool out = lane >= N && lane < kBX - N && ix < a.nx - N;
+    const bool out = (FULL ? ix >= N : (lane >= N && lane < kBX - N)) && ix < a.nx - N;
     const int cx = min(ix, a.nx - 1);   // halo lanes past the grid read a valid column
     const uint32_t c4 = (uint32_t)cx * 4u;
+    const uint32_t h4 = FULL ? (uint32_t)halo_col(tbx * OUTW, lane, N, a.nx) * 4u : 0u;
     const uint32_t P4 = (uint32_t)a.P * 4u;
     const size_t P = a.P, fs = (size_t)s * a.fs;
     // base of row z0 + d of a field of this shot, of a media array
@@ -556,9 +561,19 @@ __global__ __launch_bounds__(kBX *kBY) void k_stress5(const KArgs a, const ZTab 
     const uint32_t vlast = (uint32_t)(zend - 1 - z0) * P4 + c4;
     float hzA = ld_at(vzq, c4), hxA = ld_at(vxq, c4);
     float hzB = ld_at(vzq, min(c4 + P4, vlast)), hxB = ld_at(vxq, min(c4 + P4, vlast));
+    // FULL: halo columns of vz, vx at rows z0 and z0 + 1 (byte offsets from row z0 at h4 - c4)
+    const float *vzc_row = fld(a.vz, 0), *vxc_row = fld(a.vx, 0);
+    const uint32_t hoff = h4 - c4, hlast = (uint32_t)(zend - 1 - z0) * P4 + h4;
+    float gzA = 0.f, gxA = 0.f, gzB = 0.f, gxB = 0.f;
+    if constexpr (FULL) {
+        gzA = ld_at(vzc_row, h4);
+        gxA = ld_at(vxc_row, h4);
+        gzB = ld_at(vzc_row, min(h4 + P4, hlast));
+        gxB = ld_at(vxc_row, min(h4 + P4, hlast));
+    }
     // row z (byte offset vo of (z, cx) from row z0); hz, hx hold its heads on entry and
-    // row z + 2's on return
-    auto row = [&](int z, uint32_t vo, float &hz, float &hx) {
+    // row z + 2's on return, gz, gx (FULL) its halo columns of vz, vx and row z + 2's
+    auto row = [&](int z, uint32_t vo, float &hz, float &hx, float &gz, float &gx) {
 #pragma unroll
         for (int j = 0; j < 2 * N - 1; ++j) {
             qz[j] = qz[j + 1];
@@ -574,13 +589,26 @@ __global__ __launch_bounds__(kBX *kBY) void k_stress5(const KArgs a, const ZTab 
         hz = ld_at(vzq, v2);
         hx = ld_at(vxq, v2);
         const float vzc = qz[N], vxc = qx[N - 1];
+        float vzh = 0.f, vxh = 0.f;
+        if constexpr (FULL) {   // this row's halo, then row z + 2's
+            vzh = gz;
+            vxh = gx;
+            const uint32_t g2 = min(vo + hoff + 2 * P4, hlast);
+            gz = ld_at(vzc_row, g2);
+            gx = ld_at(vxc_row, g2);
+        }
         float vz_z = 0.f, vx_z = 0.f, vz_x = 0.f, vx_x = 0.f;
 #pragma unroll
         for (int k = 1; k <= N; ++k) {
             vz_z += a.cz[k - 1] * (qz[N + k - 1] - qz[N - k]);
             vx_z += a.cz[k - 1] * (qx[N + k - 1] - qx[N - k]);
-            vz_x += a.cx[k - 1] * (lane_off(vzc, lane, k) - lane_off(vzc, lane, 1 - k));
-            vx_x += a.cx[k - 1] * (lane_off(vxc, lane, k - 1) - lane_off(vxc, lane, -k));
+            if constexpr (FULL) {
+                vz_x += a.cx[k - 1] * (lane_off_h(vzc, vzh, lane, k) - lane_off_h(vzc, vzh, lane, 1 - k));
+                vx_x += a.cx[k - 1] * (lane_off_h(vxc, vxh, lane, k - 1) - lane_off_h(vxc, vxh, lane, -k));
+            } else {
+                vz_x += a.cx[k - 1] * (lane_off(vzc, lane, k) - lane_off(vzc, lane, 1 - k));
+                vx_x += a.cx[k - 1] * (lane_off(vxc, lane, k - 1) - lane_off(vxc, lane, -k));
+            }
         }
         if (in_zstrip(zt, z)) {   // wave-uniform; b, a, 1/K at the integer point, then at the half point
             const int zr = strip_row(zt, N, z);
@@ -611,28 +639,40 @@ __global__ __launch_bounds__(kBX *kBY) void k_stress5(const KArgs a, const ZTab 
     int z = z0;
     uint32_t vo = c4;
     for (; z + 1 < zend; z += 2, vo += 2 * P4) {
-        row(z, vo, hzA, hxA);
-        row(z + 1, vo + P4, hzB, hxB);
+        row(z, vo, hzA, hxA, gzA, gxA);
+        row(z + 1, vo + P4, hzB, hxB, gzB, gxB);
     }
-    if (z < zend) row(z, vo, hzA, hxA);
+    if (z < zend) row(z, vo, hzA, hxA, gzA, gxA);
 }
 
-// The velocity rows of one wavefront; GRAD: this step correlates with the snapshot slab.
-template <int N, int TZ, int MODE, bool GRAD>
+template <int N, int TZ>
+__global__ __launch_bounds__(kBX *kBY) void k_stress5(const KArgs a, const ZTab zt) {
+    stress5_body<N, TZ, false>(a, zt);
+}
+
+template <int N, int TZ>
+__global__ __launch_bounds__(kBX *kBY) void k_stress10(const KArgs a, const ZTab zt) {
+    stress5_body<N, TZ, true>(a, zt);
+}
+
+// The velocity rows of one wavefront; GRAD: this step correlates with the snapshot slab;
+// FULL: full 64-column tiles with halo registers (variant 10, as in stress5_body).
+template <int N, int TZ, int MODE, bool GRAD, bool FULL>
 __device__ __forceinline__ void velocity5_rows(const KArgs &a, const ZTab &zt, const StepView &sv) {
-    constexpr int OUTW = kBX - 2 * N;
+    constexpr int OUTW = FULL ? kBX : kBX - 2 * N;
     constexpr bool displ = (MODE & kVelAdjoint) != 0;   // u += dt v
     int tbx, tby, s;
-    tile_of<TZ, OUTW>(a, N, tbx, tby, s);
+    tile_of<TZ, OUTW, false, FULL>(a, N, tbx, tby, s);
     const int lane = threadIdx.x;
     const int z0 = N + (tby * kBY + wave_uniform((int)threadIdx.y)) * TZ;
     const int nzi = a.nz - N;
     if (z0 >= nzi) return;
     const int zend = min(z0 + TZ, nzi);
     const int ix = tbx * OUTW + lane;
-    const bool out = lane >= N && lane < kBX - N && ix < a.nx - N;
+    const bool out = (FULL ? ix >= N : (lane >= N && lane < kBX - N)) && ix < a.nx - N;
     const int cx = min(ix, a.nx - 1);
     const uint32_t c4 = (uint32_t)cx * 4u;
+    const uint32_t h4 = FULL ? (uint32_t)halo_col(tbx * OUTW, lane, N, a.nx) * 4u : 0u;
     const uint32_t P4 = (uint32_t)a.P * 4u;
     const size_t P = a.P, fs = (size_t)s * a.fs;
     auto fld = [&](float *f, int d) { return f + fs + (size_t)(z0 + d) * P; };
@@ -676,9 +716,20 @@ __device__ __forceinline__ void velocity5_rows(const KArgs &a, const ZTab &zt, c
     float hzzA = ld_at(szzq, c4), hzxA = ld_at(szxq, c4), hxxA = ld_at(sxx, c4);
     const uint32_t v1 = min(c4 + P4, vlast);
     float hzzB = ld_at(szzq, v1), hzxB = ld_at(szxq, v1), hxxB = ld_at(sxx, v1);
+    // FULL: halo columns of szx, sxx at rows z0 and z0 + 1
+    const float *szxc_row = fld(a.szx, 0);
+    const uint32_t hoff = h4 - c4, hlast = (uint32_t)(zend - 1 - z0) * P4 + h4;
+    float gzxA = 0.f, gxxA = 0.f, gzxB = 0.f, gxxB = 0.f;
+    if constexpr (FULL) {
+        gzxA = ld_at(szxc_row, h4);
+        gxxA = ld_at(sxx, h4);
+        gzxB = ld_at(szxc_row, min(h4 + P4, hlast));
+        gxxB = ld_at(sxx, min(h4 + P4, hlast));
+    }
     // row z (byte offset vo of (z, cx) from row z0); hzz, hzx, hxx hold its derivative
-    // inputs on entry and row z + 2's on return
-    auto row = [&](int z, uint32_t vo, float &hzz, float &hzx, float &hxx) {
+    // inputs on entry and row z + 2's on return, gzx, gxx (FULL) its halo columns of
+    // szx, sxx and row z + 2's
+    auto row = [&](int z, uint32_t vo, float &hzz, float &hzx, float &hxx, float &gzx, float &gxx) {
 #pragma unroll
         for (int j = 0; j < 2 * N - 1; ++j) {
             qzz[j] = qzz[j + 1];
@@ -718,13 +769,26 @@ __device__ __forceinline__ void velocity5_rows(const KArgs &a, const ZTab &zt, c
         hzx = ld_at(szxq, v2);
         hxx = ld_at(sxx, v2);
         const float szxc = qzx[N];
+        float szxh = 0.f, sxxh = 0.f;
+        if constexpr (FULL) {   // this row's halo, then row z + 2's
+            szxh = gzx;
+            sxxh = gxx;
+            const uint32_t g2 = min(vo + hoff + 2 * P4, hlast);
+            gzx = ld_at(szxc_row, g2);
+            gxx = ld_at(sxx, g2);
+        }
         float szz_z = 0.f, szx_z = 0.f, szx_x = 0.f, sxx_x = 0.f;
 #pragma unroll
         for (int k = 1; k <= N; ++k) {
             szz_z += a.cz[k - 1] * (qzz[N + k - 1] - qzz[N - k]);
             szx_z += a.cz[k - 1] * (qzx[N + k - 1] - qzx[N - k]);
-            szx_x += a.cx[k - 1] * (lane_off(szxc, lane, k - 1) - lane_off(szxc, lane, -k));
-            sxx_x += a.cx[k - 1] * (lane_off(sxxc, lane, k) - lane_off(sxxc, lane, 1 - k));
+            if constexpr (FULL) {
+                szx_x += a.cx[k - 1] * (lane_off_h(szxc, szxh, lane, k - 1) - lane_off_h(szxc, szxh, lane, -k));
+                sxx_x += a.cx[k - 1] * (lane_off_h(sxxc, sxxh, lane, k) - lane_off_h(sxxc, sxxh, lane, 1 - k));
+            } else {
+                szx_x += a.cx[k - 1] * (lane_off(szxc, lane, k - 1) - lane_off(szxc, lane, -k));
+                sxx_x += a.cx[k - 1] * (lane_off(sxxc, lane, k) - lane_off(sxxc, lane, 1 - k));
+            }
         }
         // integer-point profiles for szz_z and sxx_x, half-point for szx_z and szx_x,
         // the reference's slot mapping (src/cpu/n_kernel_PSV.cpp:142-146)
@@ -795,24 +859,34 @@ __device__ __forceinline__ void velocity5_rows(const KArgs &a, const ZTab &zt, c
     int z = z0;
     uint32_t vo = c4;
     for (; z + 1 < zend; z += 2, vo += 2 * P4) {
-        row(z, vo, hzzA, hzxA, hxxA);
-        row(z + 1, vo + P4, hzzB, hzxB, hxxB);
+        row(z, vo, hzzA, hzxA, hxxA, gzxA, gxxA);
+        row(z + 1, vo + P4, hzzB, hzxB, hxxB, gzxB, gxxB);
     }
-    if (z < zend) row(z, vo, hzzA, hzxA, hxxA);
+    if (z < zend) row(z, vo, hzzA, hzxA, hxxA, gzxA, gxxA);
 }
 
 // kVelAdjoint launches branch once, on the step's correlation switch, to a loop
 // whose loads are all unconditional
-template <int N, int TZ, int MODE>
-__global__ __launch_bounds__(kBX *kBY) void k_velocity5(const KArgs a, const ZTab zt) {
+template <int N, int TZ, int MODE, bool FULL>
+__device__ __forceinline__ void velocity5_body(const KArgs &a, const ZTab &zt) {
     const StepView sv = step_view(a, 0);
     if constexpr ((MODE & kVelAdjoint) != 0) {
         if (sv.grad) {
-            velocity5_rows<N, TZ, MODE, true>(a, zt, sv);
+            velocity5_rows<N, TZ, MODE, true, FULL>(a, zt, sv);
             return;
         }
     }
-    velocity5_rows<N, TZ, MODE, false>(a, zt, sv);
+    velocity5_rows<N, TZ, MODE, false, FULL>(a, zt, sv);
+}
+
+template <int N, int TZ, int MODE>
+__global__ __launch_bounds__(kBX *kBY) void k_velocity5(const KArgs a, const ZTab zt) {
+    velocity5_body<N, TZ, MODE, false>(a, zt);
+}
+
+template <int N, int TZ, int MODE>
+__global__ __launch_bounds__(kBX *kBY) void k_velocity10(const KArgs a, const ZTab zt) {
+    velocity5_body<N, TZ, MODE, true>(a, zt);
 }
 
 // ---------------------------------------------------------------------------
@@ -1235,16 +1309,16 @@ __global__ void k_store(const KArgs a, float *d0, float *d1, float *d2, float *d
 // writes 64 columns per wavefront, variants 2 and 3 write 64 - 2N.  Variant 3 is
 // variant 2 compiled for at least 6 wavefronts per SIMD (a register cap).
 static dim3 stencil_grid(const KArgs &a, int N, int tz, int var) {
-    const int nxi = a.nx - 2 * N, nzi = a.nz - 2 * N;
+    const int nxi = var == 10 ? a.nx - N : a.nx - 2 * N, nzi = a.nz - 2 * N;   // variant 10: tiles from column 0
     const int segs = (nzi + tz - 1) / tz;
-    const int outw = var >= 2 ? kBX - 2 * N : kBX;
+    const int outw = var >= 2 && var != 10 ? kBX - 2 * N : kBX;
     return dim3(((nxi + outw - 1) / outw) * ((segs + kBY - 1) / kBY) * a.B);
 }
 
 // the two-kernel form is instantiated for TZ 8, 16, 32, 64 (16 for anything else);
-// variant 5 for TZ 16 and 32
+// variants 5 and 10 for TZ 16 and 32
 static int two_kernel_tz(int tz, int var) {
-    if (var == 5) return tz == 32 ? 32 : 16;
+    if (var == 5 || var == 10) return tz == 32 ? 32 : 16;
     return (tz == 8 || tz == 32 || tz == 64) ? tz : 16;
 }
 
@@ -1264,31 +1338,47 @@ static int two_kernel_tz(int tz, int var) {
         default: PSV_TZ5_CASES(KER, 6, __VA_ARGS__) break; \
     }
 
-// variant 5's velocity launch modes: the ones the passes run get their own code
-// (false: a launch mode it has no kernel for)
+// variant 5's (and 10's) velocity launch modes: the ones the passes run get their own
+// code (false: a launch mode it has no kernel for)
+template <bool FULL>
 static bool launch_velocity5(const KArgs &a, const ZTab *zt, int N, int tz, dim3 grid, dim3 block,
                              hipStream_t st) {
-    switch (vel_mode(a)) {
-        case 0: PSV_N5_CASES(k_velocity5, , 0) return true;                                   // recomputation
-        case kVelEnergy: PSV_N5_CASES(k_velocity5, , kVelEnergy) return true;                 // forward
-        case kVelStore: PSV_N5_CASES(k_velocity5, , kVelStore) return true;                   // recomputation, stored
-        case kVelEnergy | kVelStore: PSV_N5_CASES(k_velocity5, , kVelEnergy | kVelStore) return true;
-        case kVelEnergy | kVelAdjoint: PSV_N5_CASES(k_velocity5, , kVelEnergy | kVelAdjoint) return true;
-        case kVelEnergy | kVelAdjoint | kVelMisfit:
-            PSV_N5_CASES(k_velocity5, , kVelEnergy | kVelAdjoint | kVelMisfit) return true;
-        default: return false;
+    if constexpr (FULL) {
+        switch (vel_mode(a)) {
+            case 0: PSV_N5_CASES(k_velocity10, , 0) return true;
+            case kVelEnergy: PSV_N5_CASES(k_velocity10, , kVelEnergy) return true;
+            case kVelStore: PSV_N5_CASES(k_velocity10, , kVelStore) return true;
+            case kVelEnergy | kVelStore: PSV_N5_CASES(k_velocity10, , kVelEnergy | kVelStore) return true;
+            case kVelEnergy | kVelAdjoint: PSV_N5_CASES(k_velocity10, , kVelEnergy | kVelAdjoint) return true;
+            case kVelEnergy | kVelAdjoint | kVelMisfit:
+                PSV_N5_CASES(k_velocity10, , kVelEnergy | kVelAdjoint | kVelMisfit) return true;
+            default: return false;
+        }
+    } else {
+        switch (vel_mode(a)) {
+            case 0: PSV_N5_CASES(k_velocity5, , 0) return true;                                   // recomputation
+            case kVelEnergy: PSV_N5_CASES(k_velocity5, , kVelEnergy) return true;                 // forward
+            case kVelStore: PSV_N5_CASES(k_velocity5, , kVelStore) return true;                   // recomputation, stored
+            case kVelEnergy | kVelStore: PSV_N5_CASES(k_velocity5, , kVelEnergy | kVelStore) return true;
+            case kVelEnergy | kVelAdjoint: PSV_N5_CASES(k_velocity5, , kVelEnergy | kVelAdjoint) return true;
+            case kVelEnergy | kVelAdjoint | kVelMisfit:
+                PSV_N5_CASES(k_velocity5, , kVelEnergy | kVelAdjoint | kVelMisfit) return true;
+            default: return false;
+        }
     }
 }
 
 void launch_stress(const KArgs &a, int N, int tz, int var, hipStream_t st, const ZTab *zt, int depth) {
-    if (var >= 5 && !zt) var = 2;   // variants 5 and 6 need the strip-row table
+    if (var >= 5 && !zt) var = 2;   // variants 5, 6 and 10 need the strip-row table
     if (var == 6) {
         launch6(0, a, *zt, N, depth, tz, st);
         return;
     }
     tz = two_kernel_tz(tz, var);
     const dim3 grid = stencil_grid(a, N, tz, var), block(kBX, kBY);
-    if (var == 5) {
+    if (var == 10) {
+        PSV_N5_CASES(k_stress10, )
+    } else if (var == 5) {
         PSV_N5_CASES(k_stress5, )
     } else if (var == 3) {
         PSV_N_CASES(k_stress2, , 6)
@@ -1300,16 +1390,19 @@ void launch_stress(const KArgs &a, int N, int tz, int var, hipStream_t st, const
 }
 
 void launch_velocity(const KArgs &a, int N, int tz, int var, hipStream_t st, const ZTab *zt, int depth) {
-    // variants 5 and 6 need the strip-row table and correlate in adjoint launches only
+    // variants 5, 6 and 10 need the strip-row table and correlate in adjoint launches only
     if (var >= 5 && (!zt || (a.grad && !a.displ))) var = 2;
     if (var == 6) {
         if (launch6(1, a, *zt, N, depth, tz, st)) return;
         var = 2;   // a launch mode variant 6 has no kernel for
     }
-    if (var == 5) {
-        const int tz5 = two_kernel_tz(tz, 5);
-        if (launch_velocity5(a, zt, N, tz5, stencil_grid(a, N, tz5, 5), dim3(kBX, kBY), st)) return;
-        var = 2;   // a launch mode variant 5 has no kernel for
+    if (var == 5 || var == 10) {
+        const int tz5 = two_kernel_tz(tz, var);
+        const dim3 g5 = stencil_grid(a, N, tz5, var);
+        if (var == 10 ? launch_velocity5<true>(a, zt, N, tz5, g5, dim3(kBX, kBY), st)
+                      : launch_velocity5<false>(a, zt, N, tz5, g5, dim3(kBX, kBY), st))
+            return;
+        var = 2;   // a launch mode variants 5 and 10 have no kernel for
     }
     tz = two_kernel_tz(tz, var);
     const dim3 grid = stencil_grid(a, N, tz, var), block(kBX, kBY);

@@ -146,8 +146,12 @@ typedef struct {
                                5 = variant 2 with wave-uniform addressing and ordered loads,
                                6 = variant 5 with every load issued one row ahead,
                                7 = variant 4 built as variant 6 (variant 4's conditions),
-                               8 = variant 7 with a block's wavefronts side by side in x
-                               (5 to 8 need z CPML strips of at most 64 rows, at the
+                               8 = variant 7 with a block's wavefronts side by side in x,
+                               9 = variant 8 launched as one or two waves of resident
+                                   blocks with the rows shared evenly,
+                               10 = variant 5 on full 64-column tiles from column 0 (rows
+                                   aligned, the x halo from one extra load per row)
+                               (5 to 10 need z CPML strips of at most 64 rows, at the
                                top and bottom of the grid) */
     int32_t work_order;     /* order of the stencil tiles over the grid: 0 = tuned,
                                1 = shot fastest then z then x, 2 = shot fastest then x then z,

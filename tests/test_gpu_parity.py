@@ -85,7 +85,7 @@ def test_shot_gradient_matches_reference_golden(psv, name):
 
 
 @pytest.mark.parametrize("surface", [True, False])
-@pytest.mark.parametrize("variant", [1, 2, 4, 5, 6, 7, 8, 9])
+@pytest.mark.parametrize("variant", [1, 2, 4, 5, 6, 7, 8, 9, 10])
 def test_order8_forward_vs_oracle(psv, O, surface, variant):
     c = cases.small_case(O, surface=surface, nshot=2)
     med = O.mat_av2(c)
@@ -100,7 +100,7 @@ def test_order8_forward_vs_oracle(psv, O, surface, variant):
         assert e < 1e-4
 
 
-@pytest.mark.parametrize("variant", [1, 2, 4, 5, 6, 7, 8, 9])
+@pytest.mark.parametrize("variant", [1, 2, 4, 5, 6, 7, 8, 9, 10])
 def test_order8_gradient_vs_oracle(psv, O, variant):
     c = cases.small_case(O, nshot=2, nt=450)
     med_true = O.mat_av2(c)
@@ -126,7 +126,7 @@ def test_order8_gradient_vs_oracle(psv, O, variant):
             assert e < 2e-3
 
 
-@pytest.mark.parametrize("variant", [1, 4, 5, 6, 7, 8, 9])
+@pytest.mark.parametrize("variant", [1, 4, 5, 6, 7, 8, 9, 10])
 def test_checkpointed_equals_full_storage(psv, O, variant):
     c = cases.small_case(O, nshot=2, nt=300, snap_dt=2)
     med_true = O.mat_av2(c)
@@ -148,26 +148,30 @@ def test_checkpointed_equals_full_storage(psv, O, variant):
             np.testing.assert_array_equal(G[i], ref[1][i], err_msg=str(key))
 
 
-@pytest.mark.parametrize("variant,graph", [(1, 0), (7, 0), (8, 1), (5, 1), (9, 1)])
-def test_binomial_checkpoints_equal_full_storage(psv, O, variant, graph):
-    """Fewer checkpoint buffers than segment boundaries (ckpt_slots 1, 2, 5 for 22
-    segments of 14 steps, the last one short): the reversal advances from held
-    checkpoints on the binomial schedule and recomputes more (plan's recompute > 1),
-    and the misfits and gradient still equal full snapshot storage bit for bit
-    (the same kernels replay the same steps), with and without graph replay."""
-    c = cases.small_case(O, nshot=3, nt=302, snap_dt=2, uneven=True)
+@pytest.mark.parametrize("variant,graph,sdt,S", [(1, 0, 2, 14), (7, 0, 2, 14), (8, 1, 2, 14), (5, 1, 2, 14),
+                                                (9, 1, 2, 14), (10, 0, 3, 15), (10, 1, 3, 18), (5, 0, 3, 15),
+                                                (8, 0, 3, 15)])
+def test_binomial_checkpoints_equal_full_storage(psv, O, variant, graph, sdt, S):
+    """Fewer checkpoint buffers than segment boundaries (ckpt_slots 1, 2, 5 for the
+    segments of S steps, the last one short, snapshots every sdt-th step): the reversal
+    advances from held checkpoints on the binomial schedule and recomputes more (plan's
+    recompute > 1), and the misfits and gradient still equal full snapshot storage bit
+    for bit (the same kernels replay the same steps), with and without graph replay."""
+    nt = 302
+    c = cases.small_case(O, nshot=3, nt=nt, snap_dt=sdt, uneven=True)
     med_true = O.mat_av2(c)
     obs = cases.per_shot_obs(c, O, med_true)
     lam0, mu0, rho0 = cases.start_model(c)
     res = {}
+    units = (nt + S - 1) // S
     for storage, slots in (("full", 0), ("ckpt", 0), ("ckpt", 5), ("ckpt", 2), ("ckpt", 1)):
         st = psv._lib.STORE_FULL if storage == "full" else psv._lib.STORE_CHECKPOINT
-        with psv.Simulation(c, storage=st, ckpt_interval=14, ckpt_slots=slots, kernel_variant=variant,
+        with psv.Simulation(c, storage=st, ckpt_interval=S, ckpt_slots=slots, kernel_variant=variant,
                             use_graph=graph, batch=2) as sim:
             plan = sim.plan()
             if storage == "ckpt":
-                assert plan["ckpt_units"] == 22
-                assert plan["ckpt_slots"] == (21 if slots == 0 else slots)
+                assert plan["ckpt_units"] == units
+                assert plan["ckpt_slots"] == (units - 1 if slots == 0 else slots)
                 assert (plan["recompute"] > 1.0) == (slots > 0)
             _print(f"variant {variant} graph {graph} {storage} slots {slots}: {plan}")
             sim.set_model(lam0, mu0, rho0)
@@ -180,6 +184,39 @@ def test_binomial_checkpoints_equal_full_storage(psv, O, variant, graph):
         for i in range(3):
             np.testing.assert_array_equal(G[i], ref[1][i], err_msg=str(key))
     assert res[("ckpt", 1)][2] > res[("ckpt", 2)][2] > res[("ckpt", 5)][2] > res[("ckpt", 0)][2] == 1.0
+
+
+@pytest.mark.parametrize("variant", [5, 10])
+def test_binomial_checkpoints_many_units(psv, O, variant):
+    """The C4 sweep's shape of reversal on a small grid: 51 units of 15 steps (the last
+    one 5 steps), snapshots every 3rd step, one shot per launch, 50 (one per boundary),
+    32, 8 and 3 checkpoint buffers: every schedule gives full storage's misfit and
+    gradient bit for bit, and recomputes more steps the fewer buffers it holds."""
+    S, nunits, sdt = 15, 51, 3
+    nt = (nunits - 1) * S + 5
+    c = cases.small_case(O, nshot=1, nt=nt, snap_dt=sdt, nz=56, nx=80, npml=10)
+    obs = cases.per_shot_obs(c, O, O.mat_av2(c))
+    lam0, mu0, rho0 = cases.start_model(c)
+    res = {}
+    for storage, slots in (("full", 0), ("ckpt", 0), ("ckpt", 32), ("ckpt", 8), ("ckpt", 3)):
+        st = psv._lib.STORE_FULL if storage == "full" else psv._lib.STORE_CHECKPOINT
+        with psv.Simulation(c, storage=st, ckpt_interval=S, ckpt_slots=slots, kernel_variant=variant,
+                            tile_tz=16, work_order=1) as sim:
+            plan = sim.plan()
+            if storage == "ckpt":
+                assert plan["ckpt_units"] == nunits
+                assert plan["ckpt_slots"] == (nunits - 1 if slots == 0 else slots)
+            sim.set_model(lam0, mu0, rho0)
+            sim.set_observed(*obs[0])
+            m = sim.gradient()
+            res[(storage, slots)] = (m, sim.get_gradient(), plan["recompute"])
+            _print(f"variant {variant} {storage} slots {slots}: recompute {plan['recompute']}, misfit {m[0]:.17e}")
+    ref = res[("full", 0)]
+    for key, (m, G, _) in res.items():
+        np.testing.assert_array_equal(m, ref[0], err_msg=str(key))
+        for i in range(3):
+            np.testing.assert_array_equal(G[i], ref[1][i], err_msg=str(key))
+    assert res[("ckpt", 3)][2] > res[("ckpt", 8)][2] > res[("ckpt", 32)][2] > res[("ckpt", 0)][2] == 1.0
 
 
 def test_c4_window_checkpointed_equals_full_storage(psv, O):
@@ -212,7 +249,7 @@ def test_c4_window_checkpointed_equals_full_storage(psv, O):
             np.testing.assert_array_equal(G[i], ref[1][i], err_msg=str(key))
 
 
-@pytest.mark.parametrize("variant", [1, 4, 5, 6, 7, 8, 9])
+@pytest.mark.parametrize("variant", [1, 4, 5, 6, 7, 8, 9, 10])
 def test_batched_uneven_equals_per_shot(psv, O, variant):
     c = cases.small_case(O, nshot=4, nt=260, uneven=True)
     med_true = O.mat_av2(c)
@@ -240,6 +277,39 @@ def test_batched_uneven_equals_per_shot(psv, O, variant):
         for s, a in enumerate(sim.split(uz)):
             assert a.shape == obs[s][0].shape
             assert cases.rel_l2(a, obs[s][0]) < 1e-4
+
+
+@pytest.mark.parametrize("variant", [5, 8, 10])
+def test_checkpointed_gradient_independent_of_batch_and_tuning(psv, O, variant):
+    """Checkpointed storage (segments of 30 steps, snapshots every 3rd step): the misfits
+    and gradient of three uneven shots are the same bits whether the shots run 3, 2 + 1
+    or 1 at a time per launch, and whether the autotuner timed its candidates in the
+    context first (on the context's own buffers) or the kernel and work order were
+    given, so nothing the tuner leaves behind reaches a result."""
+    c = cases.small_case(O, nshot=3, nt=270, snap_dt=3, uneven=True)
+    med_true = O.mat_av2(c)
+    obs = cases.per_shot_obs(c, O, med_true)
+    lam0, mu0, rho0 = cases.start_model(c)
+    out, pin = {}, None
+    for batch, pinned in ((3, False), (1, True), (2, True), (1, False), (3, True)):
+        kw = dict(kernel_variant=variant)
+        if pinned:
+            kw.update(pin)
+        with psv.Simulation(c, batch=batch, storage=psv._lib.STORE_CHECKPOINT, ckpt_interval=30, **kw) as sim:
+            sim.set_model(lam0, mu0, rho0)
+            sim.set_observed([o[0] for o in obs], [o[1] for o in obs])
+            m = sim.gradient()
+            out[(batch, pinned)] = (m, sim.get_gradient())
+            if pin is None:
+                import re
+                g = re.search(r"chosen o(\d)/v(\d+)/tz(\d+)", sim.tuning()[1])
+                pin = dict(work_order=int(g.group(1)) + 1, tile_tz=int(g.group(3)))
+    ref = out[(3, False)]
+    for key, (m, G) in out.items():
+        np.testing.assert_array_equal(m, ref[0], err_msg=str(key))
+        for i in range(3):
+            np.testing.assert_array_equal(G[i], ref[1][i], err_msg=str(key))
+    _print(f"variant {variant}: batches 3, 2 + 1, 1 x 3, tuned or pinned {pin}: identical")
 
 
 def test_plain_form_matches_fused(psv, O):
@@ -369,6 +439,47 @@ def test_uniform_addressing_variant_matches_variant2(psv, O, fdorder, variant, b
         assert e < 1e-4 and e2 < 1e-4
 
 
+@pytest.mark.parametrize("fdorder,nx", [(2, 96), (4, 133), (8, 96), (8, 196)])
+@pytest.mark.parametrize("tz", [16, 32])
+def test_full_tiles_equal_variant5(psv, O, fdorder, nx, tz):
+    """Variant 10 runs variant 5's arithmetic on 64-column tiles from column 0 with the x
+    halo from a separate load (lane_off_h) instead of lanes given to it: every tap reads
+    the same value in the same order, so seismograms, stored snapshots, the wavefield,
+    misfits, the gradient and the misfit-derivative sums equal variant 5's bit for bit.
+    Widths where nx - N is, and is not, a multiple of 64 (a last tile with lanes past
+    the grid); CPML on the left and right (x strips inside the first and last tiles);
+    uneven shots in one batch."""
+    c = cases.small_case(O, nshot=3, nt=200, fdorder=fdorder, uneven=True, nx=nx)
+    med_true = O.mat_av2(c)
+    obs = cases.per_shot_obs(c, O, med_true)
+    lam0, mu0, rho0 = cases.start_model(c)
+    out = {}
+    for v in (5, 10):
+        with psv.Simulation(c, kernel_variant=v, tile_tz=tz, storage=psv._lib.STORE_FULL) as sim:
+            seis = sim.forward()
+            wf = sim.wavefield(2, "sxx")
+            _, snaps = sim.forward_snapshots(1)
+            sim.set_model(lam0, mu0, rho0)
+            sim.set_observed([o[0] for o in obs], [o[1] for o in obs])
+            m = sim.gradient()
+            G = sim.get_gradient()
+            adj = [np.asarray(o, np.float64) * 1e-3 for o in obs[0]]
+            mg = sim.misfit_gradient(*adj, shot0=0, nshots=1)
+            out[v] = (seis, wf, snaps, m, G, mg)
+    a, b = out[10], out[5]
+    for i in range(2):
+        np.testing.assert_array_equal(a[0][i], b[0][i])
+    np.testing.assert_array_equal(a[1], b[1])
+    for k in b[2]:
+        np.testing.assert_array_equal(a[2][k], b[2][k], err_msg=k)
+    np.testing.assert_array_equal(a[3], b[3])
+    for i in range(3):
+        np.testing.assert_array_equal(a[4][i], b[4][i])
+        np.testing.assert_array_equal(a[5][i], b[5][i])
+    assert np.abs(b[0][0]).max() > 0 and np.abs(b[4][0]).max() > 0
+    _print(f"order {fdorder} nx {nx} tz {tz}: variant 10 equals variant 5 bit for bit")
+
+
 @pytest.mark.parametrize("variant,tz,fdorder", [(4, 0, 8), (7, 4, 8), (7, 8, 8), (7, 4, 2), (7, 8, 4),
                                                  (8, 4, 8), (8, 8, 8), (8, 4, 2)])
 def test_one_pass_step_matches_two_kernel_step(psv, O, variant, tz, fdorder):
@@ -429,7 +540,7 @@ def test_work_orders_agree_bitwise(psv, O):
             np.testing.assert_array_equal(ux, ref[2])
 
 
-@pytest.mark.parametrize("variant", [1, 4, 5, 6, 7, 8, 9])
+@pytest.mark.parametrize("variant", [1, 4, 5, 6, 7, 8, 9, 10])
 @pytest.mark.parametrize("storage", ["full", "ckpt"])
 def test_graph_replay_equals_eager(psv, O, variant, storage):
     """use_graph: chunks of time steps captured once per propagation and replayed

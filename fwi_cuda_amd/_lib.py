@@ -111,6 +111,7 @@ def load() -> C.CDLL:
         "fwi_iteration": (C.c_int32, [vp, _dp, _dp]),
         "fwi_tuning": (C.c_int32, [vp, _ip, C.c_char_p, C.c_int32]),
         "fwi_profile": (C.c_int32, [vp, C.c_int32]),
+        "fwi_profile_kernels": (C.c_int32, [vp, C.c_uint32]),
         "fwi_kernel_times": (C.c_int32, [vp, C.POINTER(FwiKernelTime)]),
         "fwi_sync": (C.c_int32, [vp]),
         "fwi_mat_av2_noalloc": (C.c_int32, [vp, vp, vp, C.c_int32, C.c_int32, vp, vp, vp, vp, vp, vp, vp]),
@@ -152,6 +153,6 @@ EXPORTED = ("fwi_abi_version", "fwi_strerror", "fwi_last_error", "fwi_default_op
             "fwi_gradient", "fwi_gradient_raw", "fwi_misfit_gradient", "fwi_zero_gradient", "fwi_get_gradient",
             "fwi_set_gradient",
             "fwi_taper_gradient", "fwi_misfit", "fwi_save_model", "fwi_update_model", "fwi_step_length",
-            "fwi_iteration", "fwi_tuning", "fwi_profile", "fwi_kernel_times", "fwi_sync", "fwi_forward_snapshots",
+            "fwi_iteration", "fwi_tuning", "fwi_profile", "fwi_profile_kernels", "fwi_kernel_times", "fwi_sync", "fwi_forward_snapshots",
             "fwi_mat_av2_noalloc", "fwi_checkpoint_plan", "fwi_set_model_device", "fwi_forward_device",
             "fwi_misfit_gradient_device", "fwi_misfit_gradient_exact", "fwi_pcg_direction", "fwi_smooth_gradient")

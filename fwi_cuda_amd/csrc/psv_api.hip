@@ -215,13 +215,13 @@ struct Timer {
     int kid;
     hipEvent_t a = nullptr;
     Timer(fwi_ctx *c_, int k) : c(c_), kid(k) {
-        if (c->prof_on) {
+        if (c->prof_on & (1 << kid)) {
             a = ev_get(c);
             hipEventRecord(a, c->st);
         }
     }
     ~Timer() {
-        if (c->prof_on) {
+        if (a) {
             hipEvent_t b = ev_get(c);
             hipEventRecord(b, c->st);
             c->ev_pending.push_back({kid, a, b});
@@ -2285,10 +2285,14 @@ int32_t fwi_tuning(const fwi_ctx *c, int32_t *tile_tz, char *log, int32_t log_le
 }
 
 int32_t fwi_profile(fwi_ctx *c, int32_t enable) {
+    return fwi_profile_kernels(c, enable ? (1u << FWI_K_COUNT) - 1 : 0u);
+}
+
+int32_t fwi_profile_kernels(fwi_ctx *c, uint32_t mask) {
     if (!c) return FWI_EINVAL;
     RETIF(sync(c));
-    c->prof_on = enable != 0;
-    if (enable) std::memset(c->times, 0, sizeof c->times);
+    c->prof_on = (int)(mask & ((1u << FWI_K_COUNT) - 1));
+    if (c->prof_on) std::memset(c->times, 0, sizeof c->times);
     return FWI_OK;
 }
 

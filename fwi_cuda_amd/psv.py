@@ -431,8 +431,16 @@ class Simulation:
         self._chk(self.lib.fwi_forward_snapshots(self.ctx, shot, dptr(uz), dptr(ux), *[dptr(a) for a in cubes]))
         return (uz, ux), dict(zip(("vz", "vx", "szz", "szx", "sxx"), cubes))
 
-    def profile(self, on=True):
-        self._chk(self.lib.fwi_profile(self.ctx, int(on)))
+    def profile(self, on=True, kernels=None):
+        """Per-launch HIP-event timing of every kernel class, or of the named ones only
+        (names of _lib.KERNELS, e.g. ("stress", "velocity", "step"))."""
+        if kernels is None or not on:
+            self._chk(self.lib.fwi_profile(self.ctx, int(on)))
+            return
+        mask = 0
+        for k in kernels:
+            mask |= 1 << _lib.KERNELS.index(k)
+        self._chk(self.lib.fwi_profile_kernels(self.ctx, mask))
 
     def kernel_times(self) -> dict:
         t = (FwiKernelTime * len(_lib.KERNELS))()   # FWI_K_COUNT entries

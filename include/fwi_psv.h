@@ -337,6 +337,10 @@ int32_t fwi_mat_av2_noalloc(const double *lam, const double *mu, const double *r
  * propagation has run) and the tuner's timings as text. */
 int32_t fwi_tuning(const fwi_ctx *ctx, int32_t *tile_tz, char *log, int32_t log_len);
 int32_t fwi_profile(fwi_ctx *ctx, int32_t enable);
+/* fwi_profile for the kernel classes whose bit (1u << FWI_K_*) is set in mask only (no
+ * counterpart in the reference): each timed launch costs two event records, so timing
+ * only the kernel whose duration matters disturbs the run less.  0 switches timing off. */
+int32_t fwi_profile_kernels(fwi_ctx *ctx, uint32_t mask);
 int32_t fwi_kernel_times(fwi_ctx *ctx, fwi_kernel_time times[FWI_K_COUNT]);
 int32_t fwi_sync(fwi_ctx *ctx);
 

@@ -491,8 +491,13 @@ bool use_graphs(const fwi_ctx *c) { return c->o.use_graph && !c->plain; }
 class ChunkGraph {
   public:
     explicit ChunkGraph(fwi_ctx *c) : c_(c) {}
+    // the last replay may still run when the pass returns: wait for it before the
+    // executable graph (and the kernel arguments it owns) is released
     ~ChunkGraph() {
-        if (exec_) hipGraphExecDestroy(exec_);
+        if (exec_) {
+            (void)hipStreamSynchronize(c_->st);
+            hipGraphExecDestroy(exec_);
+        }
     }
     ChunkGraph(const ChunkGraph &) = delete;
     ChunkGraph &operator=(const ChunkGraph &) = delete;
@@ -605,7 +610,7 @@ int autotune(fwi_ctx *c, int nb) {
     const bool order_forced = c->o.work_order >= 1 && c->o.work_order <= 4;
     // candidates (variant, rows per segment, prefetch depth); the one-pass step only where it applies
     std::vector<TuneVal> cand;
-    for (int var = 1; var <= 10; ++var) {
+    for (int var = 1; var <= 11; ++var) {
         if ((fv && var != fv) || (one_pass(var) && !c->fusable) || (var >= 5 && !c->ztab_ok)) continue;
         if (var == 9) {   // balanced: one or two waves of resident blocks
             for (int m : {1, 2})
@@ -620,7 +625,7 @@ int autotune(fwi_ctx *c, int nb) {
             continue;
         }
         const std::vector<int> tzs = var == 4   ? std::vector<int>{32, 64}
-                                     : var == 5 || var == 10 ? std::vector<int>{16, 32}
+                                     : var == 5 || var >= 10 ? std::vector<int>{16, 32}
                                                 : std::vector<int>{8, 16, 32, 64};
         for (int tz : tzs)
             if (c->o.tile_tz <= 0 || c->o.tile_tz == tz) cand.push_back({var, tz, 1, 0});
@@ -629,8 +634,8 @@ int autotune(fwi_ctx *c, int nb) {
         return c->fail(FWI_EINVAL, "no stencil kernel for kernel_variant %d, tile_tz %d%s", fv, c->o.tile_tz,
                        fv == 4   ? " (the one-pass step takes tile_tz 32 or 64, no free surface other than "
                                    "the top one at row fdorder/2, and sources and receivers inside the interior)"
-                       : fv == 5 || fv == 10
-                           ? " (variants 5 and 10 take tile_tz 16 or 32, and z CPML strips of at most 64 rows "
+                       : fv == 5 || fv >= 10
+                           ? " (variants 5, 10 and 11 take tile_tz 16 or 32, and z CPML strips of at most 64 rows "
                              "at the top and bottom of the grid)"
                        : fv == 6 ? " (variant 6 takes tile_tz 0, fdorder + 2 or 2 fdorder + 4, and z CPML "
                                    "strips of at most 64 rows at the top and bottom of the grid)"
@@ -1471,8 +1476,8 @@ int validate(fwi_ctx *c, const fwi_grid *g, const fwi_acq *q) {
         return c->fail(FWI_EINVAL,
                        "tile_tz %d (0, 8, 16, 32 or 64; variant 6: 0 or its row block times 1 or 2; "
                        "variants 7, 8: 0 or their row block times 4 or 8)", tt);
-    if (c->o.kernel_variant < 0 || c->o.kernel_variant > 10)
-        return c->fail(FWI_EINVAL, "kernel_variant %d (0..10)", c->o.kernel_variant);
+    if (c->o.kernel_variant < 0 || c->o.kernel_variant > 11)
+        return c->fail(FWI_EINVAL, "kernel_variant %d (0..11)", c->o.kernel_variant);
     if (c->o.kernel_variant == 9 && tt > 2) return c->fail(FWI_EINVAL, "variant 9 takes tile_tz 0, 1 or 2");
     if (c->o.work_order < 0 || c->o.work_order > 4)
         return c->fail(FWI_EINVAL, "work_order %d (0..4)", c->o.work_order);
@@ -1893,17 +1898,32 @@ int exact_gradient(fwi_ctx *c, int shot0, int nshots, const double *adj_z, const
     if (shot0 < 0 || nshots <= 0 || shot0 + nshots > c->nshot) return c->fail(FWI_EINVAL, "shot range");
     const int nz = c->g.nz, nx = c->g.nx, nt = c->g.nt;
     const size_t fs = c->fs();
-    // per shot: 30 float fields (state, memories, adjoint state, carries, scratch), the FP64
-    // media gradient (5) and the intermediates of every step (5 nt)
-    const size_t per_shot = fs * 4 * (30 + 5 * (size_t)nt) + fs * 8 * 5;
+    // Per shot: 30 float fields (state, memories, adjoint state, carries, scratch) and the
+    // intermediates of S steps (5 S fields); with S < nt also a checkpoint of the forward
+    // state (5 fields, 8 memories) at the start of each of the ceil(nt / S) segments, from
+    // which each segment's forward is recomputed before its reverse sweep.  S = nt (no
+    // recomputation) when that fits, else S ~ sqrt(13 nt / 5), the smallest footprint;
+    // storage FWI_STORE_CHECKPOINT with ckpt_interval > 0 asks for that S, FWI_STORE_FULL
+    // for S = nt.
+    auto per_shot_of = [&](int S_) {
+        const size_t nseg = (size_t)(nt + S_ - 1) / S_;
+        return fs * 4 * (30 + 5 * (size_t)S_ + (S_ < nt ? 13 * nseg : 0));
+    };
     size_t freeb = 0, total = 0;
     HIPCHK(hipMemGetInfo(&freeb, &total));
     const size_t avail = (size_t)(0.85 * (double)freeb);
     const size_t g_all = fs * 8 * 5 * (size_t)nshots;   // the media gradients of every shot
+    int S = nt;
+    if (c->o.storage == FWI_STORE_CHECKPOINT && c->o.ckpt_interval > 0)
+        S = std::min(c->o.ckpt_interval, nt);
+    else if (c->o.storage != FWI_STORE_FULL && per_shot_of(nt) + g_all > avail)
+        S = std::max(1, std::min(nt, (int)std::lround(std::sqrt(13.0 * nt / 5.0))));
+    const size_t per_shot = per_shot_of(S);
     if (per_shot + g_all > avail)
-        return c->fail(FWI_ENOMEM, "the exact adjoint keeps %.2f GB per shot (5 fields per step), %.2f GB free",
-                       per_shot / 1e9, avail / 1e9);
+        return c->fail(FWI_ENOMEM, "the exact adjoint keeps %.2f GB per shot (%d steps of intermediates), %.2f GB free",
+                       per_shot / 1e9, S, avail / 1e9);
     const int Bx = (int)std::min<size_t>((size_t)nshots, (avail - g_all) / per_shot);
+    const int nseg = (nt + S - 1) / S;
     std::vector<void *> mem;
     auto A = [&](size_t bytes) -> void * {
         void *p = nullptr;
@@ -1928,9 +1948,10 @@ int exact_gradient(fwi_ctx *c, int shot0, int nshots, const double *adj_z, const
     x.lam = c->lam; x.mu = c->mu; x.muzx = c->muzx; x.rhoz = c->rhoz; x.rhox = c->rhox;
     x.zp = c->zp; x.xp = c->xp;
     float *fl = static_cast<float *>(A(sizeof(float) * fs * Bx * 30));
-    float *hist = static_cast<float *>(A(sizeof(float) * fs * Bx * 5 * (size_t)nt));
+    float *hist = static_cast<float *>(A(sizeof(float) * fs * Bx * 5 * (size_t)S));
     double *G = static_cast<double *>(A(sizeof(double) * fs * 5 * nshots));
-    if (!fl || !hist || !G) {
+    float *ckp = S < nt ? static_cast<float *>(A(sizeof(float) * fs * Bx * 13 * (size_t)nseg)) : nullptr;
+    if (!fl || !hist || !G || (S < nt && !ckp)) {
         release();
         return c->fail(FWI_ENOMEM, "exact adjoint buffers");
     }
@@ -1941,6 +1962,7 @@ int exact_gradient(fwi_ctx *c, int shot0, int nshots, const double *adj_z, const
     for (int k = 0; k < 30; ++k) *slots[k] = fl + (size_t)k * fs * Bx;
     for (int k = 0; k < 4; ++k) x.D[k] = x.Dv[k] = x.E[k];   // one scratch set serves all three
     x.hist = hist;
+    x.hist_base = 0;
     const int rb = c->rec_off[shot0], rows = c->rec_off[shot0 + nshots] - rb;
     const size_t nrow = (size_t)rows * nt, off = (size_t)rb * nt;
     // the receivers' adjoint sources, in res_*: rc x reverse cumulative sum of dt adj
@@ -1966,7 +1988,24 @@ int exact_gradient(fwi_ctx *c, int shot0, int nshots, const double *adj_z, const
         ka.P = c->P; ka.fs = fs; ka.dt = x.dt; ka.rhoz = c->rhoz; ka.rhox = c->rhox;
         ka.vz = x.vz; ka.vx = x.vx;
         const SrcArgs fsa = src_args(c, s0, nb);
+        // the forward state of segment k (5 fields, 8 memories of nb shots) to or from ckp
+        float *const state[13] = {x.vz, x.vx, x.szz, x.szx, x.sxx, x.pv[0], x.pv[1], x.pv[2], x.pv[3],
+                                  x.ps[0], x.ps[1], x.ps[2], x.ps[3]};
+        auto ckpt = [&](int k, bool save) {
+            for (int i = 0; i < 13; ++i) {
+                float *slot = ckp + ((size_t)k * 13 + i) * fs * Bx;
+                (void)hipMemcpyAsync(save ? slot : state[i], save ? state[i] : slot, sizeof(float) * fs * nb,
+                                     hipMemcpyDeviceToDevice, c->st);
+            }
+        };
+        // forward over all steps (each step's intermediates at its slot of the current
+        // segment), saving the state at each segment start; the last segment's
+        // intermediates are then in place for its reverse sweep
         for (int n = 0; n < nt; ++n) {
+            if (n % S == 0) {
+                x.hist_base = n;
+                if (S < nt) ckpt(n / S, true);
+            }
             x_forward_step(x, n, isurf, c->st);
             launch_inject(ka, fsa, n, c->st);
         }
@@ -1977,9 +2016,20 @@ int exact_gradient(fwi_ctx *c, int shot0, int nshots, const double *adj_z, const
         kb.vz = x.bvz;
         kb.vx = x.bvx;
         const SrcArgs *t0 = c->stf_type == 0 ? &fsa : nullptr;
-        for (int n = nt - 1; n >= 0; --n) {
-            launch_inject(kb, asa, n, c->st);
-            x_adjoint_step(x, n, isurf, t0, c->st);
+        for (int k = nseg - 1; k >= 0; --k) {
+            const int n0 = k * S, n1 = std::min(n0 + S, nt);
+            x.hist_base = n0;
+            if (k < nseg - 1) {   // recompute the segment's intermediates from its checkpoint
+                ckpt(k, false);
+                for (int n = n0; n < n1; ++n) {
+                    x_forward_step(x, n, isurf, c->st);
+                    launch_inject(ka, fsa, n, c->st);
+                }
+            }
+            for (int n = n1 - 1; n >= n0; --n) {
+                launch_inject(kb, asa, n, c->st);
+                x_adjoint_step(x, n, isurf, t0, c->st);
+            }
         }
         if (hipGetLastError() != hipSuccess) r = c->fail(FWI_EHIP, "exact adjoint launch");
     }

@@ -51,23 +51,27 @@ __device__ __forceinline__ StepView step_view(const KArgs &a, int it) {
 // OUTW-column tile).
 // FULL: tiles of OUTW = kBX columns starting at column 0 (aligned rows), covering the
 // interior columns [N, nx - N); otherwise tiles of OUTW output columns from column N.
-template <int TZ, int OUTW = kBX, bool XW = false, bool FULL = false>
+// SB: the kBY wavefronts of a block are kBY shots of one tile and one z segment (s
+// returns the group of kBY shots, by the segment): they read the same media rows at
+// nearly the same time, so those loads are served once per block from the CU's cache.
+template <int TZ, int OUTW = kBX, bool XW = false, bool FULL = false, bool SB = false>
 __device__ __forceinline__ void tile_of(const KArgs &a, int N, int &bx, int &by, int &s) {
     const int xspan = FULL ? a.nx - N : a.nx - 2 * N;
     const int gx = XW ? ((xspan + OUTW - 1) / OUTW + kBY - 1) / kBY : (xspan + OUTW - 1) / OUTW;
-    const int gy = XW ? (a.nz - 2 * N + TZ - 1) / TZ : ((a.nz - 2 * N + TZ - 1) / TZ + kBY - 1) / kBY;
-    const int n = gx * gy * a.B;
+    const int gy = XW || SB ? (a.nz - 2 * N + TZ - 1) / TZ : ((a.nz - 2 * N + TZ - 1) / TZ + kBY - 1) / kBY;
+    const int ns = SB ? (a.B + kBY - 1) / kBY : a.B;
+    const int n = gx * gy * ns;
     const int L = blockIdx.x, xcd = L & 7, loc = L >> 3;
     const int q = n >> 3, r = n & 7;
     const int W = xcd * q + min(xcd, r) + loc;
     if (a.order == 0) {          // shot fastest, then z segments, then x tiles
-        s = W % a.B;
-        const int t = W / a.B;
+        s = W % ns;
+        const int t = W / ns;
         by = t % gy;
         bx = t / gy;
     } else if (a.order == 1) {   // shot fastest, then x tiles, then z segments
-        s = W % a.B;
-        const int t = W / a.B;
+        s = W % ns;
+        const int t = W / ns;
         bx = t % gx;
         by = t / gx;
     } else if (a.order == 2) {   // x tiles fastest, then z segments, shots outermost
@@ -78,8 +82,8 @@ __device__ __forceinline__ void tile_of(const KArgs &a, int N, int &bx, int &by,
     } else {                     // x tiles fastest, then shots, z segments outermost
         bx = W % gx;
         const int t = W / gx;
-        s = t % a.B;
-        by = t / a.B;
+        s = t % ns;
+        by = t / ns;
     }
 }
 

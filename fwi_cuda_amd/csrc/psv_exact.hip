@@ -81,7 +81,7 @@ __global__ void xk_stress(const XArgs x, int n) {
     x.szz[o] += dt * (l * (d3 + d0) + 2.f * u * d0);
     x.szx[o] += dt * x.muzx[m] * (d2 + d1);
     x.sxx[o] += dt * (l * (d3 + d0) + 2.f * u * d3);
-    float *h = x.hist + ((size_t)n * x.B + s) * 5 * x.fs;
+    float *h = x.hist + ((size_t)(n - x.hist_base) * x.B + s) * 5 * x.fs;
     h[0 * x.fs + c] = d0;
     h[1 * x.fs + c] = d3;
     h[2 * x.fs + c] = d2 + d1;
@@ -157,7 +157,7 @@ __global__ void xk_velocity(const XArgs x, int n) {
     const float Sz = d2 + d0, Sx = d3 + d1;
     x.vz[o] += x.dt * x.rhoz[c] * Sz;
     x.vx[o] += x.dt * x.rhox[c] * Sx;
-    float *h = x.hist + ((size_t)n * x.B + s) * 5 * x.fs;
+    float *h = x.hist + ((size_t)(n - x.hist_base) * x.B + s) * 5 * x.fs;
     h[3 * x.fs + c] = Sz;
     h[4 * x.fs + c] = Sx;
 }
@@ -189,7 +189,7 @@ __global__ void xa_velocity(const XArgs x, int n) {
         x.D[0][o] = x.D[1][o] = x.D[2][o] = x.D[3][o] = 0.f;
         return;
     }
-    const float *h = x.hist + ((size_t)n * x.B + s) * 5 * x.fs;
+    const float *h = x.hist + ((size_t)(n - x.hist_base) * x.B + s) * 5 * x.fs;
     const float az = x.bvz[o], ax = x.bvx[o];
     double *G = x.G + (size_t)s * 5 * x.fs;
     G[3 * x.fs + c] += (double)az * x.dt * h[3 * x.fs + c];
@@ -280,7 +280,7 @@ __global__ void xa_surface(const XArgs x, int side, int is, int n) {
     const double l = x.lam[r], u = x.mu[r], den = l + 2.0 * u;
     const double f = (l * u + u * u) / den, f_l = u * u / (den * den),
                  f_u = (l * l + 2.0 * l * u + 2.0 * u * u) / (den * den);
-    const float *h = x.hist + ((size_t)n * x.B + s) * 5 * x.fs;
+    const float *h = x.hist + ((size_t)(n - x.hist_base) * x.B + s) * 5 * x.fs;
     const double a = side < 2 ? x.bxx[f0 + r] : x.bzz[f0 + r];
     const double e = side < 2 ? h[1 * x.fs + r] : h[0 * x.fs + r];
     float *Ds = side < 2 ? x.Dv[3] : x.Dv[0];
@@ -307,7 +307,7 @@ __global__ void xa_stress(const XArgs x, int n) {
     int s, iz, ix;
     if (!cell_of(x, (size_t)blockIdx.x * blockDim.x + threadIdx.x, s, iz, ix) || !interior(x, iz, ix)) return;
     const size_t c = (size_t)iz * x.P + ix, o = (size_t)s * x.fs + c;
-    const float *h = x.hist + ((size_t)n * x.B + s) * 5 * x.fs;
+    const float *h = x.hist + ((size_t)(n - x.hist_base) * x.B + s) * 5 * x.fs;
     const float zz = x.bzz[o], zx = x.bzx[o], xx = x.bxx[o];
     const float e_zz = h[c], e_xx = h[x.fs + c], e_zx = h[2 * x.fs + c];
     const float l = x.lam[c], u = x.mu[c], mzx = x.muzx[c], dt = x.dt;

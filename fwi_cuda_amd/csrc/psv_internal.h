@@ -190,7 +190,8 @@ struct XArgs {
     float *vz, *vx, *szz, *szx, *sxx;      // forward state
     float *pv[4], *ps[4];                  // forward CPML memories, stress / velocity pass (slots
                                            // d_z vz|szz, d_z vx|szx, d_x vz|szx, d_x vx|sxx)
-    float *hist;                           // [nt][B][5][nz][P]
+    float *hist;                           // [steps][B][5][nz][P]: step n at slot n - hist_base
+    int hist_base;
     float *bvz, *bvx, *bzz, *bzx, *bxx;    // adjoint state
     float *cv[4], *cs[4];                  // adjoint CPML carries, stress / velocity pass
     float *E[4], *D[4], *Dv[4];            // scratch (may be one set): forward strain rates of a

@@ -150,8 +150,10 @@ typedef struct {
                                9 = variant 8 launched as one or two waves of resident
                                    blocks with the rows shared evenly,
                                10 = variant 5 on full 64-column tiles from column 0 (rows
-                                   aligned, the x halo from one extra load per row)
-                               (5 to 10 need z CPML strips of at most 64 rows, at the
+                                   aligned, the x halo from one extra load per row),
+                               11 = variant 10 with a block's four wavefronts on four
+                                   shots of one tile and segment (shared media rows)
+                               (5 to 11 need z CPML strips of at most 64 rows, at the
                                top and bottom of the grid) */
     int32_t work_order;     /* order of the stencil tiles over the grid: 0 = tuned,
                                1 = shot fastest then z then x, 2 = shot fastest then x then z,

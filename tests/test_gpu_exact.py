@@ -6,7 +6,7 @@ gradient (FP32 forward and adjoint fields, FP64 products and sums) is compared a
 chain rule through mat_av2, inside the inversion box: rel-L2 <= 1e-4 per parameter, at
 orders 2, 4 and 8, CPML or a free surface (top, or four sides with image rows), type-0
 and type-1 sources, one shot or two shots with uneven receivers, host and device
-pointers; the dot-product test <J dm, y> = <dm, J^T y> with J dm from central
+pointers, and with the forward recomputed from checkpoints; the dot-product test <J dm, y> = <dm, J^T y> with J dm from central
 differences of the FP64 oracle's forward, relative 1e-5.  Then the autograd backward in exact mode against central differences of the
 misfit along smooth directions (relative step 1e-3), tolerance 1e-2.
 """
@@ -120,6 +120,29 @@ def test_exact_adjoint_dot_product(O, fdorder, surf, stf_type):
     _print(f"dot product order {fdorder} surface {surf} stf {stf_type}: <J dm, y> {lhs:.10e} (oracle differences), "
            f"<dm, J^T y> {rhs:.10e} (GPU), relative {rel:.2e}")
     assert rel < 1e-5
+
+
+@pytest.mark.parametrize("S", [17, 40])
+def test_exact_adjoint_checkpointed_equals_full_storage(O, S):
+    """The exact adjoint with the forward's intermediates of only S steps held (a checkpoint
+    of the forward state at each segment start, each segment recomputed before its reverse
+    sweep; the last one short) equals the run that holds every step's, bit for bit."""
+    from fwi_cuda_amd import psv
+    c = cases.adjoint_case(O, 8, "top", 0, nshot=2, uneven=True)
+    nt = int(c["nt"])
+    rng = np.random.default_rng(S)
+    out = {}
+    for st, kw in (("full", dict(storage=psv._lib.STORE_FULL)),
+                   ("ckpt", dict(storage=psv._lib.STORE_CHECKPOINT, ckpt_interval=S))):
+        with psv.Simulation(c, **kw) as sim:
+            rows = sim._rows(0, 2)
+            if not out:
+                y = rng.standard_normal((rows, nt)), rng.standard_normal((rows, nt))
+            out[st] = sim.misfit_gradient_exact(*y)
+    for a, b in zip(out["ckpt"], out["full"]):
+        np.testing.assert_array_equal(a, b)
+    assert np.abs(out["full"][0]).max() > 0
+    _print(f"exact adjoint, segments of {S} of {nt} steps recomputed: equal to full storage bit for bit")
 
 
 def test_exact_adjoint_device_pointers_equal_host(O):

@@ -85,7 +85,7 @@ def test_shot_gradient_matches_reference_golden(psv, name):
 
 
 @pytest.mark.parametrize("surface", [True, False])
-@pytest.mark.parametrize("variant", [1, 2, 4, 5, 6, 7, 8, 9, 10])
+@pytest.mark.parametrize("variant", [1, 2, 4, 5, 6, 7, 8, 9, 10, 11])
 def test_order8_forward_vs_oracle(psv, O, surface, variant):
     c = cases.small_case(O, surface=surface, nshot=2)
     med = O.mat_av2(c)
@@ -100,7 +100,7 @@ def test_order8_forward_vs_oracle(psv, O, surface, variant):
         assert e < 1e-4
 
 
-@pytest.mark.parametrize("variant", [1, 2, 4, 5, 6, 7, 8, 9, 10])
+@pytest.mark.parametrize("variant", [1, 2, 4, 5, 6, 7, 8, 9, 10, 11])
 def test_order8_gradient_vs_oracle(psv, O, variant):
     c = cases.small_case(O, nshot=2, nt=450)
     med_true = O.mat_av2(c)
@@ -126,7 +126,7 @@ def test_order8_gradient_vs_oracle(psv, O, variant):
             assert e < 2e-3
 
 
-@pytest.mark.parametrize("variant", [1, 4, 5, 6, 7, 8, 9, 10])
+@pytest.mark.parametrize("variant", [1, 4, 5, 6, 7, 8, 9, 10, 11])
 def test_checkpointed_equals_full_storage(psv, O, variant):
     c = cases.small_case(O, nshot=2, nt=300, snap_dt=2)
     med_true = O.mat_av2(c)
@@ -249,7 +249,7 @@ def test_c4_window_checkpointed_equals_full_storage(psv, O):
             np.testing.assert_array_equal(G[i], ref[1][i], err_msg=str(key))
 
 
-@pytest.mark.parametrize("variant", [1, 4, 5, 6, 7, 8, 9, 10])
+@pytest.mark.parametrize("variant", [1, 4, 5, 6, 7, 8, 9, 10, 11])
 def test_batched_uneven_equals_per_shot(psv, O, variant):
     c = cases.small_case(O, nshot=4, nt=260, uneven=True)
     med_true = O.mat_av2(c)
@@ -279,7 +279,7 @@ def test_batched_uneven_equals_per_shot(psv, O, variant):
             assert cases.rel_l2(a, obs[s][0]) < 1e-4
 
 
-@pytest.mark.parametrize("variant", [5, 8, 10])
+@pytest.mark.parametrize("variant", [5, 8, 10, 11])
 def test_checkpointed_gradient_independent_of_batch_and_tuning(psv, O, variant):
     """Checkpointed storage (segments of 30 steps, snapshots every 3rd step): the misfits
     and gradient of three uneven shots are the same bits whether the shots run 3, 2 + 1
@@ -439,22 +439,24 @@ def test_uniform_addressing_variant_matches_variant2(psv, O, fdorder, variant, b
         assert e < 1e-4 and e2 < 1e-4
 
 
-@pytest.mark.parametrize("fdorder,nx", [(2, 96), (4, 133), (8, 96), (8, 196)])
+@pytest.mark.parametrize("fdorder,nx,nshot", [(2, 96, 3), (4, 133, 5), (8, 96, 3), (8, 196, 4)])
 @pytest.mark.parametrize("tz", [16, 32])
-def test_full_tiles_equal_variant5(psv, O, fdorder, nx, tz):
+def test_full_tiles_equal_variant5(psv, O, fdorder, nx, nshot, tz):
     """Variant 10 runs variant 5's arithmetic on 64-column tiles from column 0 with the x
-    halo from a separate load (lane_off_h) instead of lanes given to it: every tap reads
-    the same value in the same order, so seismograms, stored snapshots, the wavefield,
-    misfits, the gradient and the misfit-derivative sums equal variant 5's bit for bit.
-    Widths where nx - N is, and is not, a multiple of 64 (a last tile with lanes past
-    the grid); CPML on the left and right (x strips inside the first and last tiles);
-    uneven shots in one batch."""
-    c = cases.small_case(O, nshot=3, nt=200, fdorder=fdorder, uneven=True, nx=nx)
+    halo from a separate load (lane_off_h) instead of lanes given to it, and variant 11
+    runs variant 10 with a block's four wavefronts on four shots of one tile: every tap
+    reads the same value in the same order, so seismograms, stored snapshots, the
+    wavefield, misfits, the gradient and the misfit-derivative sums equal variant 5's bit
+    for bit.  Widths where nx - N is, and is not, a multiple of 64 (a last tile with lanes
+    past the grid); CPML on the left and right (x strips inside the first and last tiles);
+    uneven shots in one batch, 3, 4 or 5 of them (a last group of four with idle
+    wavefronts)."""
+    c = cases.small_case(O, nshot=nshot, nt=200, fdorder=fdorder, uneven=True, nx=nx)
     med_true = O.mat_av2(c)
     obs = cases.per_shot_obs(c, O, med_true)
     lam0, mu0, rho0 = cases.start_model(c)
     out = {}
-    for v in (5, 10):
+    for v in (5, 10, 11):
         with psv.Simulation(c, kernel_variant=v, tile_tz=tz, storage=psv._lib.STORE_FULL) as sim:
             seis = sim.forward()
             wf = sim.wavefield(2, "sxx")
@@ -466,18 +468,20 @@ def test_full_tiles_equal_variant5(psv, O, fdorder, nx, tz):
             adj = [np.asarray(o, np.float64) * 1e-3 for o in obs[0]]
             mg = sim.misfit_gradient(*adj, shot0=0, nshots=1)
             out[v] = (seis, wf, snaps, m, G, mg)
-    a, b = out[10], out[5]
-    for i in range(2):
-        np.testing.assert_array_equal(a[0][i], b[0][i])
-    np.testing.assert_array_equal(a[1], b[1])
-    for k in b[2]:
-        np.testing.assert_array_equal(a[2][k], b[2][k], err_msg=k)
-    np.testing.assert_array_equal(a[3], b[3])
-    for i in range(3):
-        np.testing.assert_array_equal(a[4][i], b[4][i])
-        np.testing.assert_array_equal(a[5][i], b[5][i])
+    b = out[5]
+    for v in (10, 11):
+        a = out[v]
+        for i in range(2):
+            np.testing.assert_array_equal(a[0][i], b[0][i], err_msg=f"variant {v}")
+        np.testing.assert_array_equal(a[1], b[1], err_msg=f"variant {v}")
+        for k in b[2]:
+            np.testing.assert_array_equal(a[2][k], b[2][k], err_msg=f"variant {v} {k}")
+        np.testing.assert_array_equal(a[3], b[3], err_msg=f"variant {v}")
+        for i in range(3):
+            np.testing.assert_array_equal(a[4][i], b[4][i], err_msg=f"variant {v}")
+            np.testing.assert_array_equal(a[5][i], b[5][i], err_msg=f"variant {v}")
     assert np.abs(b[0][0]).max() > 0 and np.abs(b[4][0]).max() > 0
-    _print(f"order {fdorder} nx {nx} tz {tz}: variant 10 equals variant 5 bit for bit")
+    _print(f"order {fdorder} nx {nx} shots {nshot} tz {tz}: variants 10 and 11 equal variant 5 bit for bit")
 
 
 @pytest.mark.parametrize("variant,tz,fdorder", [(4, 0, 8), (7, 4, 8), (7, 8, 8), (7, 4, 2), (7, 8, 4),
@@ -540,7 +544,7 @@ def test_work_orders_agree_bitwise(psv, O):
             np.testing.assert_array_equal(ux, ref[2])
 
 
-@pytest.mark.parametrize("variant", [1, 4, 5, 6, 7, 8, 9, 10])
+@pytest.mark.parametrize("variant", [1, 4, 5, 6, 7, 8, 9, 10, 11])
 @pytest.mark.parametrize("storage", ["full", "ckpt"])
 def test_graph_replay_equals_eager(psv, O, variant, storage):
     """use_graph: chunks of time steps captured once per propagation and replayed

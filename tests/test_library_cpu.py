@@ -143,7 +143,7 @@ def test_invalid_options_are_rejected_before_any_device_work():
     from fwi_cuda_amd import _lib, psv
     O.load("oracle")
     c = cases.small_case(O, nshot=1, nt=10)
-    for kw in ({"tile_tz": 5}, {"kernel_variant": 11}, {"work_order": 5}, {"kernel_variant": 9, "tile_tz": 3},
+    for kw in ({"tile_tz": 5}, {"kernel_variant": 12}, {"work_order": 5}, {"kernel_variant": 9, "tile_tz": 3},
                {"ckpt_slots": -1}, {"smooth_hfs": 1}):
         with pytest.raises(_lib.FwiError) as e:
             psv.Simulation(c, **kw)
@@ -230,6 +230,7 @@ def test_bench_names_the_tuned_instance_and_finds_its_traffic(tmp_path):
     assert bench.kernel_instantiation("velocity", "chosen o1/v5/tz16/d1; ", 4) == "k_velocity5<4, 16, *>"
     assert bench.kernel_instantiation("velocity", "chosen o1/v10/tz32/d1; ", 4) == "k_velocity10<4, 32, *>"
     assert bench.kernel_instantiation("stress", "chosen o1/v10/tz16/d1; ", 4) == "k_stress10<4, 16>"
+    assert bench.kernel_instantiation("velocity", "chosen o0/v11/tz16/d1; ", 4) == "k_velocity11<4, 16, *>"
     assert bench.kernel_instantiation("stress", "chosen o2/v2/tz16/d1; ", 4) == "k_stress2<4, 16, 1>"
     f = tmp_path / "traffic.txt"
     f.write_text("header\n"

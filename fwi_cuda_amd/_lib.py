@@ -20,7 +20,7 @@ STATUS = {0: "FWI_OK", -1: "FWI_EINVAL", -2: "FWI_ENOMEM", -3: "FWI_EHIP", -4: "
 STORE_AUTO, STORE_FULL, STORE_CHECKPOINT = 0, 1, 2
 FORM_FUSED, FORM_PLAIN = 0, 1
 PARAM_LAME, PARAM_VELOCITY = 0, 1
-ABI_VERSION = 3
+ABI_VERSION = 4
 KERNELS = ("stress", "velocity", "surface", "source", "receiver", "store", "other", "step", "graph")
 
 
@@ -46,7 +46,8 @@ class FwiOpts(C.Structure):
                 ("use_graph", C.c_int32), ("tile_tz", C.c_int32), ("device", C.c_int32),
                 ("check_finite", C.c_int32), ("kernel_variant", C.c_int32), ("work_order", C.c_int32),
                 ("param", C.c_int32), ("vel_bounds", C.c_double * 4), ("pcg", C.c_int32), ("smooth_hfs", C.c_int32),
-                ("ckpt_slots", C.c_int32)]
+                ("ckpt_slots", C.c_int32),
+                ("seis_fp64", C.c_int32)]
 
 
 class FwiKernelTime(C.Structure):

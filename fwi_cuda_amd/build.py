@@ -26,7 +26,7 @@ LIB = PKG / "libfwi_psv.so"
 DRIVER = PKG / "fwi_psv"
 
 HIP_SOURCES = ["psv_stencil6.hip", "psv_step7.hip", "psv_kernels.hip", "psv_misc.hip", "psv_plain.hip",
-               "psv_api.hip", "psv_noalloc.hip", "psv_exact.hip"]
+               "psv_api.hip", "psv_noalloc.hip", "psv_exact.hip", "psv_fp64.hip"]
 CPP_SOURCES = ["psv_preproc.cpp"]
 DRIVER_SOURCES = ["host/psv_io.cpp", "host/psv_main.cpp"]
 

@@ -14,6 +14,8 @@
 //   --grad-sign S  -1: the CPU path's gradient sign (default), +1: the GPU path's
 //   --batch B      shots per launch (default: as many as memory allows)
 //   --storage full|checkpoint|auto
+//   --seis-fp64    seismograms, misfits and adjoint sources from an FP64 propagation
+//                  (the reference's precision for them; the wavefields stay FP32)
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -36,13 +38,14 @@ int die(const char *what, int status, const fwi_ctx *ctx) {
 
 int main(int argc, char **argv) {
     std::string dir = ".";
-    int niter = 10, grad_sign = -1, batch = 0, storage = FWI_STORE_AUTO;
+    int niter = 10, grad_sign = -1, batch = 0, storage = FWI_STORE_AUTO, seis_fp64 = 0;
     for (int i = 1; i < argc; ++i) {
         std::string a = argv[i];
         auto next = [&]() -> const char * { return i + 1 < argc ? argv[++i] : ""; };
         if (a == "--niter") niter = std::atoi(next());
         else if (a == "--grad-sign") grad_sign = std::atoi(next());
         else if (a == "--batch") batch = std::atoi(next());
+        else if (a == "--seis-fp64") seis_fp64 = 1;
         else if (a == "--storage") {
             std::string s = next();
             storage = s == "full" ? FWI_STORE_FULL : s == "checkpoint" ? FWI_STORE_CHECKPOINT : FWI_STORE_AUTO;
@@ -129,6 +132,7 @@ int main(int argc, char **argv) {
     o.batch = batch;
     o.storage = (!m.fwinv && m.accu_save) ? FWI_STORE_FULL : storage;
     o.grad_sign = grad_sign;
+    o.seis_fp64 = seis_fp64;
     fwi_ctx *ctx = nullptr;
     if ((st = fwi_create(&g, &q, &o, &ctx))) return die("create", st, nullptr);
     if ((st = fwi_set_model(ctx, lam.data(), mu.data(), rho.data()))) return die("set_model", st, ctx);

@@ -125,6 +125,11 @@ struct fwi_ctx {
     float *ck = nullptr;                // [slots][13 arrays of the batch]
     float *rf_canon = nullptr;          // rf[0] as allocated: the recompute state's level order
     size_t ck_bytes_per_seg = 0;
+    // seis_fp64: the FP64 propagation's fields [15][B][nz][P] (5 fields, 8 CPML memories, 2
+    // strain rates), its media [5][nz][P] and CPML profiles [nz + nx][6]; media_ver counts
+    // model changes, d64_ver the one its media were made from
+    double *d64 = nullptr, *dmed = nullptr, *dprof = nullptr;
+    long long media_ver = 0, d64_ver = -1;
     int last_s0 = -1, last_nb = 0;
     // plain form: full-grid derivative scratch and CPML memory (one shot)
     bool plain = false;
@@ -1030,8 +1035,51 @@ int batches(const fwi_ctx *c, int shot0, int nshots, std::vector<std::pair<int, 
 
 // forward + residual + adjoint for one batch; leaves raw gradients in gr, each
 // shot's scaled by alpha[shot] (and the lam / mu sums by kGainLM)
+// The seismograms of shots [s0, s0 + nb) from the FP64 propagation into syn_* (seis_fp64).
+int run_forward64(fwi_ctx *c, int s0, int nb) {
+    const int nz = c->g.nz, nx = c->g.nx, P = c->P, nt = c->g.nt;
+    const size_t fs = c->fs();
+    if (!c->d64) return c->fail(FWI_ESTATE, "the FP64 propagation's buffers were not planned (seis_fp64)");
+    double *med[5];
+    for (int k = 0; k < 5; ++k) med[k] = c->dmed + (size_t)k * fs;
+    if (c->d64_ver != c->media_ver) {
+        d_media(c->m_lam, c->m_mu, c->m_rho, med[0], med[1], med[2], med[3], med[4], nz, nx, P, c->st);
+        c->d64_ver = c->media_ver;
+    }
+    DArgs d{};
+    d.nz = nz; d.nx = nx; d.P = P; d.B = nb; d.N = c->N;
+    d.pml_z = c->g.pml_z; d.pml_x = c->g.pml_x;
+    d.fs = fs;
+    d.dt = c->g.dt;
+    for (int k = 0; k < 6; ++k) {
+        d.cz[k] = k < c->N ? c->g.hc[k + 1] / c->g.dz : 0.0;
+        d.cx[k] = k < c->N ? c->g.hc[k + 1] / c->g.dx : 0.0;
+    }
+    d.lam = med[0]; d.mu = med[1]; d.muzx = med[2]; d.rhoz = med[3]; d.rhox = med[4];
+    d.zp = c->dprof;
+    d.xp = c->dprof + (size_t)6 * nz;
+    double **slot[15] = {&d.vz, &d.vx, &d.szz, &d.szx, &d.sxx, &d.pv[0], &d.pv[1], &d.pv[2], &d.pv[3],
+                         &d.ps[0], &d.ps[1], &d.ps[2], &d.ps[3], &d.E0, &d.E3};
+    for (int k = 0; k < 15; ++k) *slot[k] = c->d64 + (size_t)k * c->B * fs;
+    for (int k = 0; k < 13; ++k) HIPCHK(hipMemsetAsync(*slot[k], 0, sizeof(double) * nb * fs, c->st));
+    HIPCHK(hipMemsetAsync(c->acc, 0, sizeof(double) * 2 * c->nrec_tot, c->st));
+    const SrcArgs sa = src_args(c, s0, nb);
+    const RecArgs ra = rec_args(c, s0, nb);
+    int isurf[4];
+    for (int k = 0; k < 4; ++k) isurf[k] = c->g.isurf[k];
+    for (int it = 0; it < nt; ++it) {
+        d_forward_step(d, isurf, c->st);
+        d_inject(d, sa, it, c->st);
+        d_record(d, ra, it, c->st);
+    }
+    if (hipGetLastError() != hipSuccess) return c->fail(FWI_EHIP, "FP64 propagation launch");
+    return FWI_OK;
+}
+
 int batch_gradient(fwi_ctx *c, int s0, int nb, const double *adj_z, const double *adj_x, double *misfit_out) {
-    RETIF(run_forward(c, s0, nb, true, adj_z == nullptr, true));
+    const bool seis64 = c->o.seis_fp64 && adj_z == nullptr;   // residuals from the FP64 propagation
+    RETIF(run_forward(c, s0, nb, true, adj_z == nullptr && !seis64, true));
+    if (seis64) RETIF(run_forward64(c, s0, nb));
     const int r0 = c->rec_off[s0], r1 = c->rec_off[s0 + nb];
     const size_t rows = (size_t)(r1 - r0) * c->g.nt, off = (size_t)r0 * c->g.nt;
     if (adj_z && c->adj_on_device) {   // the caller's device arrays, read by a kernel
@@ -1057,6 +1105,7 @@ int batch_gradient(fwi_ctx *c, int s0, int nb, const double *adj_z, const double
 }
 
 int refresh_media(fwi_ctx *c) {
+    ++c->media_ver;
     dev_mat_av(c->m_lam, c->m_mu, c->m_rho, c->lam, c->mu, c->muzx, c->rhoz, c->rhox, c->red, c->g.nz, c->g.nx,
                c->P, c->st);
     HIPCHK(hipMemcpyAsync(c->C, c->red, sizeof(double) * 3, hipMemcpyDeviceToHost, c->st));
@@ -1067,7 +1116,7 @@ int refresh_media(fwi_ctx *c) {
 // planning and allocation
 // ---------------------------------------------------------------------------
 struct Need {
-    size_t shared, per_shot;
+    size_t shared, per_shot, per_shot_fp64;
 };
 
 Need need_bytes(const fwi_ctx *c, int storage, int S, int slots) {
@@ -1077,8 +1126,13 @@ Need need_bytes(const fwi_ctx *c, int storage, int S, int slots) {
     n.shared = 5 * fs * 4 + 9 * nz * nx * 8 + 2 * nz * nx * 8 + (size_t)c->nrec_tot * nt * 8 * 6 +
                (size_t)c->nsrc_tot * nt * 8 * 2 + (64 << 20);
     if (c->o.param == FWI_PARAM_VELOCITY) n.shared += 4 * nz * nx * 8;
+    if (c->o.seis_fp64) {   // the FP64 propagation: media and profiles, 15 fields per shot
+        n.shared += 5 * fs * 8 + 6 * (nz + nx) * 8;
+        n.per_shot_fp64 = 15 * fs * 8;
+    }
     const size_t state_ck = 5 * fs * 4 + 4 * (c->psiz() + c->psix()) * 4;
-    n.per_shot = 9 * fs * 4 + 8 * (c->psiz() + c->psix()) * 4 + 5 * snapcell * 4 + 5 * snapcell * 8;
+    n.per_shot = 9 * fs * 4 + 8 * (c->psiz() + c->psix()) * 4 + 5 * snapcell * 4 + 5 * snapcell * 8 +
+                 n.per_shot_fp64;
     if (c->plain) n.per_shot += 12 * fs * 4;
     // second time level of the one-pass step (5 fields, 4 stress-side psi)
     const size_t level2 = c->fusable ? (5 * fs + 2 * (c->psiz() + c->psix())) * 4 : 0;
@@ -1290,6 +1344,11 @@ int allocate(fwi_ctx *c) {
         c->rf_canon = c->rf[0];
     }
     for (int i = 0; i < 5; ++i) c->snap[i] = F(B * c->snap_stride);
+    if (c->o.seis_fp64) {
+        c->d64 = D(15 * (size_t)B * fs);
+        c->dmed = D(5 * fs);
+        c->dprof = D(6 * ((size_t)c->g.nz + c->g.nx));
+    }
     if (c->plain) {
         PlainBufs &p = c->pb;
         float **m[12] = {&p.dz_z, &p.dx_z, &p.dz_x, &p.dx_x, &p.mvz_z, &p.mvx_z,
@@ -1421,6 +1480,16 @@ int setup_device(fwi_ctx *c, const fwi_acq *q) {
         px[i] = {hx[i], hx[nx + i], hx[2 * nx + i], hx[3 * nx + i], hx[4 * nx + i], hx[5 * nx + i], xmap[i], 0};
     RETIF(upload(c, c->zp, pz.data(), nz));
     RETIF(upload(c, c->xp, px.data(), nx));
+    if (c->dprof) {   // the FP64 propagation's profiles: b, a, 1/K, then at the half point, z then x
+        std::vector<double> dp(6 * ((size_t)nz + nx));
+        for (int i = 0; i < nz + nx; ++i) {
+            const int base = i < nz ? 0 : 6, j = i < nz ? i : i - nz;
+            const std::vector<double> *pr = c->prof + base;
+            const double v[6] = {pr[1][j], pr[0][j], 1.0 / pr[2][j], pr[4][j], pr[3][j], 1.0 / pr[5][j]};
+            for (int k = 0; k < 6; ++k) dp[(size_t)i * 6 + k] = v[k];
+        }
+        RETIF(upload(c, c->dprof, dp.data(), dp.size()));
+    }
     if (c->ztab_ok)
         for (int i = 0; i < nz; ++i)
             if (zmap[i] >= 0)
@@ -1482,6 +1551,7 @@ int validate(fwi_ctx *c, const fwi_grid *g, const fwi_acq *q) {
     if (c->o.work_order < 0 || c->o.work_order > 4)
         return c->fail(FWI_EINVAL, "work_order %d (0..4)", c->o.work_order);
     if (c->o.ckpt_slots < 0) return c->fail(FWI_EINVAL, "ckpt_slots %d < 0", c->o.ckpt_slots);
+    if (c->o.seis_fp64 != 0 && c->o.seis_fp64 != 1) return c->fail(FWI_EINVAL, "seis_fp64 %d (0 or 1)", c->o.seis_fp64);
     if (c->o.smooth_hfs != 0 && (c->o.smooth_hfs < 2 || c->o.smooth_hfs > 64))
         return c->fail(FWI_EINVAL, "smooth_hfs %d (0, or 2..64)", c->o.smooth_hfs);
     if (c->o.param != FWI_PARAM_LAME && c->o.param != FWI_PARAM_VELOCITY)
@@ -1725,7 +1795,8 @@ int32_t fwi_forward(fwi_ctx *c, int32_t shot0, int32_t nshots, double *seis_z, d
     if (batches(c, shot0, nshots, bs) != FWI_OK) return c->fail(FWI_EINVAL, "shot range");
     const int nt = c->g.nt;
     for (auto [s0, nb] : bs) {
-        RETIF(run_forward(c, s0, nb, false, true, true));
+        RETIF(run_forward(c, s0, nb, false, !c->o.seis_fp64, true));
+        if (c->o.seis_fp64) RETIF(run_forward64(c, s0, nb));
         const int r0 = c->rec_off[s0], r1 = c->rec_off[s0 + nb], rb = c->rec_off[shot0];
         const size_t n = (size_t)(r1 - r0) * nt;
         if (seis_z)
@@ -1746,7 +1817,8 @@ int32_t fwi_forward_device(fwi_ctx *c, int32_t shot0, int32_t nshots, double *se
     if (batches(c, shot0, nshots, bs) != FWI_OK) return c->fail(FWI_EINVAL, "shot range");
     const int nt = c->g.nt;
     for (auto [s0, nb] : bs) {
-        RETIF(run_forward(c, s0, nb, false, true, true));
+        RETIF(run_forward(c, s0, nb, false, !c->o.seis_fp64, true));
+        if (c->o.seis_fp64) RETIF(run_forward64(c, s0, nb));
         const int r0 = c->rec_off[s0], r1 = c->rec_off[s0 + nb], rb = c->rec_off[shot0];
         const size_t n = (size_t)(r1 - r0) * nt;
         if (seis_z) dev_copy(seis_z + (size_t)(r0 - rb) * nt, c->syn_z + (size_t)r0 * nt, n, c->st);
@@ -2172,7 +2244,10 @@ int32_t fwi_misfit(fwi_ctx *c, int32_t shot0, int32_t nshots, double *L2) {
     double total = 0;
     std::vector<double> m(c->B);
     for (auto [s0, nb] : bs) {
-        RETIF(run_forward(c, s0, nb, false, true, false));
+        if (c->o.seis_fp64)
+            RETIF(run_forward64(c, s0, nb));
+        else
+            RETIF(run_forward(c, s0, nb, false, true, false));
         dev_residual(c->syn_z, c->syn_x, c->obs_z, c->obs_x, c->res_z, c->res_x, c->rec_off_d + s0, nb, c->g.nt,
                      c->g.dt, c->misfit, c->st);
         HIPCHK(hipMemcpyAsync(m.data(), c->misfit, sizeof(double) * nb, hipMemcpyDeviceToHost, c->st));

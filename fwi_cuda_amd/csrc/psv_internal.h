@@ -208,6 +208,28 @@ void x_revcumsum(const double *adj_z, const double *adj_x, double *A_z, double *
 void x_media_chain(const XArgs &x, int nb, const double *mu, const double *rho, int z1, int z2, int x1, int x2,
                    double *d_lam, double *d_mu, double *d_rho, hipStream_t st);
 
+// ---- the FP64 propagation for seismograms (psv_fp64.hip) ------------------------------
+// The forward step of the exact path's kernels in FP64 (one thread per cell), with its own
+// FP64 media and CPML profiles, for the seismograms that the misfit and the adjoint
+// sources are made of when fwi_opts.seis_fp64 is set.
+struct DArgs {
+    int nz, nx, P, B, N, pml_z, pml_x;
+    size_t fs;                                        // nz * P
+    double dt;
+    double cz[6], cx[6];                              // hc[k] / dz, hc[k] / dx
+    const double *lam, *mu, *muzx, *rhoz, *rhox;      // [nz][P]
+    const double *zp, *xp;                            // [nz][6], [nx][6]: b, a, 1/K, then at the half point
+    double *vz, *vx, *szz, *szx, *sxx;                // [B][nz][P]
+    double *pv[4], *ps[4];                            // CPML memories, stress / velocity pass
+    double *E0, *E3;                                  // d_z vz, d_x vx after CPML (the surfaces use them)
+};
+// media from the FP64 model: lam, mu as they are, mat_av2's mu_zx, rho_zp, rho_xp
+void d_media(const double *lam, const double *mu, const double *rho, double *dlam, double *dmu, double *dmuzx,
+             double *drhoz, double *drhox, int nz, int nx, int P, hipStream_t st);
+void d_forward_step(const DArgs &d, const int isurf[4], hipStream_t st);
+void d_inject(const DArgs &d, const SrcArgs &sa, int it, hipStream_t st);
+void d_record(const DArgs &d, const RecArgs &ra, int it, hipStream_t st);
+
 // ---- small kernels on the model and gradient (psv_misc.hip) ------------------------
 // mat_av2: float lam, mu and averaged media [nz][P] from the FP64 model; the means of lam,
 // mu, rho over (nz - 1)(nx - 1) cells into means[3], summed in a fixed order from

@@ -152,7 +152,7 @@ class Simulation:
     def __init__(self, case: dict, *, batch=0, storage=_lib.STORE_AUTO, ckpt_interval=0, mem_budget=0,
                  form=_lib.FORM_FUSED, grad_sign=-1, tile_tz=0, device=0, check_finite=1, kernel_variant=0,
                  work_order=0, use_graph=0, param=_lib.PARAM_LAME, vel_bounds=None, ckpt_slots=0, pcg=0,
-                 smooth_hfs=0):
+                 smooth_hfs=0, seis_fp64=0):
         lib = _lib.load()
         self.lib = lib
         self.case = case
@@ -190,6 +190,7 @@ class Simulation:
         o.param = int(param)
         o.ckpt_slots = int(ckpt_slots)
         o.pcg, o.smooth_hfs = int(pcg), int(smooth_hfs)
+        o.seis_fp64 = int(seis_fp64)
         if vel_bounds is not None:
             o.vel_bounds[:] = [float(v) for v in vel_bounds]
         h = C.c_void_p()

@@ -52,7 +52,7 @@
 extern "C" {
 #endif
 
-#define FWI_PSV_ABI_VERSION 3
+#define FWI_PSV_ABI_VERSION 4
 
 enum fwi_status {
     FWI_OK = 0,
@@ -171,6 +171,10 @@ typedef struct {
                                as the budget allows).  With fewer than the ceil(nt / ckpt_interval) - 1
                                segment boundaries, the reversal recomputes segments from held
                                checkpoints on a binomial (revolve-style) schedule */
+    int32_t seis_fp64;      /* 1: the seismograms that fwi_forward returns and that misfits and
+                               adjoint sources are made of come from an FP64 propagation (FP64
+                               media, CPML and fields); the FP32 stencil still runs the wavefields
+                               the gradient correlates (ABI 4) */
 } fwi_opts;
 
 /* Per-kernel timing collected while profiling is on (HIP events on the

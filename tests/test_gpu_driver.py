@@ -67,6 +67,24 @@ def test_fwi_mode_reproduces_reference_misfits_and_failed_line_search(tmp_path):
     assert "Steplength estimation failed" in r.stdout
 
 
+def test_fwi_mode_with_fp64_seismograms_matches_the_reference_to_rounding(tmp_path):
+    """--seis-fp64: the misfits the driver prints come from the FP64 propagation, so they equal
+    the reference executable's in all six digits it prints (with FP32 seismograms the sixth
+    digit differs: 4.88013e-13 against 4.88014e-13), and the line search fails where the
+    reference's does."""
+    fx = load_fixture("fwi_bin")
+    _write(tmp_path, fx["files_fwi"], fx["start"]["lam"], fx["start"]["mu"], fx["start"]["rho"])
+    for s in range(2):
+        for comp in ("uz", "ux"):
+            fx["obs"][f"shot{s}_{comp}"].tofile(tmp_path / "io" / f"shot{s}_rtf_{comp}.bin")
+    r = _run(tmp_path, "--niter", "1", "--seis-fp64")
+    assert r.returncode == 2, (r.returncode, r.stdout[-2000:], r.stderr)
+    l2 = [float(line.split(",")[1]) for line in r.stdout.splitlines() if line.startswith("L2 NORM:")]
+    print("driver L2 norms with FP64 seismograms:", l2, "reference:", list(fx["l2_printed"]), flush=True)
+    assert [f"{v:.5e}" for v in l2] == [f"{v:.5e}" for v in fx["l2_printed"]]
+    assert "Steplength estimation failed" in r.stdout
+
+
 def test_fwi_mode_with_descent_sign_writes_models(tmp_path):
     """With the GPU twin's sign the same inversion descends: two iterations, the
     misfit drops and iter*_mat.bin (lam, mu, rho FP64) are written each iteration."""

@@ -120,7 +120,7 @@ def kernel_instantiation(kind, tune_log, N):
     base = {"stress": "k_stress", "velocity": "k_velocity"}[kind]
     if var == 1:
         return f"{base}<{N}, {tz}>"
-    if var in (5, 10, 11):   # 10, 11: variant 5's kernels on full 64-column tiles
+    if var in (5, 10, 11, 12):   # 10-12: variant 5's kernels on full 64-column tiles
         return f"{base}{var}<{N}, {tz}>" if kind == "stress" else f"{base}{var}<{N}, {tz}, *>"
     if var == 6:   # segments of tz = M blocks of 2N + 2 rows, loads one row ahead
         m = max(1, tz // (2 * N + 2))
@@ -284,6 +284,8 @@ def main():
     ap.add_argument("--nshot", type=int, default=0)
     ap.add_argument("--storage", choices=("auto", "full", "checkpoint"), default="auto")
     ap.add_argument("--ckpt-interval", type=int, default=0)
+    ap.add_argument("--ckpt-keep", type=int, default=0,
+                    help="segments whose snapshots the forward pass keeps (0: as many as fit)")
     ap.add_argument("--tile-tz", type=int, default=0)
     ap.add_argument("--variant", type=int, default=0, help="stencil kernel variant (0 = tuned)")
     ap.add_argument("--order", type=int, default=0, help="stencil work order 1..4 (0 = tuned)")
@@ -349,7 +351,7 @@ def main():
         f"({int(c['rec_off'][-1]) if 'rec_off' in c else ''} receivers), {len(groups)} groups of <= {sps} shots")
     storage = {"auto": _lib.STORE_AUTO, "full": _lib.STORE_FULL, "checkpoint": _lib.STORE_CHECKPOINT}[args.storage]
     kw = dict(batch=sps, tile_tz=args.tile_tz, kernel_variant=args.variant, work_order=args.order,
-              storage=storage, ckpt_interval=args.ckpt_interval)
+              storage=storage, ckpt_interval=args.ckpt_interval, ckpt_keep=args.ckpt_keep)
     sim = psv.Simulation(c, **kw)
     plan = sim.plan()
     log("plan", plan)
@@ -414,28 +416,31 @@ def main():
     # step 60 when checkpointed, CPML 32 per strip cell and direction; checkpoint
     # copies (<0.3 % at C3) are left out
     ckpt = plan["storage"] == "checkpoint"
+    # forward steps recomputed per time step: below 1 when the forward pass keeps the last
+    # segments' snapshots, above 1 on a binomial schedule
+    rec = float(plan["recompute"]) if ckpt else 0.0
     sdt = int(c["snap"][4])
     zs = int(np.count_nonzero((c["a_z"] != 0) | (c["a_half_z"] != 0))) if c["pml_z"] else 0
     xs = int(np.count_nonzero((c["a_x"] != 0) | (c["a_half_x"] != 0))) if c["pml_x"] else 0
     pml_cells = zs * nx + xs * nz
-    model_bytes = (cells * (B_STEP_FWD + B_STEP_ADJ + (B_GRAD + B_SNAP) / sdt + (B_STEP_RECOMP if ckpt else 0))
-                   + pml_cells * B_PML * (3 if ckpt else 2))
+    model_bytes = (cells * (B_STEP_FWD + B_STEP_ADJ + (B_GRAD + B_SNAP) / sdt + B_STEP_RECOMP * rec)
+                   + pml_cells * B_PML * (2 + rec))
 
     # roofline of the dominant kernel from its live per-launch times.  Every launch
     # advances one batch by one step; the bytes are summed over the launch modes the
     # gradient runs (forward, recomputation when checkpointed, adjoint) and divided
     # by the kernel's summed launch time.
     shot_steps = shots_timed * nt
-    passes = 3 if ckpt else 2
+    passes = 2 + rec
     zs_i = int(np.count_nonzero(((c["a_z"] != 0) | (c["a_half_z"] != 0))[N:nz - N])) if c["pml_z"] else 0
     xs_i = int(np.count_nonzero(((c["a_x"] != 0) | (c["a_half_x"] != 0))[N:nx - N])) if c["pml_x"] else 0
     strip = zs_i * (nx - 2 * N) + xs_i * (nz - 2 * N)   # interior cells inside a CPML strip (per direction)
     per_shot_step = {
         "step": interior * (B_STEP_FWD + B_STEP_ADJ + B_GRAD / sdt + B_SNAP / sdt
-                            + (B_STEP_RECOMP if ckpt else 0)) + strip * B_PML * passes,
+                            + B_STEP_RECOMP * rec) + strip * B_PML * passes,
         "stress": interior * B_STRESS * passes + strip * B_PML / 2 * passes,
         "velocity": interior * (B_VEL_FWD + B_VEL_ADJ + (B_VEL_GRAD + B_SNAP) / sdt
-                                + (B_VEL_RECOMP if ckpt else 0)) + strip * B_PML / 2 * passes,
+                                + B_VEL_RECOMP * rec) + strip * B_PML / 2 * passes,
     }
     kern = {k: (per_shot_step[k] * shot_steps, kt[k][1], kt[k][0]) for k in per_shot_step if kt[k][0] > 0}
     dom = max(kern, key=lambda k: kern[k][1])
@@ -570,7 +575,8 @@ def main():
                    "step": f"full gradient of {sps} shots (one launch batch); {len(groups)} steps = one "
                            f"gradient over all {nshot} shots",
                    "shots_per_step": sps, "batch": plan["batch"], "storage": plan["storage"],
-                   "ckpt_interval": plan["ckpt_interval"], "snap_dt": sdt, "tile_rows": tz,
+                   "ckpt_interval": plan["ckpt_interval"], "ckpt_units": plan["ckpt_units"],
+                   "ckpt_kept": plan["ckpt_kept"], "recompute": plan["recompute"], "snap_dt": sdt, "tile_rows": tz,
                    "parallelism": "single GPU, shots batched per launch"},
         "shots_per_s": round(shots_per_s, 3),
         "hbm_frac": round(hbm_frac, 4),

@@ -20,7 +20,7 @@ STATUS = {0: "FWI_OK", -1: "FWI_EINVAL", -2: "FWI_ENOMEM", -3: "FWI_EHIP", -4: "
 STORE_AUTO, STORE_FULL, STORE_CHECKPOINT = 0, 1, 2
 FORM_FUSED, FORM_PLAIN = 0, 1
 PARAM_LAME, PARAM_VELOCITY = 0, 1
-ABI_VERSION = 4
+ABI_VERSION = 5
 KERNELS = ("stress", "velocity", "surface", "source", "receiver", "store", "other", "step", "graph")
 
 
@@ -47,7 +47,8 @@ class FwiOpts(C.Structure):
                 ("check_finite", C.c_int32), ("kernel_variant", C.c_int32), ("work_order", C.c_int32),
                 ("param", C.c_int32), ("vel_bounds", C.c_double * 4), ("pcg", C.c_int32), ("smooth_hfs", C.c_int32),
                 ("ckpt_slots", C.c_int32),
-                ("seis_fp64", C.c_int32)]
+                ("seis_fp64", C.c_int32),
+                ("ckpt_keep", C.c_int32)]
 
 
 class FwiKernelTime(C.Structure):
@@ -85,6 +86,7 @@ def load() -> C.CDLL:
         "fwi_destroy": (None, [vp]),
         "fwi_plan": (C.c_int32, [vp, _ip, _ip, _ip, C.POINTER(C.c_int64)]),
         "fwi_checkpoint_plan": (C.c_int32, [vp, _ip, _ip, _dp]),
+        "fwi_checkpoint_kept": (C.c_int32, [vp, _ip]),
         "fwi_set_model_device": (C.c_int32, [vp, vp, vp, vp]),
         "fwi_forward_device": (C.c_int32, [vp, C.c_int32, C.c_int32, vp, vp]),
         "fwi_misfit_gradient_device": (C.c_int32, [vp, C.c_int32, C.c_int32, vp, vp, vp, vp, vp]),
@@ -155,5 +157,5 @@ EXPORTED = ("fwi_abi_version", "fwi_strerror", "fwi_last_error", "fwi_default_op
             "fwi_set_gradient",
             "fwi_taper_gradient", "fwi_misfit", "fwi_save_model", "fwi_update_model", "fwi_step_length",
             "fwi_iteration", "fwi_tuning", "fwi_profile", "fwi_profile_kernels", "fwi_kernel_times", "fwi_sync", "fwi_forward_snapshots",
-            "fwi_mat_av2_noalloc", "fwi_checkpoint_plan", "fwi_set_model_device", "fwi_forward_device",
+            "fwi_mat_av2_noalloc", "fwi_checkpoint_plan", "fwi_checkpoint_kept", "fwi_set_model_device", "fwi_forward_device",
             "fwi_misfit_gradient_device", "fwi_misfit_gradient_exact", "fwi_pcg_direction", "fwi_smooth_gradient")

@@ -64,7 +64,10 @@ struct fwi_ctx {
     // CHECKPOINT: nseg units of S steps and `slots` checkpoint buffers; with slots < nseg - 1
     // the reversal advances from held states on a binomial schedule (ck_top / ck_rev: best
     // split of a chain of n units with s free slots, index n * (slots + 1) + s)
-    int slots = 0;
+    // The forward pass keeps the snapshots of the last `keep` units (from time step t_keep):
+    // the reversal runs their adjoint steps first, without recomputation, then reverses the
+    // nrev = nseg - keep units before them from checkpoints.
+    int slots = 0, keep = 0, nrev = 0, t_keep = 0;
     std::vector<int> ck_top, ck_rev;
     std::vector<int> spine_slot;   // [nseg]: slot the forward pass saves a unit's start state into, -1 none
     long long extra_units = 0;     // units advanced again beyond the forward pass and the leaves
@@ -615,7 +618,7 @@ int autotune(fwi_ctx *c, int nb) {
     const bool order_forced = c->o.work_order >= 1 && c->o.work_order <= 4;
     // candidates (variant, rows per segment, prefetch depth); the one-pass step only where it applies
     std::vector<TuneVal> cand;
-    for (int var = 1; var <= 11; ++var) {
+    for (int var = 1; var <= 12; ++var) {
         if ((fv && var != fv) || (one_pass(var) && !c->fusable) || (var >= 5 && !c->ztab_ok)) continue;
         if (var == 9) {   // balanced: one or two waves of resident blocks
             for (int m : {1, 2})
@@ -640,7 +643,7 @@ int autotune(fwi_ctx *c, int nb) {
                        fv == 4   ? " (the one-pass step takes tile_tz 32 or 64, no free surface other than "
                                    "the top one at row fdorder/2, and sources and receivers inside the interior)"
                        : fv == 5 || fv >= 10
-                           ? " (variants 5, 10 and 11 take tile_tz 16 or 32, and z CPML strips of at most 64 rows "
+                           ? " (variants 5, 10, 11 and 12 take tile_tz 16 or 32, and z CPML strips of at most 64 rows "
                              "at the top and bottom of the grid)"
                        : fv == 6 ? " (variant 6 takes tile_tz 0, fdorder + 2 or 2 fdorder + 4, and z CPML "
                                    "strips of at most 64 rows at the top and bottom of the grid)"
@@ -751,31 +754,35 @@ int run_forward(fwi_ctx *c, int s0, int nb, bool store, bool record, bool energy
     // step's two time levels are back in place after each chunk)
     const int K = ckpt ? c->S : kGraphChunk;
     const bool graph = use_graphs(c) && K % 2 == 0;
-    ChunkGraph cg(c);
+    ChunkGraph cg(c), cgk(c);   // cgk: the kept units of a checkpointed pass (they store)
     float *snap0[5] = {c->snap[0], c->snap[1], c->snap[2], c->snap[3], c->snap[4]};
+    // snapshots go to slab (it - tb) / snap_dt: every step with full storage, from t_keep
+    // (a unit boundary) on with checkpoints
+    const int tb = ckpt ? c->t_keep : 0;
     for (int it = 0; it < nt;) {
         if (ckpt && it % c->S == 0 && c->spine_slot[it / c->S] >= 0) {
             Timer t(c, FWI_K_STORE);
             RETIF(ckpt_copy(c, c->spine_slot[it / c->S], nb, w, true));
         }
+        const bool keep = store && it >= tb;
         if (graph && it + K <= nt) {
             KArgs ag = a;
             ag.si = c->si_fwd;
             ag.si_base = c->si_base;
             ag.si_dir = 1;
-            RETIF(cg.run(it, [&]() {
+            RETIF((keep && ckpt ? cgk : cg).run(it, [&]() {
                 for (int k = 0; k < K; ++k) {
                     ag.si_k = k;
-                    time_step(c, ag, w, sa, it + k, store && !ckpt ? snap0 : nullptr);
+                    time_step(c, ag, w, sa, it + k, keep ? snap0 : nullptr);
                     if (record) launch_record(ag, ra, it + k, c->st);
                 }
             }));
             it += K;
             continue;
         }
-        const size_t off = (size_t)(it / c->g.snap_dt) * c->snz * c->snx;
+        const size_t off = keep ? (size_t)((it - tb) / c->g.snap_dt) * c->snz * c->snx : 0;
         float *dst[5] = {c->snap[0] + off, c->snap[1] + off, c->snap[2] + off, c->snap[3] + off, c->snap[4] + off};
-        const bool snap_step = store && !ckpt && it % c->g.snap_dt == 0;
+        const bool snap_step = keep && it % c->g.snap_dt == 0;
         time_step(c, a, w, sa, it, snap_step ? dst : nullptr);
         if (record) {
             Timer t(c, FWI_K_RECEIVER);
@@ -932,6 +939,24 @@ int run_adjoint(fwi_ctx *c, int s0, int nb) {
             }
             return FWI_OK;
         };
+        // the adjoint steps of unit u, its snapshots in the buffer from time step tb on
+        auto adjoint_unit = [&](int u, int tb) -> int {
+            const int t0 = u * S, t1 = std::min(t0 + S, nt);
+            if (graph && t1 - t0 == S) {   // the step table places the slabs
+                return cga.run(t1 - 1, [&]() {
+                    for (int j = 0; j < S; ++j) {
+                        ag.si_k = j;
+                        adj_step(c, ag, w, sa, t1 - 1 - j, 0);
+                        flush_sums_at(c, nb, j);
+                    }
+                });
+            }
+            for (int it = t1 - 1; it >= t0; --it) {
+                adj_step(c, a, w, sa, it, (size_t)((it - tb) / sdt) * slab);
+                if (flush_after(nt, it)) flush_sums(c, nb);
+            }
+            return FWI_OK;
+        };
         // unit u recomputed from the state in rw into the segment buffer, then its adjoint steps
         auto leaf = [&](int u) -> int {
             const int t0 = u * S, t1 = std::min(t0 + S, nt);
@@ -942,14 +967,7 @@ int run_adjoint(fwi_ctx *c, int s0, int nb) {
                         time_step(c, rg, rw, fsa, t0 + j, snap0);
                     }
                 }));
-                RETIF(cga.run(t1 - 1, [&]() {
-                    for (int j = 0; j < S; ++j) {
-                        ag.si_k = j;
-                        adj_step(c, ag, w, sa, t1 - 1 - j, 0);
-                        flush_sums_at(c, nb, j);
-                    }
-                }));
-                return FWI_OK;
+                return adjoint_unit(u, t0);
             }
             for (int it = t0; it < t1; ++it) {
                 const size_t off = (size_t)((it - t0) / sdt) * slab;
@@ -957,11 +975,7 @@ int run_adjoint(fwi_ctx *c, int s0, int nb) {
                                  c->snap[4] + off};
                 time_step(c, r, rw, fsa, it, it % sdt == 0 ? dst : nullptr);
             }
-            for (int it = t1 - 1; it >= t0; --it) {
-                adj_step(c, a, w, sa, it, (size_t)((it - t0) / sdt) * slab);
-                if (flush_after(nt, it)) flush_sums(c, nb);
-            }
-            return FWI_OK;
+            return adjoint_unit(u, t0);
         };
         // the leaves of units [a0, a0 + n) in reverse order; the state at a0 is in slot hold
         // (-1: the zero state of unit 0) and slots [slots - s, slots) are free.  top() is the
@@ -998,7 +1012,9 @@ int run_adjoint(fwi_ctx *c, int s0, int nb) {
             RETIF(top(a0 + m, n - m, sfree - 1, slot));
             return rev(a0, m, sfree, hold);
         };
-        RETIF(top(0, c->nseg, c->slots, -1));
+        // the kept units first (the forward pass stored them), then the others' reversal
+        for (int u = c->nseg - 1; u >= c->nrev; --u) RETIF(adjoint_unit(u, c->t_keep));
+        if (c->nrev > 0) RETIF(top(0, c->nrev, c->slots, -1));
         if (nt % kFlushSteps) flush_sums(c, nb);
     }
     c->last_s0 = s0;
@@ -1119,7 +1135,7 @@ struct Need {
     size_t shared, per_shot, per_shot_fp64;
 };
 
-Need need_bytes(const fwi_ctx *c, int storage, int S, int slots) {
+Need need_bytes(const fwi_ctx *c, int storage, int S, int slots, int keep = 1) {
     const size_t nz = c->g.nz, nx = c->g.nx, nt = c->g.nt, fs = c->fs();
     const size_t snapcell = (size_t)c->snz * c->snx;
     Need n{};
@@ -1140,7 +1156,8 @@ Need need_bytes(const fwi_ctx *c, int storage, int S, int slots) {
     if (storage == FWI_STORE_FULL) {
         n.per_shot += 5 * (size_t)c->snt * snapcell * 4;
     } else {
-        n.per_shot += state_ck + level2 + (size_t)slots * state_ck + 5 * (size_t)(S / c->g.snap_dt) * snapcell * 4;
+        n.per_shot += state_ck + level2 + (size_t)slots * state_ck +
+                      (size_t)std::max(keep, 1) * 5 * (size_t)(S / c->g.snap_dt) * snapcell * 4;
     }
     return n;
 }
@@ -1266,16 +1283,38 @@ int plan(fwi_ctx *c) {
     c->storage = storage;
     c->S = S;
     c->nseg = units_of(S);
-    c->slots = storage == FWI_STORE_CHECKPOINT ? slots : 0;
+    c->slots = 0;
+    c->keep = 0;
+    c->nrev = 0;
+    c->t_keep = nt;
     c->spine_slot.assign(c->nseg, -1);
     if (storage == FWI_STORE_CHECKPOINT) {
-        if (c->nseg > kMaxUnits && c->slots < c->nseg - 1)
+        // Units whose snapshots the forward pass keeps: the last one always (the segment
+        // buffer holds it); with a checkpoint at every boundary, more while the budget left
+        // beside the batch allows, each costing one unit of snapshots less one checkpoint.
+        int keep = 1;
+        const int cap = c->o.ckpt_keep > 0 ? std::min(c->o.ckpt_keep, c->nseg) : c->nseg;
+        if (cap > 1 && slots >= c->nseg - 1) {
+            const Need k1 = need_bytes(c, storage, S, slots, 1), k2 = need_bytes(c, storage, S, slots - 1, 2);
+            const size_t used = k1.shared + (size_t)c->B * k1.per_shot;
+            if (k2.per_shot <= k1.per_shot) {
+                keep = cap;
+            } else if (used <= budget) {
+                const size_t more = (budget - used) / ((size_t)c->B * (k2.per_shot - k1.per_shot));
+                keep = (int)std::min<size_t>((size_t)cap, 1 + more);
+            }
+        }
+        c->keep = keep;
+        c->nrev = c->nseg - keep;
+        c->t_keep = c->nrev * S;
+        c->slots = std::min(slots, std::max(c->nrev - 1, 0));
+        if (c->nrev > kMaxUnits && c->slots < c->nrev - 1)
             return c->fail(FWI_EINVAL, "%d segments with %d checkpoints: more than %d segments need one each",
-                           c->nseg, c->slots, kMaxUnits);
-        c->extra_units = plan_schedule(c->nseg, c->slots, c->ck_top, c->ck_rev);
+                           c->nrev, c->slots, kMaxUnits);
+        c->extra_units = plan_schedule(c->nrev, c->slots, c->ck_top, c->ck_rev);
         // the states the forward pass saves: the splits along top()'s first chain
-        for (int a0 = 0, k = c->nseg, j = c->slots; k > 1 && j > 0; --j) {
-            const int m = c->slots >= c->nseg - 1 ? 1 : c->ck_top[(size_t)k * (c->slots + 1) + j];
+        for (int a0 = 0, k = c->nrev, j = c->slots; k > 1 && j > 0; --j) {
+            const int m = c->slots >= c->nrev - 1 ? 1 : c->ck_top[(size_t)k * (c->slots + 1) + j];
             c->spine_slot[a0 + m] = c->slots - j;
             a0 += m;
             k -= m;
@@ -1334,7 +1373,7 @@ int allocate(fwi_ctx *c) {
     if (c->storage == FWI_STORE_FULL) {
         c->snap_stride = (size_t)c->snt * snapcell;
     } else {
-        c->snap_stride = (size_t)(c->S / c->g.snap_dt) * snapcell;
+        c->snap_stride = (size_t)c->keep * (c->S / c->g.snap_dt) * snapcell;   // >= one unit
         for (int i = 0; i < 5; ++i) c->rf[i] = F(B * fs);
         for (int i = 0; i < 4; ++i) c->rpsi[i] = F(B * c->psiz());
         for (int i = 4; i < 8; ++i) c->rpsi[i] = F(B * c->psix());
@@ -1437,14 +1476,17 @@ int setup_device(fwi_ctx *c, const fwi_acq *q) {
     {   // per-step tables for graph replay (what time_step / adj_step derive from it)
         const int nt = c->g.nt, sdt = c->g.snap_dt;
         const long long slab = (long long)c->snz * c->snx;
+        // full storage: slab it / snap_dt; checkpoints: from the unit's start in a recomputed
+        // unit, from t_keep in the kept ones (which the forward pass stores)
         const bool full = c->storage != FWI_STORE_CHECKPOINT;
         std::vector<StepInfo> f(nt), r(nt), a(nt);
         for (int it = 0; it < nt; ++it) {
-            const int t0 = full ? 0 : (it / c->S) * c->S;
-            const long long off = (long long)((it - t0) / sdt) * slab;
+            const bool kept = full || it >= c->t_keep;
+            const int tu = full ? 0 : (it / c->S) * c->S, t0 = full ? 0 : kept ? c->t_keep : tu;
+            const long long off = (long long)((it - t0) / sdt) * slab, roff = (long long)((it - tu) / sdt) * slab;
             const int snap = it % sdt == 0;
-            f[it] = {it, 0, full && snap, 0, full ? off : 0};
-            r[it] = {it, 0, snap, 0, off};
+            f[it] = {it, 0, kept && snap, 0, kept ? off : 0};
+            r[it] = {it, 0, snap, 0, roff};
             a[it] = {it, snap, 0, 0, off};
         }
         RETIF(upload(c, c->si_fwd, f.data(), nt));
@@ -1545,12 +1587,13 @@ int validate(fwi_ctx *c, const fwi_grid *g, const fwi_acq *q) {
         return c->fail(FWI_EINVAL,
                        "tile_tz %d (0, 8, 16, 32 or 64; variant 6: 0 or its row block times 1 or 2; "
                        "variants 7, 8: 0 or their row block times 4 or 8)", tt);
-    if (c->o.kernel_variant < 0 || c->o.kernel_variant > 11)
-        return c->fail(FWI_EINVAL, "kernel_variant %d (0..11)", c->o.kernel_variant);
+    if (c->o.kernel_variant < 0 || c->o.kernel_variant > 12)
+        return c->fail(FWI_EINVAL, "kernel_variant %d (0..12)", c->o.kernel_variant);
     if (c->o.kernel_variant == 9 && tt > 2) return c->fail(FWI_EINVAL, "variant 9 takes tile_tz 0, 1 or 2");
     if (c->o.work_order < 0 || c->o.work_order > 4)
         return c->fail(FWI_EINVAL, "work_order %d (0..4)", c->o.work_order);
     if (c->o.ckpt_slots < 0) return c->fail(FWI_EINVAL, "ckpt_slots %d < 0", c->o.ckpt_slots);
+    if (c->o.ckpt_keep < 0) return c->fail(FWI_EINVAL, "ckpt_keep %d < 0", c->o.ckpt_keep);
     if (c->o.seis_fp64 != 0 && c->o.seis_fp64 != 1) return c->fail(FWI_EINVAL, "seis_fp64 %d (0 or 1)", c->o.seis_fp64);
     if (c->o.smooth_hfs != 0 && (c->o.smooth_hfs < 2 || c->o.smooth_hfs > 64))
         return c->fail(FWI_EINVAL, "smooth_hfs %d (0, or 2..64)", c->o.smooth_hfs);
@@ -1725,8 +1768,14 @@ int32_t fwi_checkpoint_plan(const fwi_ctx *c, int32_t *units, int32_t *slots, do
     const bool ck = c->storage == FWI_STORE_CHECKPOINT;
     if (units) *units = ck ? c->nseg : 0;
     if (slots) *slots = ck ? c->slots : 0;
-    // the leaves recompute every step once; advances are whole segments
-    if (recompute) *recompute = ck ? ((double)c->g.nt + (double)c->extra_units * c->S) / c->g.nt : 0.0;
+    // the leaves recompute each unit before t_keep once; advances are whole units
+    if (recompute) *recompute = ck ? ((double)c->nrev + (double)c->extra_units) * c->S / c->g.nt : 0.0;
+    return FWI_OK;
+}
+
+int32_t fwi_checkpoint_kept(const fwi_ctx *c, int32_t *kept) {
+    if (!c) return FWI_EINVAL;
+    if (kept) *kept = c->storage == FWI_STORE_CHECKPOINT ? c->keep : 0;
     return FWI_OK;
 }
 

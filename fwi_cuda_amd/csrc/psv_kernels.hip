@@ -514,17 +514,22 @@ __global__ __launch_bounds__(kBX *kBY, MINW) void k_velocity2(const KArgs a) {
 // columns each side that the x taps need come from one extra load per field and row
 // into a halo register, read through lane_off_h.
 // SB (variant 11): a block's kBY wavefronts are kBY shots of one tile and segment.
-template <int N, int TZ, bool FULL, bool SB = false>
+// XW (variant 12): a block's kBY wavefronts are kBY adjacent tiles of one segment.
+template <int N, int TZ, bool FULL, bool SB = false, bool XW = false>
 __device__ __forceinline__ void stress5_body(const KArgs &a, const ZTab &zt) {
     constexpr int OUTW = FULL ? kBX : kBX - 2 * N;
     int tbx, tby, s;
-    tile_of<TZ, OUTW, false, FULL, SB>(a, N, tbx, tby, s);
+    tile_of<TZ, OUTW, XW, FULL, SB>(a, N, tbx, tby, s);
     const int lane = threadIdx.x;
     if constexpr (SB) {
         s = s * kBY + wave_uniform((int)threadIdx.y);
         if (s >= a.B) return;
     }
-    const int z0 = N + (SB ? tby : tby * kBY + wave_uniform((int)threadIdx.y)) * TZ;
+    if constexpr (XW) {   // past the last tile: nothing to do
+        tbx = tbx * kBY + wave_uniform((int)threadIdx.y);
+        if (tbx * OUTW >= (FULL ? a.nx - N : a.nx - 2 * N)) return;
+    }
+    const int z0 = N + (SB || XW ? tby : tby * kBY + wave_uniform((int)threadIdx.y)) * TZ;
     const int nzi = a.nz - N;
     if (z0 >= nzi) return;
     const int zend = min(z0 + TZ, nzi);
@@ -665,20 +670,29 @@ __global__ __launch_bounds__(kBX *kBY) void k_stress11(const KArgs a, const ZTab
     stress5_body<N, TZ, true, true>(a, zt);
 }
 
+template <int N, int TZ>
+__global__ __launch_bounds__(kBX *kBY) void k_stress12(const KArgs a, const ZTab zt) {
+    stress5_body<N, TZ, true, false, true>(a, zt);
+}
+
 // The velocity rows of one wavefront; GRAD: this step correlates with the snapshot slab;
 // FULL: full 64-column tiles with halo registers (variant 10, as in stress5_body).
-template <int N, int TZ, int MODE, bool GRAD, bool FULL, bool SB>
+template <int N, int TZ, int MODE, bool GRAD, bool FULL, bool SB, bool XW>
 __device__ __forceinline__ void velocity5_rows(const KArgs &a, const ZTab &zt, const StepView &sv) {
     constexpr int OUTW = FULL ? kBX : kBX - 2 * N;
     constexpr bool displ = (MODE & kVelAdjoint) != 0;   // u += dt v
     int tbx, tby, s;
-    tile_of<TZ, OUTW, false, FULL, SB>(a, N, tbx, tby, s);
+    tile_of<TZ, OUTW, XW, FULL, SB>(a, N, tbx, tby, s);
     const int lane = threadIdx.x;
     if constexpr (SB) {
         s = s * kBY + wave_uniform((int)threadIdx.y);
         if (s >= a.B) return;
     }
-    const int z0 = N + (SB ? tby : tby * kBY + wave_uniform((int)threadIdx.y)) * TZ;
+    if constexpr (XW) {   // past the last tile: nothing to do
+        tbx = tbx * kBY + wave_uniform((int)threadIdx.y);
+        if (tbx * OUTW >= (FULL ? a.nx - N : a.nx - 2 * N)) return;
+    }
+    const int z0 = N + (SB || XW ? tby : tby * kBY + wave_uniform((int)threadIdx.y)) * TZ;
     const int nzi = a.nz - N;
     if (z0 >= nzi) return;
     const int zend = min(z0 + TZ, nzi);
@@ -881,16 +895,16 @@ __device__ __forceinline__ void velocity5_rows(const KArgs &a, const ZTab &zt, c
 
 // kVelAdjoint launches branch once, on the step's correlation switch, to a loop
 // whose loads are all unconditional
-template <int N, int TZ, int MODE, bool FULL, bool SB = false>
+template <int N, int TZ, int MODE, bool FULL, bool SB = false, bool XW = false>
 __device__ __forceinline__ void velocity5_body(const KArgs &a, const ZTab &zt) {
     const StepView sv = step_view(a, 0);
     if constexpr ((MODE & kVelAdjoint) != 0) {
         if (sv.grad) {
-            velocity5_rows<N, TZ, MODE, true, FULL, SB>(a, zt, sv);
+            velocity5_rows<N, TZ, MODE, true, FULL, SB, XW>(a, zt, sv);
             return;
         }
     }
-    velocity5_rows<N, TZ, MODE, false, FULL, SB>(a, zt, sv);
+    velocity5_rows<N, TZ, MODE, false, FULL, SB, XW>(a, zt, sv);
 }
 
 template <int N, int TZ, int MODE>
@@ -906,6 +920,11 @@ __global__ __launch_bounds__(kBX *kBY) void k_velocity10(const KArgs a, const ZT
 template <int N, int TZ, int MODE>
 __global__ __launch_bounds__(kBX *kBY) void k_velocity11(const KArgs a, const ZTab zt) {
     velocity5_body<N, TZ, MODE, true, true>(a, zt);
+}
+
+template <int N, int TZ, int MODE>
+__global__ __launch_bounds__(kBX *kBY) void k_velocity12(const KArgs a, const ZTab zt) {
+    velocity5_body<N, TZ, MODE, true, false, true>(a, zt);
 }
 
 // ---------------------------------------------------------------------------
@@ -1328,19 +1347,22 @@ __global__ void k_store(const KArgs a, float *d0, float *d1, float *d2, float *d
 // writes 64 columns per wavefront, variants 2 and 3 write 64 - 2N.  Variant 3 is
 // variant 2 compiled for at least 6 wavefronts per SIMD (a register cap).
 static dim3 stencil_grid(const KArgs &a, int N, int tz, int var) {
-    const bool full = var == 10 || var == 11;   // tiles of 64 columns from column 0
+    const bool full = var >= 10;   // tiles of 64 columns from column 0
     const int nxi = full ? a.nx - N : a.nx - 2 * N, nzi = a.nz - 2 * N;
     const int segs = (nzi + tz - 1) / tz;
     const int outw = var >= 2 && !full ? kBX - 2 * N : kBX;
+    const int tiles = (nxi + outw - 1) / outw;
     if (var == 11)   // a block: kBY shots of one tile and segment
-        return dim3(((nxi + outw - 1) / outw) * segs * ((a.B + kBY - 1) / kBY));
-    return dim3(((nxi + outw - 1) / outw) * ((segs + kBY - 1) / kBY) * a.B);
+        return dim3(tiles * segs * ((a.B + kBY - 1) / kBY));
+    if (var == 12)   // a block: kBY adjacent tiles of one segment
+        return dim3((tiles + kBY - 1) / kBY * segs * a.B);
+    return dim3(tiles * ((segs + kBY - 1) / kBY) * a.B);
 }
 
 // the two-kernel form is instantiated for TZ 8, 16, 32, 64 (16 for anything else);
 // variants 5 and 10 for TZ 16 and 32
 static int two_kernel_tz(int tz, int var) {
-    if (var == 5 || var == 10 || var == 11) return tz == 32 ? 32 : 16;
+    if (var == 5 || var >= 10) return tz == 32 ? 32 : 16;
     return (tz == 8 || tz == 32 || tz == 64) ? tz : 16;
 }
 
@@ -1365,7 +1387,18 @@ static int two_kernel_tz(int tz, int var) {
 template <int V>
 static bool launch_velocity5(const KArgs &a, const ZTab *zt, int N, int tz, dim3 grid, dim3 block,
                              hipStream_t st) {
-    if constexpr (V == 11) {
+    if constexpr (V == 12) {
+        switch (vel_mode(a)) {
+            case 0: PSV_N5_CASES(k_velocity12, , 0) return true;
+            case kVelEnergy: PSV_N5_CASES(k_velocity12, , kVelEnergy) return true;
+            case kVelStore: PSV_N5_CASES(k_velocity12, , kVelStore) return true;
+            case kVelEnergy | kVelStore: PSV_N5_CASES(k_velocity12, , kVelEnergy | kVelStore) return true;
+            case kVelEnergy | kVelAdjoint: PSV_N5_CASES(k_velocity12, , kVelEnergy | kVelAdjoint) return true;
+            case kVelEnergy | kVelAdjoint | kVelMisfit:
+                PSV_N5_CASES(k_velocity12, , kVelEnergy | kVelAdjoint | kVelMisfit) return true;
+            default: return false;
+        }
+    } else if constexpr (V == 11) {
         switch (vel_mode(a)) {
             case 0: PSV_N5_CASES(k_velocity11, , 0) return true;
             case kVelEnergy: PSV_N5_CASES(k_velocity11, , kVelEnergy) return true;
@@ -1409,7 +1442,9 @@ void launch_stress(const KArgs &a, int N, int tz, int var, hipStream_t st, const
     }
     tz = two_kernel_tz(tz, var);
     const dim3 grid = stencil_grid(a, N, tz, var), block(kBX, kBY);
-    if (var == 11) {
+    if (var == 12) {
+        PSV_N5_CASES(k_stress12, )
+    } else if (var == 11) {
         PSV_N5_CASES(k_stress11, )
     } else if (var == 10) {
         PSV_N5_CASES(k_stress10, )
@@ -1431,14 +1466,15 @@ void launch_velocity(const KArgs &a, int N, int tz, int var, hipStream_t st, con
         if (launch6(1, a, *zt, N, depth, tz, st)) return;
         var = 2;   // a launch mode variant 6 has no kernel for
     }
-    if (var == 5 || var == 10 || var == 11) {
+    if (var == 5 || var >= 10) {
         const int tz5 = two_kernel_tz(tz, var);
         const dim3 g5 = stencil_grid(a, N, tz5, var);
-        if (var == 11 ? launch_velocity5<11>(a, zt, N, tz5, g5, dim3(kBX, kBY), st)
+        if (var == 12   ? launch_velocity5<12>(a, zt, N, tz5, g5, dim3(kBX, kBY), st)
+            : var == 11 ? launch_velocity5<11>(a, zt, N, tz5, g5, dim3(kBX, kBY), st)
             : var == 10 ? launch_velocity5<10>(a, zt, N, tz5, g5, dim3(kBX, kBY), st)
                         : launch_velocity5<5>(a, zt, N, tz5, g5, dim3(kBX, kBY), st))
             return;
-        var = 2;   // a launch mode variants 5, 10 and 11 have no kernel for
+        var = 2;   // a launch mode variants 5, 10, 11 and 12 have no kernel for
     }
     tz = two_kernel_tz(tz, var);
     const dim3 grid = stencil_grid(a, N, tz, var), block(kBX, kBY);

@@ -152,7 +152,7 @@ class Simulation:
     def __init__(self, case: dict, *, batch=0, storage=_lib.STORE_AUTO, ckpt_interval=0, mem_budget=0,
                  form=_lib.FORM_FUSED, grad_sign=-1, tile_tz=0, device=0, check_finite=1, kernel_variant=0,
                  work_order=0, use_graph=0, param=_lib.PARAM_LAME, vel_bounds=None, ckpt_slots=0, pcg=0,
-                 smooth_hfs=0, seis_fp64=0):
+                 smooth_hfs=0, seis_fp64=0, ckpt_keep=0):
         lib = _lib.load()
         self.lib = lib
         self.case = case
@@ -191,6 +191,7 @@ class Simulation:
         o.ckpt_slots = int(ckpt_slots)
         o.pcg, o.smooth_hfs = int(pcg), int(smooth_hfs)
         o.seis_fp64 = int(seis_fp64)
+        o.ckpt_keep = int(ckpt_keep)
         if vel_bounds is not None:
             o.vel_bounds[:] = [float(v) for v in vel_bounds]
         h = C.c_void_p()
@@ -231,9 +232,11 @@ class Simulation:
         self._chk(self.lib.fwi_plan(self.ctx, C.byref(b), C.byref(s), C.byref(k), C.byref(n)))
         u, sl, rf = C.c_int32(), C.c_int32(), C.c_double()
         self._chk(self.lib.fwi_checkpoint_plan(self.ctx, C.byref(u), C.byref(sl), C.byref(rf)))
+        kept = C.c_int32()
+        self._chk(self.lib.fwi_checkpoint_kept(self.ctx, C.byref(kept)))
         return dict(batch=b.value, storage={1: "full", 2: "checkpoint"}.get(s.value, s.value),
                     ckpt_interval=k.value, device_bytes=n.value, ckpt_units=u.value, ckpt_slots=sl.value,
-                    recompute=round(rf.value, 4))
+                    ckpt_kept=kept.value, recompute=round(rf.value, 4))
 
     # -- model (mat_av2) --------------------------------------------------------
     def set_model(self, lam, mu, rho):

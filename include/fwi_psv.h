@@ -52,7 +52,7 @@
 extern "C" {
 #endif
 
-#define FWI_PSV_ABI_VERSION 4
+#define FWI_PSV_ABI_VERSION 5
 
 enum fwi_status {
     FWI_OK = 0,
@@ -133,8 +133,9 @@ typedef struct {
                                times then only see the replays (FWI_K_GRAPH) */
     int32_t tile_tz;        /* rows per thread in the stencil kernels; 0 = tuned;
                                8, 16, 32, 64 (variants 1-3), 32, 64 (variant 4), 16, 32
-                               (variant 5), fdorder + 2 or 2 fdorder + 4 (variant 6),
-                               4 fdorder + 8 or 8 fdorder + 16 (variants 7, 8) */
+                               (variants 5, 10-12), fdorder + 2 or 2 fdorder + 4 (variant 6),
+                               4 fdorder + 8 or 8 fdorder + 16 (variants 7, 8), 1 or 2
+                               (variant 9: waves of resident blocks) */
     int32_t device;         /* HIP device ordinal */
     int32_t check_finite;   /* scan the wavefield for NaN/Inf after each propagation */
     int32_t kernel_variant; /* stencil kernels: 0 = tuned, 1 = x taps as shifted loads,
@@ -152,8 +153,10 @@ typedef struct {
                                10 = variant 5 on full 64-column tiles from column 0 (rows
                                    aligned, the x halo from one extra load per row),
                                11 = variant 10 with a block's four wavefronts on four
-                                   shots of one tile and segment (shared media rows)
-                               (5 to 11 need z CPML strips of at most 64 rows, at the
+                                   shots of one tile and segment (shared media rows),
+                               12 = variant 10 with a block's four wavefronts on four
+                                   adjacent tiles of one segment (1 KiB row runs)
+                               (5 to 12 need z CPML strips of at most 64 rows, at the
                                top and bottom of the grid) */
     int32_t work_order;     /* order of the stencil tiles over the grid: 0 = tuned,
                                1 = shot fastest then z then x, 2 = shot fastest then x then z,
@@ -175,6 +178,11 @@ typedef struct {
                                adjoint sources are made of come from an FP64 propagation (FP64
                                media, CPML and fields); the FP32 stencil still runs the wavefields
                                the gradient correlates (ABI 4) */
+    int32_t ckpt_keep;      /* CHECKPOINT storage: the forward pass keeps the snapshots of the last
+                               segments, which the reversal then runs without recomputing them;
+                               0 = as many segments as the budget allows beside the batch and one
+                               checkpoint per remaining boundary (at least the last one), k > 0 =
+                               at most k (ABI 5) */
 } fwi_opts;
 
 /* Per-kernel timing collected while profiling is on (HIP events on the
@@ -222,8 +230,11 @@ int32_t fwi_plan(const fwi_ctx *ctx, int32_t *batch, int32_t *storage, int32_t *
 
 /* Checkpoint schedule of CHECKPOINT storage: units = segments of ckpt_interval steps,
  * slots = checkpoint buffers, recompute = forward steps run again per time step of the
- * adjoint (1 when every segment boundary is kept: each segment recomputed once). */
+ * adjoint (below 1 when the forward pass keeps the snapshots of the last segments; with a
+ * checkpoint at every other boundary each remaining segment is recomputed once). */
 int32_t fwi_checkpoint_plan(const fwi_ctx *ctx, int32_t *units, int32_t *slots, double *recompute);
+/* Segments at the end of the time axis whose snapshots the forward pass keeps (ABI 5). */
+int32_t fwi_checkpoint_kept(const fwi_ctx *ctx, int32_t *kept);
 
 /* Model lam, mu, rho [nz][nx]; derives mu_zx, rho_zp, rho_xp and the scalar means
  * (mat_av2) on the device. */

@@ -85,7 +85,7 @@ def test_shot_gradient_matches_reference_golden(psv, name):
 
 
 @pytest.mark.parametrize("surface", [True, False])
-@pytest.mark.parametrize("variant", [1, 2, 4, 5, 6, 7, 8, 9, 10, 11])
+@pytest.mark.parametrize("variant", [1, 2, 4, 5, 6, 7, 8, 9, 10, 11, 12])
 def test_order8_forward_vs_oracle(psv, O, surface, variant):
     c = cases.small_case(O, surface=surface, nshot=2)
     med = O.mat_av2(c)
@@ -100,7 +100,7 @@ def test_order8_forward_vs_oracle(psv, O, surface, variant):
         assert e < 1e-4
 
 
-@pytest.mark.parametrize("variant", [1, 2, 4, 5, 6, 7, 8, 9, 10, 11])
+@pytest.mark.parametrize("variant", [1, 2, 4, 5, 6, 7, 8, 9, 10, 11, 12])
 def test_order8_gradient_vs_oracle(psv, O, variant):
     c = cases.small_case(O, nshot=2, nt=450)
     med_true = O.mat_av2(c)
@@ -126,22 +126,28 @@ def test_order8_gradient_vs_oracle(psv, O, variant):
             assert e < 2e-3
 
 
-@pytest.mark.parametrize("variant", [1, 4, 5, 6, 7, 8, 9, 10, 11])
+@pytest.mark.parametrize("variant", [1, 4, 5, 6, 7, 8, 9, 10, 11, 12])
 def test_checkpointed_equals_full_storage(psv, O, variant):
     c = cases.small_case(O, nshot=2, nt=300, snap_dt=2)
     med_true = O.mat_av2(c)
     obs = cases.per_shot_obs(c, O, med_true)
     lam0, mu0, rho0 = cases.start_model(c)
     res = {}
-    for storage, interval in (("full", 0), ("ckpt", 14), ("ckpt", 300)):
+    # checkpoints every 14 steps with the last 1, 5 or (budget) all 22 segments' snapshots kept
+    # by the forward pass, and one segment of 300 steps
+    for storage, interval, keep in (("full", 0, 0), ("ckpt", 14, 1), ("ckpt", 14, 5), ("ckpt", 14, 0),
+                                    ("ckpt", 300, 0)):
         st = psv._lib.STORE_FULL if storage == "full" else psv._lib.STORE_CHECKPOINT
-        with psv.Simulation(c, storage=st, ckpt_interval=interval, kernel_variant=variant) as sim:
-            assert sim.plan()["storage"] == ("full" if storage == "full" else "checkpoint")
+        with psv.Simulation(c, storage=st, ckpt_interval=interval, ckpt_keep=keep, kernel_variant=variant) as sim:
+            plan = sim.plan()
+            assert plan["storage"] == ("full" if storage == "full" else "checkpoint")
+            if storage == "ckpt":
+                assert plan["ckpt_kept"] == (keep or plan["ckpt_units"])
             sim.set_model(lam0, mu0, rho0)
             sim.set_observed([o[0] for o in obs], [o[1] for o in obs])
             m = sim.gradient()
-            res[(storage, interval)] = (m, sim.get_gradient())
-    ref = res[("full", 0)]
+            res[(storage, interval, keep)] = (m, sim.get_gradient())
+    ref = res[("full", 0, 0)]
     for key, (m, G) in res.items():
         np.testing.assert_array_equal(m, ref[0])
         for i in range(3):
@@ -154,9 +160,11 @@ def test_checkpointed_equals_full_storage(psv, O, variant):
 def test_binomial_checkpoints_equal_full_storage(psv, O, variant, graph, sdt, S):
     """Fewer checkpoint buffers than segment boundaries (ckpt_slots 1, 2, 5 for the
     segments of S steps, the last one short, snapshots every sdt-th step): the reversal
-    advances from held checkpoints on the binomial schedule and recomputes more (plan's
-    recompute > 1), and the misfits and gradient still equal full snapshot storage bit
-    for bit (the same kernels replay the same steps), with and without graph replay."""
+    advances from held checkpoints on the binomial schedule and recomputes more, and the
+    misfits and gradient still equal full snapshot storage bit for bit (the same kernels
+    replay the same steps), with and without graph replay.  With a checkpoint at every
+    boundary the forward pass keeps the snapshots of the last 1, 4 or (budget) all
+    segments, which are then not recomputed: the same results again."""
     nt = 302
     c = cases.small_case(O, nshot=3, nt=nt, snap_dt=sdt, uneven=True)
     med_true = O.mat_av2(c)
@@ -164,26 +172,34 @@ def test_binomial_checkpoints_equal_full_storage(psv, O, variant, graph, sdt, S)
     lam0, mu0, rho0 = cases.start_model(c)
     res = {}
     units = (nt + S - 1) // S
-    for storage, slots in (("full", 0), ("ckpt", 0), ("ckpt", 5), ("ckpt", 2), ("ckpt", 1)):
+    for storage, slots, keep in (("full", 0, 0), ("ckpt", 0, 1), ("ckpt", 0, 4), ("ckpt", 0, 0), ("ckpt", 5, 0),
+                                 ("ckpt", 2, 0), ("ckpt", 1, 0)):
         st = psv._lib.STORE_FULL if storage == "full" else psv._lib.STORE_CHECKPOINT
-        with psv.Simulation(c, storage=st, ckpt_interval=S, ckpt_slots=slots, kernel_variant=variant,
+        with psv.Simulation(c, storage=st, ckpt_interval=S, ckpt_slots=slots, ckpt_keep=keep, kernel_variant=variant,
                             use_graph=graph, batch=2) as sim:
             plan = sim.plan()
             if storage == "ckpt":
+                kept = (keep or units) if slots == 0 else 1   # fewer slots: only the last unit kept
                 assert plan["ckpt_units"] == units
-                assert plan["ckpt_slots"] == (units - 1 if slots == 0 else slots)
-                assert (plan["recompute"] > 1.0) == (slots > 0)
-            _print(f"variant {variant} graph {graph} {storage} slots {slots}: {plan}")
+                assert plan["ckpt_kept"] == kept
+                assert plan["ckpt_slots"] == (max(units - kept - 1, 0) if slots == 0 else slots)
+                recomputed = (units - kept) * S / nt
+                if slots == 0:
+                    assert plan["recompute"] == pytest.approx(recomputed, abs=1e-4)
+                else:
+                    assert plan["recompute"] > recomputed
+            _print(f"variant {variant} graph {graph} {storage} slots {slots} keep {keep}: {plan}")
             sim.set_model(lam0, mu0, rho0)
             sim.set_observed([o[0] for o in obs], [o[1] for o in obs])
             m = sim.gradient()
-            res[(storage, slots)] = (m, sim.get_gradient(), plan["recompute"])
-    ref = res[("full", 0)]
+            res[(storage, slots, keep)] = (m, sim.get_gradient(), plan["recompute"])
+    ref = res[("full", 0, 0)]
     for key, (m, G, _) in res.items():
         np.testing.assert_array_equal(m, ref[0])
         for i in range(3):
             np.testing.assert_array_equal(G[i], ref[1][i], err_msg=str(key))
-    assert res[("ckpt", 1)][2] > res[("ckpt", 2)][2] > res[("ckpt", 5)][2] > res[("ckpt", 0)][2] == 1.0
+    r = {k[1:]: v[2] for k, v in res.items() if k[0] == "ckpt"}
+    assert r[(1, 0)] > r[(2, 0)] > r[(5, 0)] > r[(0, 1)] > r[(0, 4)] > r[(0, 0)] == 0.0
 
 
 @pytest.mark.parametrize("variant", [5, 10])
@@ -200,12 +216,12 @@ def test_binomial_checkpoints_many_units(psv, O, variant):
     res = {}
     for storage, slots in (("full", 0), ("ckpt", 0), ("ckpt", 32), ("ckpt", 8), ("ckpt", 3)):
         st = psv._lib.STORE_FULL if storage == "full" else psv._lib.STORE_CHECKPOINT
-        with psv.Simulation(c, storage=st, ckpt_interval=S, ckpt_slots=slots, kernel_variant=variant,
+        with psv.Simulation(c, storage=st, ckpt_interval=S, ckpt_slots=slots, ckpt_keep=1, kernel_variant=variant,
                             tile_tz=16, work_order=1) as sim:
             plan = sim.plan()
-            if storage == "ckpt":
-                assert plan["ckpt_units"] == nunits
-                assert plan["ckpt_slots"] == (nunits - 1 if slots == 0 else slots)
+            if storage == "ckpt":   # the last unit kept: 50 units reversed, 49 boundaries
+                assert plan["ckpt_units"] == nunits and plan["ckpt_kept"] == 1
+                assert plan["ckpt_slots"] == (nunits - 2 if slots == 0 else slots)
             sim.set_model(lam0, mu0, rho0)
             sim.set_observed(*obs[0])
             m = sim.gradient()
@@ -216,14 +232,16 @@ def test_binomial_checkpoints_many_units(psv, O, variant):
         np.testing.assert_array_equal(m, ref[0], err_msg=str(key))
         for i in range(3):
             np.testing.assert_array_equal(G[i], ref[1][i], err_msg=str(key))
-    assert res[("ckpt", 3)][2] > res[("ckpt", 8)][2] > res[("ckpt", 32)][2] > res[("ckpt", 0)][2] == 1.0
+    assert res[("ckpt", 3)][2] > res[("ckpt", 8)][2] > res[("ckpt", 32)][2] > res[("ckpt", 0)][2]
+    assert res[("ckpt", 0)][2] == pytest.approx((nunits - 1) * S / nt, abs=1e-4)
 
 
 def test_c4_window_checkpointed_equals_full_storage(psv, O):
     """The C4 model (15 layers and a dipping dyke, order 8) cropped to a 1001 x 2001
     window, two shots, 1600 steps (the 100 Hz wavelet peaks at step ~620): checkpointed
-    recomputation at two intervals against full snapshot storage of the same window
-    (64 GB of snapshots per shot), bit for bit (misfits and gradient)."""
+    recomputation at two intervals (every segment but the last recomputed, and the last
+    segments' snapshots kept as the budget allows) against full snapshot storage of the
+    same window (64 GB of snapshots per shot), bit for bit (misfits and gradient)."""
     from scipy.ndimage import gaussian_filter
     from fwi_cuda_amd import configs
     c = configs.c4(nt=1600, nshot=2, nz=1001, nx=2001)
@@ -231,17 +249,17 @@ def test_c4_window_checkpointed_equals_full_storage(psv, O):
     with psv.Simulation(c) as sim:
         obs = sim.forward()
     res = {}
-    for storage, interval in (("full", 0), ("ckpt", 26), ("ckpt", 100)):
+    for storage, interval, keep in (("full", 0, 0), ("ckpt", 26, 1), ("ckpt", 100, 0)):
         st = psv._lib.STORE_FULL if storage == "full" else psv._lib.STORE_CHECKPOINT
-        with psv.Simulation(c, storage=st, ckpt_interval=interval) as sim:
+        with psv.Simulation(c, storage=st, ckpt_interval=interval, ckpt_keep=keep) as sim:
             plan = sim.plan()
             assert plan["storage"] == ("full" if storage == "full" else "checkpoint")
             sim.set_model(*start)
             sim.set_observed(*obs)
             m = sim.gradient()
-            res[(storage, interval)] = (m, sim.get_gradient())
+            res[(storage, interval, keep)] = (m, sim.get_gradient())
             _print(f"C4 window {storage} {interval}: plan {plan}, misfits {m}")
-    ref = res[("full", 0)]
+    ref = res[("full", 0, 0)]
     assert np.all(ref[0] > 0)
     for key, (m, G) in res.items():
         np.testing.assert_array_equal(m, ref[0])
@@ -249,7 +267,7 @@ def test_c4_window_checkpointed_equals_full_storage(psv, O):
             np.testing.assert_array_equal(G[i], ref[1][i], err_msg=str(key))
 
 
-@pytest.mark.parametrize("variant", [1, 4, 5, 6, 7, 8, 9, 10, 11])
+@pytest.mark.parametrize("variant", [1, 4, 5, 6, 7, 8, 9, 10, 11, 12])
 def test_batched_uneven_equals_per_shot(psv, O, variant):
     c = cases.small_case(O, nshot=4, nt=260, uneven=True)
     med_true = O.mat_av2(c)
@@ -279,10 +297,11 @@ def test_batched_uneven_equals_per_shot(psv, O, variant):
             assert cases.rel_l2(a, obs[s][0]) < 1e-4
 
 
-@pytest.mark.parametrize("variant", [5, 8, 10, 11])
+@pytest.mark.parametrize("variant", [5, 8, 10, 11, 12])
 def test_checkpointed_gradient_independent_of_batch_and_tuning(psv, O, variant):
-    """Checkpointed storage (segments of 30 steps, snapshots every 3rd step): the misfits
-    and gradient of three uneven shots are the same bits whether the shots run 3, 2 + 1
+    """Checkpointed storage (segments of 30 steps, the last three kept by the forward pass,
+    snapshots every 3rd step): the misfits and gradient of three uneven shots are the same
+    bits whether the shots run 3, 2 + 1
     or 1 at a time per launch, and whether the autotuner timed its candidates in the
     context first (on the context's own buffers) or the kernel and work order were
     given, so nothing the tuner leaves behind reaches a result."""
@@ -295,7 +314,8 @@ def test_checkpointed_gradient_independent_of_batch_and_tuning(psv, O, variant):
         kw = dict(kernel_variant=variant)
         if pinned:
             kw.update(pin)
-        with psv.Simulation(c, batch=batch, storage=psv._lib.STORE_CHECKPOINT, ckpt_interval=30, **kw) as sim:
+        with psv.Simulation(c, batch=batch, storage=psv._lib.STORE_CHECKPOINT, ckpt_interval=30, ckpt_keep=3,
+                            **kw) as sim:
             sim.set_model(lam0, mu0, rho0)
             sim.set_observed([o[0] for o in obs], [o[1] for o in obs])
             m = sim.gradient()
@@ -439,24 +459,25 @@ def test_uniform_addressing_variant_matches_variant2(psv, O, fdorder, variant, b
         assert e < 1e-4 and e2 < 1e-4
 
 
-@pytest.mark.parametrize("fdorder,nx,nshot", [(2, 96, 3), (4, 133, 5), (8, 96, 3), (8, 196, 4)])
+@pytest.mark.parametrize("fdorder,nx,nshot", [(2, 96, 3), (4, 133, 5), (8, 96, 3), (8, 196, 4), (8, 300, 4)])
 @pytest.mark.parametrize("tz", [16, 32])
 def test_full_tiles_equal_variant5(psv, O, fdorder, nx, nshot, tz):
     """Variant 10 runs variant 5's arithmetic on 64-column tiles from column 0 with the x
-    halo from a separate load (lane_off_h) instead of lanes given to it, and variant 11
-    runs variant 10 with a block's four wavefronts on four shots of one tile: every tap
-    reads the same value in the same order, so seismograms, stored snapshots, the
-    wavefield, misfits, the gradient and the misfit-derivative sums equal variant 5's bit
-    for bit.  Widths where nx - N is, and is not, a multiple of 64 (a last tile with lanes
-    past the grid); CPML on the left and right (x strips inside the first and last tiles);
-    uneven shots in one batch, 3, 4 or 5 of them (a last group of four with idle
-    wavefronts)."""
+    halo from a separate load (lane_off_h) instead of lanes given to it, variant 11 runs
+    variant 10 with a block's four wavefronts on four shots of one tile, and variant 12
+    with them on four adjacent tiles of one segment: every tap reads the same value in
+    the same order, so seismograms, stored snapshots, the wavefield, misfits, the gradient
+    and the misfit-derivative sums equal variant 5's bit for bit.  Widths where nx - N
+    is, and is not, a multiple of 64 (a last tile with lanes past the grid) and 2 to 5
+    tiles (blocks of variant 12 with idle wavefronts past the last tile); CPML on the left
+    and right (x strips inside the first and last tiles); uneven shots in one batch, 3, 4
+    or 5 of them (a last group of four with idle wavefronts)."""
     c = cases.small_case(O, nshot=nshot, nt=200, fdorder=fdorder, uneven=True, nx=nx)
     med_true = O.mat_av2(c)
     obs = cases.per_shot_obs(c, O, med_true)
     lam0, mu0, rho0 = cases.start_model(c)
     out = {}
-    for v in (5, 10, 11):
+    for v in (5, 10, 11, 12):
         with psv.Simulation(c, kernel_variant=v, tile_tz=tz, storage=psv._lib.STORE_FULL) as sim:
             seis = sim.forward()
             wf = sim.wavefield(2, "sxx")
@@ -469,7 +490,7 @@ def test_full_tiles_equal_variant5(psv, O, fdorder, nx, nshot, tz):
             mg = sim.misfit_gradient(*adj, shot0=0, nshots=1)
             out[v] = (seis, wf, snaps, m, G, mg)
     b = out[5]
-    for v in (10, 11):
+    for v in (10, 11, 12):
         a = out[v]
         for i in range(2):
             np.testing.assert_array_equal(a[0][i], b[0][i], err_msg=f"variant {v}")
@@ -544,13 +565,15 @@ def test_work_orders_agree_bitwise(psv, O):
             np.testing.assert_array_equal(ux, ref[2])
 
 
-@pytest.mark.parametrize("variant", [1, 4, 5, 6, 7, 8, 9, 10, 11])
+@pytest.mark.parametrize("variant", [1, 4, 5, 6, 7, 8, 9, 10, 11, 12])
 @pytest.mark.parametrize("storage", ["full", "ckpt"])
 def test_graph_replay_equals_eager(psv, O, variant, storage):
     """use_graph: chunks of time steps captured once per propagation and replayed
     (64 steps, or one checkpoint segment), with eager steps for the remainder.  The
     same kernels see the same arguments, so seismograms, misfits and gradients are
-    bit-identical to the eager launches; two gradients in a row re-capture."""
+    bit-identical to the eager launches; two gradients in a row re-capture.  With
+    checkpoints the last two of the eight segments are kept by the forward pass (the
+    last one short: eager), the other six recomputed."""
     c = cases.small_case(O, nshot=3, nt=300, snap_dt=2, uneven=True)
     obs = cases.per_shot_obs(c, O, O.mat_av2(c))
     lam0, mu0, rho0 = cases.start_model(c)
@@ -558,7 +581,7 @@ def test_graph_replay_equals_eager(psv, O, variant, storage):
     out = {}
     for g in (0, 1):
         with psv.Simulation(c, kernel_variant=variant, storage=st, ckpt_interval=40, use_graph=g,
-                            batch=2) as sim:
+                            batch=2, ckpt_keep=2) as sim:
             seis = sim.forward()
             sim.set_model(lam0, mu0, rho0)
             sim.set_observed([o[0] for o in obs], [o[1] for o in obs])

@@ -48,8 +48,9 @@ def test_struct_layouts_match_the_header(tmp_path):
         "int main(void) {\n"
         '  printf("%zu %zu %zu %zu %d\\n", sizeof(fwi_grid), sizeof(fwi_acq), sizeof(fwi_opts),\n'
         "         sizeof(fwi_kernel_time), (int)FWI_K_COUNT);\n"
-        '  printf("%zu %zu %zu %zu %zu\\n", offsetof(fwi_grid, taper), offsetof(fwi_opts, work_order),\n'
-        "         offsetof(fwi_acq, rec_x), offsetof(fwi_opts, param), offsetof(fwi_opts, vel_bounds));\n"
+        '  printf("%zu %zu %zu %zu %zu %zu\\n", offsetof(fwi_grid, taper), offsetof(fwi_opts, work_order),\n'
+        "         offsetof(fwi_acq, rec_x), offsetof(fwi_opts, param), offsetof(fwi_opts, vel_bounds),\n"
+        "         offsetof(fwi_opts, ckpt_keep));\n"
         "  return 0;\n}\n")
     exe = tmp_path / "layout"
     subprocess.run(["gcc", "-I", str(ROOT / "include"), str(src), "-o", str(exe)], check=True)
@@ -60,7 +61,7 @@ def test_struct_layouts_match_the_header(tmp_path):
                          C.sizeof(_lib.FwiKernelTime)]
     assert sizes[4] == len(_lib.KERNELS)
     assert offs == [_lib.FwiGrid.taper.offset, _lib.FwiOpts.work_order.offset, _lib.FwiAcq.rec_x.offset,
-                    _lib.FwiOpts.param.offset, _lib.FwiOpts.vel_bounds.offset]
+                    _lib.FwiOpts.param.offset, _lib.FwiOpts.vel_bounds.offset, _lib.FwiOpts.ckpt_keep.offset]
 
 
 def test_default_options(lib):
@@ -69,10 +70,10 @@ def test_default_options(lib):
     lib.fwi_default_opts(C.byref(o))
     assert (o.storage, o.form, o.grad_sign, o.check_finite, o.kernel_variant, o.work_order) == \
         (_lib.STORE_AUTO, _lib.FORM_FUSED, -1, 1, 0, 0)
-    assert lib.fwi_abi_version() == _lib.ABI_VERSION == 4
+    assert lib.fwi_abi_version() == _lib.ABI_VERSION == 5
     assert (o.pcg, o.smooth_hfs) == (0, 0)
     assert o.param == _lib.PARAM_LAME and list(o.vel_bounds) == [0.0] * 4 and o.ckpt_slots == 0
-    assert o.seis_fp64 == 0
+    assert o.seis_fp64 == 0 and o.ckpt_keep == 0
     assert lib.fwi_strerror(-8)
 
 
@@ -144,8 +145,8 @@ def test_invalid_options_are_rejected_before_any_device_work():
     from fwi_cuda_amd import _lib, psv
     O.load("oracle")
     c = cases.small_case(O, nshot=1, nt=10)
-    for kw in ({"tile_tz": 5}, {"kernel_variant": 12}, {"work_order": 5}, {"kernel_variant": 9, "tile_tz": 3},
-               {"ckpt_slots": -1}, {"smooth_hfs": 1}):
+    for kw in ({"tile_tz": 5}, {"kernel_variant": 13}, {"work_order": 5}, {"kernel_variant": 9, "tile_tz": 3},
+               {"ckpt_slots": -1}, {"ckpt_keep": -1}, {"smooth_hfs": 1}):
         with pytest.raises(_lib.FwiError) as e:
             psv.Simulation(c, **kw)
         assert e.value.status == -1, kw

@@ -120,7 +120,7 @@ def kernel_instantiation(kind, tune_log, N):
     base = {"stress": "k_stress", "velocity": "k_velocity"}[kind]
     if var == 1:
         return f"{base}<{N}, {tz}>"
-    if var in (5, 10, 11, 12):   # 10-12: variant 5's kernels on full 64-column tiles
+    if var in (5, 10, 11, 12, 13):   # 10-13: variant 5's kernels on full 64-column tiles
         return f"{base}{var}<{N}, {tz}>" if kind == "stress" else f"{base}{var}<{N}, {tz}, *>"
     if var == 6:   # segments of tz = M blocks of 2N + 2 rows, loads one row ahead
         m = max(1, tz // (2 * N + 2))

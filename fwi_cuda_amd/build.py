@@ -25,8 +25,10 @@ BUILD = PKG / "_build"
 LIB = PKG / "libfwi_psv.so"
 DRIVER = PKG / "fwi_psv"
 
-HIP_SOURCES = ["psv_stencil6.hip", "psv_step7.hip", "psv_kernels.hip", "psv_misc.hip", "psv_plain.hip",
-               "psv_api.hip", "psv_noalloc.hip", "psv_exact.hip", "psv_fp64.hip"]
+HIP_SOURCES = [  # the longest compiles first
+    "psv_step7_b.hip", "psv_step7_c.hip", "psv_step7.hip", "psv_stencil6.hip", "psv_v5.hip", "psv_v11.hip",
+    "psv_v13.hip", "psv_kernels.hip", "psv_misc.hip", "psv_plain.hip", "psv_api.hip", "psv_noalloc.hip",
+    "psv_exact.hip", "psv_fp64.hip"]
 CPP_SOURCES = ["psv_preproc.cpp"]
 DRIVER_SOURCES = ["host/psv_io.cpp", "host/psv_main.cpp"]
 

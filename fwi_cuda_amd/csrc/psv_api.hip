@@ -618,7 +618,7 @@ int autotune(fwi_ctx *c, int nb) {
     const bool order_forced = c->o.work_order >= 1 && c->o.work_order <= 4;
     // candidates (variant, rows per segment, prefetch depth); the one-pass step only where it applies
     std::vector<TuneVal> cand;
-    for (int var = 1; var <= 12; ++var) {
+    for (int var = 1; var <= 13; ++var) {
         if ((fv && var != fv) || (one_pass(var) && !c->fusable) || (var >= 5 && !c->ztab_ok)) continue;
         if (var == 9) {   // balanced: one or two waves of resident blocks
             for (int m : {1, 2})
@@ -643,7 +643,7 @@ int autotune(fwi_ctx *c, int nb) {
                        fv == 4   ? " (the one-pass step takes tile_tz 32 or 64, no free surface other than "
                                    "the top one at row fdorder/2, and sources and receivers inside the interior)"
                        : fv == 5 || fv >= 10
-                           ? " (variants 5, 10, 11 and 12 take tile_tz 16 or 32, and z CPML strips of at most 64 rows "
+                           ? " (variants 5 and 10 to 13 take tile_tz 16 or 32, and z CPML strips of at most 64 rows "
                              "at the top and bottom of the grid)"
                        : fv == 6 ? " (variant 6 takes tile_tz 0, fdorder + 2 or 2 fdorder + 4, and z CPML "
                                    "strips of at most 64 rows at the top and bottom of the grid)"
@@ -1587,8 +1587,8 @@ int validate(fwi_ctx *c, const fwi_grid *g, const fwi_acq *q) {
         return c->fail(FWI_EINVAL,
                        "tile_tz %d (0, 8, 16, 32 or 64; variant 6: 0 or its row block times 1 or 2; "
                        "variants 7, 8: 0 or their row block times 4 or 8)", tt);
-    if (c->o.kernel_variant < 0 || c->o.kernel_variant > 12)
-        return c->fail(FWI_EINVAL, "kernel_variant %d (0..12)", c->o.kernel_variant);
+    if (c->o.kernel_variant < 0 || c->o.kernel_variant > 13)
+        return c->fail(FWI_EINVAL, "kernel_variant %d (0..13)", c->o.kernel_variant);
     if (c->o.kernel_variant == 9 && tt > 2) return c->fail(FWI_EINVAL, "variant 9 takes tile_tz 0, 1 or 2");
     if (c->o.work_order < 0 || c->o.work_order > 4)
         return c->fail(FWI_EINVAL, "work_order %d (0..4)", c->o.work_order);

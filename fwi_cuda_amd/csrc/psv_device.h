@@ -122,6 +122,25 @@ __device__ __forceinline__ void st_at(T *base, uint32_t off, T v) {
 }
 __device__ __forceinline__ int wave_uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
+// ld_at / st_at of a value used once per launch: with NT the access carries the
+// non-temporal hint, so its line is not kept in the caches for reuse (fields updated in
+// place, sums, snapshots; not the z-queue rows and halo columns that neighbouring
+// segments and tiles read again, nor the media that every shot reads)
+template <bool NT, class T>
+__device__ __forceinline__ T ld_once(const T *base, uint32_t off) {
+    const T *p = reinterpret_cast<const T *>(reinterpret_cast<const char *>(base) + off);
+    if constexpr (NT) return __builtin_nontemporal_load(p);
+    return *p;
+}
+template <bool NT, class T>
+__device__ __forceinline__ void st_once(T *base, uint32_t off, T v) {
+    T *p = reinterpret_cast<T *>(reinterpret_cast<char *>(base) + off);
+    if constexpr (NT)
+        __builtin_nontemporal_store(v, p);
+    else
+        *p = v;
+}
+
 // Fields of the launch's KArgs loaded again from the kernel-argument segment where they
 // are used, for blocks that run on a few rows only, such as the CPML strips: they then
 // keep no base pointer live in SGPRs across the row loop.  The kernels that use it take

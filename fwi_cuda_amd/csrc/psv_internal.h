@@ -154,6 +154,17 @@ bool launch6(int which, const KArgs &a, const ZTab &zt, int N, int depth, int tz
 // other in z.  Same conditions as launch_step (top free surface at row N only, interior
 // sources); false for a launch mode it has no kernel for.
 bool launch_step7(const KArgs &a, const ZTab &zt, int N, int tz, bool xw, hipStream_t st);
+// launch_step7's kernels for N = 4, 5, 6 and for N = 5, 6 on a grid it has sized
+// (psv_step7_b.hip, psv_step7_c.hip)
+bool launch_step7_high(const KArgs &a, const ZTab &zt, int N, int M, bool xw, dim3 grid, int mult, hipStream_t st);
+bool launch_step7_top(const KArgs &a, const ZTab &zt, int N, int M, bool xw, dim3 grid, int mult, hipStream_t st);
+// A half-step (stress, or velocity) of stencil variant 5, 10, 11, 12 or 13 on the given grid
+// (psv_v5.hip, psv_v11.hip, psv_v13.hip); false: a velocity launch mode it has no kernel for.
+bool launch_stencil5_v5(bool velocity, const KArgs &a, const ZTab &zt, int N, int tz, dim3 grid, hipStream_t st);
+bool launch_stencil5_v10(bool velocity, const KArgs &a, const ZTab &zt, int N, int tz, dim3 grid, hipStream_t st);
+bool launch_stencil5_v11(bool velocity, const KArgs &a, const ZTab &zt, int N, int tz, dim3 grid, hipStream_t st);
+bool launch_stencil5_v12(bool velocity, const KArgs &a, const ZTab &zt, int N, int tz, dim3 grid, hipStream_t st);
+bool launch_stencil5_v13(bool velocity, const KArgs &a, const ZTab &zt, int N, int tz, dim3 grid, hipStream_t st);
 // stress, top free surface and velocity of one time step in one pass (kernel variant 4);
 // tz = rows per segment (32, 64), depth = rows loaded ahead (1 is the one instantiated)
 void launch_step(const KArgs &a, int N, int tz, int depth, hipStream_t st);

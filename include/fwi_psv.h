@@ -133,7 +133,7 @@ typedef struct {
                                times then only see the replays (FWI_K_GRAPH) */
     int32_t tile_tz;        /* rows per thread in the stencil kernels; 0 = tuned;
                                8, 16, 32, 64 (variants 1-3), 32, 64 (variant 4), 16, 32
-                               (variants 5, 10-12), fdorder + 2 or 2 fdorder + 4 (variant 6),
+                               (variants 5, 10-13), fdorder + 2 or 2 fdorder + 4 (variant 6),
                                4 fdorder + 8 or 8 fdorder + 16 (variants 7, 8), 1 or 2
                                (variant 9: waves of resident blocks) */
     int32_t device;         /* HIP device ordinal */
@@ -155,8 +155,10 @@ typedef struct {
                                11 = variant 10 with a block's four wavefronts on four
                                    shots of one tile and segment (shared media rows),
                                12 = variant 10 with a block's four wavefronts on four
-                                   adjacent tiles of one segment (1 KiB row runs)
-                               (5 to 12 need z CPML strips of at most 64 rows, at the
+                                   adjacent tiles of one segment (1 KiB row runs),
+                               13 = variant 12 with the non-temporal hint on the loads and
+                                   stores of values used once per launch
+                               (5 to 13 need z CPML strips of at most 64 rows, at the
                                top and bottom of the grid) */
     int32_t work_order;     /* order of the stencil tiles over the grid: 0 = tuned,
                                1 = shot fastest then z then x, 2 = shot fastest then x then z,

@@ -85,7 +85,7 @@ def test_shot_gradient_matches_reference_golden(psv, name):
 
 
 @pytest.mark.parametrize("surface", [True, False])
-@pytest.mark.parametrize("variant", [1, 2, 4, 5, 6, 7, 8, 9, 10, 11, 12])
+@pytest.mark.parametrize("variant", [1, 2, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13])
 def test_order8_forward_vs_oracle(psv, O, surface, variant):
     c = cases.small_case(O, surface=surface, nshot=2)
     med = O.mat_av2(c)
@@ -100,7 +100,7 @@ def test_order8_forward_vs_oracle(psv, O, surface, variant):
         assert e < 1e-4
 
 
-@pytest.mark.parametrize("variant", [1, 2, 4, 5, 6, 7, 8, 9, 10, 11, 12])
+@pytest.mark.parametrize("variant", [1, 2, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13])
 def test_order8_gradient_vs_oracle(psv, O, variant):
     c = cases.small_case(O, nshot=2, nt=450)
     med_true = O.mat_av2(c)
@@ -126,7 +126,7 @@ def test_order8_gradient_vs_oracle(psv, O, variant):
             assert e < 2e-3
 
 
-@pytest.mark.parametrize("variant", [1, 4, 5, 6, 7, 8, 9, 10, 11, 12])
+@pytest.mark.parametrize("variant", [1, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13])
 def test_checkpointed_equals_full_storage(psv, O, variant):
     c = cases.small_case(O, nshot=2, nt=300, snap_dt=2)
     med_true = O.mat_av2(c)
@@ -267,7 +267,7 @@ def test_c4_window_checkpointed_equals_full_storage(psv, O):
             np.testing.assert_array_equal(G[i], ref[1][i], err_msg=str(key))
 
 
-@pytest.mark.parametrize("variant", [1, 4, 5, 6, 7, 8, 9, 10, 11, 12])
+@pytest.mark.parametrize("variant", [1, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13])
 def test_batched_uneven_equals_per_shot(psv, O, variant):
     c = cases.small_case(O, nshot=4, nt=260, uneven=True)
     med_true = O.mat_av2(c)
@@ -297,7 +297,7 @@ def test_batched_uneven_equals_per_shot(psv, O, variant):
             assert cases.rel_l2(a, obs[s][0]) < 1e-4
 
 
-@pytest.mark.parametrize("variant", [5, 8, 10, 11, 12])
+@pytest.mark.parametrize("variant", [5, 8, 10, 11, 12, 13])
 def test_checkpointed_gradient_independent_of_batch_and_tuning(psv, O, variant):
     """Checkpointed storage (segments of 30 steps, the last three kept by the forward pass,
     snapshots every 3rd step): the misfits and gradient of three uneven shots are the same
@@ -464,8 +464,9 @@ def test_uniform_addressing_variant_matches_variant2(psv, O, fdorder, variant, b
 def test_full_tiles_equal_variant5(psv, O, fdorder, nx, nshot, tz):
     """Variant 10 runs variant 5's arithmetic on 64-column tiles from column 0 with the x
     halo from a separate load (lane_off_h) instead of lanes given to it, variant 11 runs
-    variant 10 with a block's four wavefronts on four shots of one tile, and variant 12
-    with them on four adjacent tiles of one segment: every tap reads the same value in
+    variant 10 with a block's four wavefronts on four shots of one tile, variant 12 with
+    them on four adjacent tiles of one segment, and variant 13 is variant 12 with the
+    non-temporal hint on single-use loads and stores: every tap reads the same value in
     the same order, so seismograms, stored snapshots, the wavefield, misfits, the gradient
     and the misfit-derivative sums equal variant 5's bit for bit.  Widths where nx - N
     is, and is not, a multiple of 64 (a last tile with lanes past the grid) and 2 to 5
@@ -477,7 +478,7 @@ def test_full_tiles_equal_variant5(psv, O, fdorder, nx, nshot, tz):
     obs = cases.per_shot_obs(c, O, med_true)
     lam0, mu0, rho0 = cases.start_model(c)
     out = {}
-    for v in (5, 10, 11, 12):
+    for v in (5, 10, 11, 12, 13):
         with psv.Simulation(c, kernel_variant=v, tile_tz=tz, storage=psv._lib.STORE_FULL) as sim:
             seis = sim.forward()
             wf = sim.wavefield(2, "sxx")
@@ -490,7 +491,7 @@ def test_full_tiles_equal_variant5(psv, O, fdorder, nx, nshot, tz):
             mg = sim.misfit_gradient(*adj, shot0=0, nshots=1)
             out[v] = (seis, wf, snaps, m, G, mg)
     b = out[5]
-    for v in (10, 11, 12):
+    for v in (10, 11, 12, 13):
         a = out[v]
         for i in range(2):
             np.testing.assert_array_equal(a[0][i], b[0][i], err_msg=f"variant {v}")
@@ -565,7 +566,7 @@ def test_work_orders_agree_bitwise(psv, O):
             np.testing.assert_array_equal(ux, ref[2])
 
 
-@pytest.mark.parametrize("variant", [1, 4, 5, 6, 7, 8, 9, 10, 11, 12])
+@pytest.mark.parametrize("variant", [1, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13])
 @pytest.mark.parametrize("storage", ["full", "ckpt"])
 def test_graph_replay_equals_eager(psv, O, variant, storage):
     """use_graph: chunks of time steps captured once per propagation and replayed

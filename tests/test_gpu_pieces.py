@@ -69,7 +69,7 @@ def test_taper_matches_reference_output(psv, O):
         np.testing.assert_allclose(g, want, rtol=1e-12, atol=1e-300)
 
 
-@pytest.mark.parametrize("variant", [1, 5, 7, 8, 9, 10, 11, 12])
+@pytest.mark.parametrize("variant", [1, 5, 7, 8, 9, 10, 11, 12, 13])
 def test_strided_snapshot_gradient_vs_oracle(psv, O, variant):
     """Snapshots on a lattice of every 2nd row and 3rd column (snap_dz = 2,
     snap_dx = 3, snap_dt = 2, the strides of the golden interpol_grad2 piece): the
@@ -152,7 +152,7 @@ SURFACES = {
 }
 
 
-@pytest.mark.parametrize("variant", [1, 2, 5, 6, 10, 11, 12])
+@pytest.mark.parametrize("variant", [1, 2, 5, 6, 10, 11, 12, 13])
 @pytest.mark.parametrize("name", sorted(SURFACES))
 def test_free_surfaces_on_every_side_vs_oracle(psv, O, name, variant):
     """surf_mirror on the bottom, left and right sides (src/cpu/n_kernel_lib_PSV.cpp:317-394):
@@ -181,7 +181,7 @@ def test_free_surfaces_on_every_side_vs_oracle(psv, O, name, variant):
         assert np.all(st["sxx"][4:-4, isurf[2]] == 0) and np.all(st["szx"][4:-4, isurf[2]] == 0)
 
 
-@pytest.mark.parametrize("variant", [1, 6, 10, 11, 12])
+@pytest.mark.parametrize("variant", [1, 6, 10, 11, 12, 13])
 def test_free_surfaces_gradient_vs_oracle(psv, O, variant):
     """A gradient with surfaces on the bottom, left and right (the adjoint pass
     applies them too): misfit and preconditioned gradient against the FP64 oracle, to

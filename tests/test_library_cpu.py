@@ -145,7 +145,7 @@ def test_invalid_options_are_rejected_before_any_device_work():
     from fwi_cuda_amd import _lib, psv
     O.load("oracle")
     c = cases.small_case(O, nshot=1, nt=10)
-    for kw in ({"tile_tz": 5}, {"kernel_variant": 13}, {"work_order": 5}, {"kernel_variant": 9, "tile_tz": 3},
+    for kw in ({"tile_tz": 5}, {"kernel_variant": 14}, {"work_order": 5}, {"kernel_variant": 9, "tile_tz": 3},
                {"ckpt_slots": -1}, {"ckpt_keep": -1}, {"smooth_hfs": 1}):
         with pytest.raises(_lib.FwiError) as e:
             psv.Simulation(c, **kw)

@@ -119,6 +119,7 @@ struct fwi_ctx {
     // one-pass step: second buffer set of the 5 fields and the stress-side psi
     // (pvz_z pvx_z pvx_x pvz_x); f / psi always name the current time level
     bool fusable = false;               // k_step supports this grid, surface and acquisition
+    bool surf_top_only = false;         // the only free surface is the top one at row N
     ZTab ztab{};                        // z profiles of the strip rows for variant 5
     bool ztab_ok = false;               // the z strips are [N, lo) and [hi, nz - N), at most kZTabRows rows
     float *f2[5] = {}, *psi2[4] = {};
@@ -335,6 +336,7 @@ KArgs base_args(fwi_ctx *c, int nb, const Wave &w) {
     a.xp = c->xp;
     set_state(a, w);
     a.surf_top = c->g.isurf[0] > 0 ? c->g.isurf[0] : -1;   // used by k_step only (fusable: row N)
+    a.surf5 = -1;                                          // set per launch by half_steps
     a.snap_z1 = c->g.snap_z1;
     a.snap_z2 = c->g.snap_z2;
     a.snap_x1 = c->g.snap_x1;
@@ -436,11 +438,20 @@ void half_steps(fwi_ctx *c, const KArgs &a) {
         return;
     }
     const ZTab *zt = c->ztab_ok ? &c->ztab : nullptr;
+    // variants 5 and 10-13 apply a free surface at the top row N (and none elsewhere) in
+    // the stress kernel itself, as the one-pass step does
+    const bool surf_in_stress = c->surf_top_only && zt && (c->var == 5 || c->var >= 10);
     {
         Timer t(c, FWI_K_STRESS);
-        launch_stress(a, c->N, c->tz, c->var, c->st, zt, c->depth);
+        if (surf_in_stress) {
+            KArgs b = a;
+            b.surf5 = c->N;
+            launch_stress(b, c->N, c->tz, c->var, c->st, zt, c->depth);
+        } else {
+            launch_stress(a, c->N, c->tz, c->var, c->st, zt, c->depth);
+        }
     }
-    if (has_surface(c)) {
+    if (has_surface(c) && !surf_in_stress) {
         Timer t(c, FWI_K_SURFACE);
         launch_surface(a, c->N, c->g.isurf, c->st);
     }
@@ -1466,6 +1477,7 @@ int setup_device(fwi_ctx *c, const fwi_acq *q) {
     auto interior = [&](int z, int x) { return z >= N && z < nz - N && x >= N && x < nx - N; };
     c->fusable = !c->plain && c->g.isurf[1] == 0 && c->g.isurf[2] == 0 && c->g.isurf[3] == 0 &&
                  (c->g.isurf[0] == 0 || c->g.isurf[0] == N);
+    c->surf_top_only = c->g.isurf[0] == N && c->g.isurf[1] == 0 && c->g.isurf[2] == 0 && c->g.isurf[3] == 0;
     for (int i = 0; c->fusable && i < c->nsrc_tot; ++i) c->fusable = interior(q->src_z[i], q->src_x[i]);
     for (int i = 0; c->fusable && i < c->nrec_tot; ++i) c->fusable = interior(q->rec_z[i], q->rec_x[i]);
     RETIF(plan(c));

@@ -96,6 +96,7 @@ struct KArgs {
     float *o_vz, *o_vx, *o_szz, *o_szx, *o_sxx;
     float *o_pvz_z, *o_pvx_z, *o_pvx_x, *o_pvz_x;
     int surf_top;              // free-surface row handled inside k_step (== N), or -1
+    int surf5;                 // the same inside the stress kernel of variants 5, 10-13 (== N), or -1
     // snapshot of this step (gard_fwd_storage2) written by k_step and patched by k_inject
     int store;
     float *st_vz, *st_vx, *st_szz, *st_szx, *st_sxx;  // slab of shot 0

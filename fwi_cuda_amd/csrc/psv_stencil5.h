@@ -166,10 +166,18 @@ __device__ __forceinline__ void stress5_body(const KArgs &a, const ZTab &zt) {
             vz_x = vz_x * xq.ikh + p2;
         }
         const float div = l * (vx_x + vz_z);
+        float nzz = c_zz + dt * (div + 2.f * m * vz_z);
+        float nzx = c_zx + dt * mzx * (vz_x + vx_z);
+        float nxx = c_xx + dt * (div + 2.f * m * vx_x);
+        if (z == a.surf5) {   // wave-uniform; surf_mirror at row N: no image rows (src/cpu/n_kernel_lib_PSV.cpp:297-301)
+            nzz = 0.f;
+            nzx = 0.f;
+            nxx = 4.f * dt * vx_x * (l * m + m * m) / (l + 2.f * m);
+        }
         const uint32_t so = vo + dtr;
-        st_once<NT>(szz, so, c_zz + dt * (div + 2.f * m * vz_z));
-        st_once<NT>(szx, so, c_zx + dt * mzx * (vz_x + vx_z));
-        st_once<NT>(sxx, so, c_xx + dt * (div + 2.f * m * vx_x));
+        st_once<NT>(szz, so, nzz);
+        st_once<NT>(szx, so, nzx);
+        st_once<NT>(sxx, so, nxx);
     };
     int z = z0;
     uint32_t vo = c4;

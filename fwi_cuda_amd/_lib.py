@@ -20,7 +20,7 @@ STATUS = {0: "FWI_OK", -1: "FWI_EINVAL", -2: "FWI_ENOMEM", -3: "FWI_EHIP", -4: "
 STORE_AUTO, STORE_FULL, STORE_CHECKPOINT = 0, 1, 2
 FORM_FUSED, FORM_PLAIN = 0, 1
 PARAM_LAME, PARAM_VELOCITY = 0, 1
-ABI_VERSION = 5
+ABI_VERSION = 6
 KERNELS = ("stress", "velocity", "surface", "source", "receiver", "store", "other", "step", "graph")
 
 
@@ -52,7 +52,7 @@ class FwiOpts(C.Structure):
 
 
 class FwiKernelTime(C.Structure):
-    _fields_ = [("launches", C.c_int64), ("total_ms", C.c_double)]
+    _fields_ = [("launches", C.c_int64), ("total_ms", C.c_double), ("timed", C.c_int64)]
 
 
 class FwiError(RuntimeError):
@@ -116,6 +116,7 @@ def load() -> C.CDLL:
         "fwi_profile": (C.c_int32, [vp, C.c_int32]),
         "fwi_profile_kernels": (C.c_int32, [vp, C.c_uint32]),
         "fwi_kernel_times": (C.c_int32, [vp, C.POINTER(FwiKernelTime)]),
+        "fwi_profile_stride": (C.c_int32, [vp, C.c_int32]),
         "fwi_sync": (C.c_int32, [vp]),
         "fwi_mat_av2_noalloc": (C.c_int32, [vp, vp, vp, C.c_int32, C.c_int32, vp, vp, vp, vp, vp, vp, vp]),
     }
@@ -156,6 +157,6 @@ EXPORTED = ("fwi_abi_version", "fwi_strerror", "fwi_last_error", "fwi_default_op
             "fwi_gradient", "fwi_gradient_raw", "fwi_misfit_gradient", "fwi_zero_gradient", "fwi_get_gradient",
             "fwi_set_gradient",
             "fwi_taper_gradient", "fwi_misfit", "fwi_save_model", "fwi_update_model", "fwi_step_length",
-            "fwi_iteration", "fwi_tuning", "fwi_profile", "fwi_profile_kernels", "fwi_kernel_times", "fwi_sync", "fwi_forward_snapshots",
+            "fwi_iteration", "fwi_tuning", "fwi_profile", "fwi_profile_kernels", "fwi_profile_stride", "fwi_kernel_times", "fwi_sync", "fwi_forward_snapshots",
             "fwi_mat_av2_noalloc", "fwi_checkpoint_plan", "fwi_checkpoint_kept", "fwi_set_model_device", "fwi_forward_device",
             "fwi_misfit_gradient_device", "fwi_misfit_gradient_exact", "fwi_pcg_direction", "fwi_smooth_gradient")

@@ -145,7 +145,7 @@ struct fwi_ctx {
     std::string tune_log;
 
     // ---- profiling
-    int prof_on = 0;
+    int prof_on = 0, prof_stride = 1;
     std::vector<hipEvent_t> ev_pool;
     std::vector<EventPair> ev_pending;
     fwi_kernel_time times[FWI_K_COUNT] = {};
@@ -207,7 +207,7 @@ void prof_take(fwi_ctx *c, size_t n) {
         const EventPair &p = c->ev_pending[i];
         float ms = 0;
         hipEventElapsedTime(&ms, p.a, p.b);
-        c->times[p.kid].launches += 1;
+        c->times[p.kid].timed += 1;
         c->times[p.kid].total_ms += ms;
         c->ev_pool.push_back(p.a);
         c->ev_pool.push_back(p.b);
@@ -224,9 +224,11 @@ struct Timer {
     int kid;
     hipEvent_t a = nullptr;
     Timer(fwi_ctx *c_, int k) : c(c_), kid(k) {
-        if (c->prof_on & (1 << kid)) {
-            a = ev_get(c);
-            hipEventRecord(a, c->st);
+        if (c->prof_on & (1 << kid)) {   // every launch counts; every prof_stride-th is timed
+            if (c->times[kid].launches++ % c->prof_stride == 0) {
+                a = ev_get(c);
+                hipEventRecord(a, c->st);
+            }
         }
     }
     ~Timer() {
@@ -2479,6 +2481,14 @@ int32_t fwi_profile_kernels(fwi_ctx *c, uint32_t mask) {
     RETIF(sync(c));
     c->prof_on = (int)(mask & ((1u << FWI_K_COUNT) - 1));
     if (c->prof_on) std::memset(c->times, 0, sizeof c->times);
+    return FWI_OK;
+}
+
+int32_t fwi_profile_stride(fwi_ctx *c, int32_t stride) {
+    if (!c) return FWI_EINVAL;
+    if (stride < 1) return c->fail(FWI_EINVAL, "profile stride %d < 1", stride);
+    RETIF(sync(c));
+    c->prof_stride = stride;
     return FWI_OK;
 }
 

@@ -435,9 +435,11 @@ class Simulation:
         self._chk(self.lib.fwi_forward_snapshots(self.ctx, shot, dptr(uz), dptr(ux), *[dptr(a) for a in cubes]))
         return (uz, ux), dict(zip(("vz", "vx", "szz", "szx", "sxx"), cubes))
 
-    def profile(self, on=True, kernels=None):
+    def profile(self, on=True, kernels=None, stride=1):
         """Per-launch HIP-event timing of every kernel class, or of the named ones only
-        (names of _lib.KERNELS, e.g. ("stress", "velocity", "step"))."""
+        (names of _lib.KERNELS, e.g. ("stress", "velocity", "step")); with stride k only
+        every k-th launch of a class carries events."""
+        self._chk(self.lib.fwi_profile_stride(self.ctx, int(stride)))
         if kernels is None or not on:
             self._chk(self.lib.fwi_profile(self.ctx, int(on)))
             return
@@ -447,9 +449,10 @@ class Simulation:
         self._chk(self.lib.fwi_profile_kernels(self.ctx, mask))
 
     def kernel_times(self) -> dict:
+        """{kernel class: (launches, summed ms of the timed launches, timed launches)}."""
         t = (FwiKernelTime * len(_lib.KERNELS))()   # FWI_K_COUNT entries
         self._chk(self.lib.fwi_kernel_times(self.ctx, t))
-        return {n: (t[i].launches, t[i].total_ms) for i, n in enumerate(_lib.KERNELS)}
+        return {n: (t[i].launches, t[i].total_ms, t[i].timed) for i, n in enumerate(_lib.KERNELS)}
 
     def sync(self):
         self._chk(self.lib.fwi_sync(self.ctx))

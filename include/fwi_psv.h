@@ -52,7 +52,7 @@
 extern "C" {
 #endif
 
-#define FWI_PSV_ABI_VERSION 5
+#define FWI_PSV_ABI_VERSION 6
 
 enum fwi_status {
     FWI_OK = 0,
@@ -187,11 +187,13 @@ typedef struct {
                                at most k (ABI 5) */
 } fwi_opts;
 
-/* Per-kernel timing collected while profiling is on (HIP events on the
- * context's stream around every launch of that kernel). */
+/* Per-kernel timing collected while profiling is on: HIP events on the context's
+ * stream around every launch of that kernel, or around every stride-th one
+ * (fwi_profile_stride). */
 typedef struct {
-    int64_t launches;
-    double total_ms;
+    int64_t launches;   /* launches while profiling was on */
+    double total_ms;    /* summed duration of the timed ones */
+    int64_t timed;      /* launches that carried events (ABI 6) */
 } fwi_kernel_time;
 
 enum fwi_kernel_id {
@@ -360,6 +362,10 @@ int32_t fwi_profile(fwi_ctx *ctx, int32_t enable);
  * counterpart in the reference): each timed launch costs two event records, so timing
  * only the kernel whose duration matters disturbs the run less.  0 switches timing off. */
 int32_t fwi_profile_kernels(fwi_ctx *ctx, uint32_t mask);
+/* Events on every stride-th launch of each timed kernel class only (1 = every launch,
+ * the default; no counterpart in the reference): the average duration is then sampled
+ * uniformly over the launches while their event records cost 1/stride as much (ABI 6). */
+int32_t fwi_profile_stride(fwi_ctx *ctx, int32_t stride);
 int32_t fwi_kernel_times(fwi_ctx *ctx, fwi_kernel_time times[FWI_K_COUNT]);
 int32_t fwi_sync(fwi_ctx *ctx);
 

@@ -267,3 +267,23 @@ def test_gradient_smoothing_vs_oracle(psv, O, hfs):
         assert not np.allclose(ref, G[p])
     assert filt.sum() == pytest.approx(1.0)
     _print(f"Gaussian smoothing hfs {hfs}: equal to the oracle's to 1e-12")
+
+
+@pytest.mark.parametrize("stride", [1, 3])
+def test_sampled_kernel_timing(psv, O, stride):
+    """fwi_profile_stride: every launch of a timed kernel class is counted and every
+    stride-th one carries events; classes left untimed count nothing."""
+    nt = 50
+    c = cases.small_case(O, nshot=2, nt=nt)
+    with psv.Simulation(c, kernel_variant=13) as sim:
+        assert sim.plan()["batch"] == 2
+        sim.forward()   # the tuner's launches stay out of the counts below
+        sim.profile(True, kernels=("stress", "velocity"), stride=stride)
+        sim.forward()
+        kt = sim.kernel_times()
+        sim.profile(False)
+    for k in ("stress", "velocity"):
+        n, ms, timed = kt[k]
+        assert n == nt and timed == (nt + stride - 1) // stride and ms > 0, (k, kt[k])
+    assert kt["source"] == (0, 0.0, 0) and kt["receiver"] == (0, 0.0, 0)
+    _print(f"stride {stride}: {kt['stress']}, {kt['velocity']}")

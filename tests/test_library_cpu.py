@@ -70,7 +70,7 @@ def test_default_options(lib):
     lib.fwi_default_opts(C.byref(o))
     assert (o.storage, o.form, o.grad_sign, o.check_finite, o.kernel_variant, o.work_order) == \
         (_lib.STORE_AUTO, _lib.FORM_FUSED, -1, 1, 0, 0)
-    assert lib.fwi_abi_version() == _lib.ABI_VERSION == 5
+    assert lib.fwi_abi_version() == _lib.ABI_VERSION == 6
     assert (o.pcg, o.smooth_hfs) == (0, 0)
     assert o.param == _lib.PARAM_LAME and list(o.vel_bounds) == [0.0] * 4 and o.ckpt_slots == 0
     assert o.seis_fp64 == 0 and o.ckpt_keep == 0

@@ -234,6 +234,8 @@ def test_bench_names_the_tuned_instance_and_finds_its_traffic(tmp_path):
     assert bench.kernel_instantiation("stress", "chosen o1/v10/tz16/d1; ", 4) == "k_stress10<4, 16>"
     assert bench.kernel_instantiation("velocity", "chosen o0/v11/tz16/d1; ", 4) == "k_velocity11<4, 16, *>"
     assert bench.kernel_instantiation("stress", "chosen o2/v2/tz16/d1; ", 4) == "k_stress2<4, 16, 1>"
+    assert bench.kernel_instantiation("velocity", "chosen o3/v13/tz16/d1; ", 4) == "k_velocity13<4, 16, *>"
+    assert bench.kernel_instantiation("stress", "chosen o3/v12/tz32/d1; ", 4) == "k_stress12<4, 32>"
     f = tmp_path / "traffic.txt"
     f.write_text("header\n"
                  "traffic k_step7<4, 4, *, false> gradient-mix bytes_per_launch=9.0e9 "
@@ -252,6 +254,8 @@ def test_bench_names_the_tuned_instance_and_finds_its_traffic(tmp_path):
     lines = [l for l in (ROOT / "profiles" / "roofline_traffic.txt").read_text().splitlines()
              if l.startswith("traffic ")]
     assert lines and all("bytes_per_launch=" in l and "algorithmic_bytes_per_launch=" in l for l in lines)
+    # the bench's default run (batch 8, 22 of 35 units kept) finds the tuned instance's line
+    assert bench.measured_traffic("k_velocity13<4, 16, *>", 952501703.0) == pytest.approx(8.888624e8)
 
 
 def test_bench_shot_groups_cover_the_workload():

@@ -84,11 +84,12 @@ def test_backward_matches_misfit_finite_differences(setup, adjoint):
         assert e < tol, (name, e)
 
 
-@pytest.mark.parametrize("adjoint", ["correlation", "exact"])
-def test_c2_directional_derivatives(adjoint):
+@pytest.mark.parametrize("adjoint,seis64", [("correlation", 0), ("exact", 0), ("exact", 1)])
+def test_c2_directional_derivatives(adjoint, seis64):
     """C2 (301 x 601, 10 layers, 8 shots, 120 receivers, 250 Hz, nt 3000): the
     backward pass along smooth random directions against central differences of
-    the misfit along the same directions (relative step 1e-3), all on the GPU."""
+    the misfit along the same directions (relative step 1e-3), all on the GPU; with
+    seis64 the misfits (and the adjoint sources) come from the FP64 propagation."""
     import torch
     from scipy.ndimage import gaussian_filter
     from fwi_cuda_amd import autograd as A, configs, psv
@@ -100,7 +101,7 @@ def test_c2_directional_derivatives(adjoint):
     z1, z2, x1, x2 = (int(v) for v in c["snap"][:4])
     rng = np.random.default_rng(22)
     errs = {}
-    with psv.Simulation(c) as sim:
+    with psv.Simulation(c, seis_fp64=seis64) as sim:
         obs = sim.forward()
         sim.set_observed(*obs)
         model = [torch.tensor(m, requires_grad=True) for m in start]
@@ -122,18 +123,20 @@ def test_c2_directional_derivatives(adjoint):
             fd = (Js[0] - Js[1]) / (2 * h)
             ad = float(np.sum(model[p].grad.numpy() * d))
             e = abs(ad - fd) / abs(fd)
-            _print(f"C2 directional derivative ({adjoint}) along a smooth {name} direction: backward {ad:.6e}, "
-                   f"finite difference {fd:.6e}, relative difference {e:.3e}")
+            _print(f"C2 directional derivative ({adjoint}, FP64 seismograms {seis64}) along a smooth {name} "
+                   f"direction: backward {ad:.6e}, finite difference {fd:.6e}, relative difference {e:.3e}")
             errs[name] = e
-    # correlation form measured: lam 0.22 (its derivative along this direction is 1e3 times
-    # smaller than mu's and rho's, so the O(dt) pairing error of the stress correlation
-    # shows), mu 5e-3, rho 1e-4
-    # exact adjoint: mu and rho to 1e-2.  lam's derivative along this direction (~3e-27) is
-    # below what the GPU's own FP32 misfits resolve at this step (their rounding noise is
-    # ~1e-29 in J, 1e-7 of it, against a change of 7e-30): the finite difference is noise
-    # for lam, so only its sign is asserted here; the exact lam derivative is pinned by the
-    # oracle's adjoint (1e-4), the dot-product test (1e-5) and FP64 finite differences (1e-2)
-    tols = (0.3, 0.05, 0.01) if adjoint == "correlation" else (1.0, 1e-2, 1e-2)
+    # lam's derivative along this direction (3.42e-27) is 1e3 times smaller than mu's and
+    # rho's and below what FP32 misfits resolve at this step (their rounding noise is ~1e-29
+    # in J, 1e-7 of it, against a change of 7e-30): with FP32 seismograms the lam finite
+    # difference is noise (7.8e-27), so only its sign is asserted there.  With the misfits
+    # from the FP64 propagation (seis64) the finite difference resolves it, and the exact
+    # adjoint matches all three: measured lam 7.3e-5, mu 1.8e-5, rho 2.7e-6.  The correlation
+    # form measured against the FP32 differences: lam 0.22, mu 5.4e-3, rho 1.1e-4; against the
+    # FP64 ones its lam derivative along this direction is 1.76 times the true one (the O(dt)
+    # pairing of forward and adjoint stresses), mu 5.4e-3, rho 1.1e-4.
+    tols = {("correlation", 0): (0.3, 0.05, 0.01), ("exact", 0): (1.0, 1e-2, 1e-2),
+            ("exact", 1): (1e-3, 1e-3, 1e-3)}[(adjoint, seis64)]
     for name, tol in zip(("lam", "mu", "rho"), tols):
         assert errs[name] < tol, (name, errs[name])
 

@@ -605,3 +605,49 @@ def test_one_pass_step_refused_where_it_does_not_apply(psv, O, variant):
         with pytest.raises(psv._lib.FwiError) as e:
             sim.forward()
         assert e.value.status == -1
+
+
+def test_kept_units_follow_the_memory_budget(psv, O):
+    """The planner keeps as many final units' snapshots as the budget holds beside the batch
+    and one checkpoint per remaining boundary: over rising budgets (at the same batch) the kept
+    count rises to all units, the allocation stays within the budget, and at a budget that keeps some
+    but not all the gradient is still bit-identical to full snapshot storage."""
+    c = cases.small_case(O, nshot=2, nt=300, snap_dt=3, uneven=True)
+    obs = cases.per_shot_obs(c, O, O.mat_av2(c))
+    lam0, mu0, rho0 = cases.start_model(c)
+    S, units = 15, 20
+    sizes = {}
+    for keep in (1, 2):
+        with psv.Simulation(c, storage=psv._lib.STORE_CHECKPOINT, ckpt_interval=S, ckpt_keep=keep) as sim:
+            p = sim.plan()
+            assert p["ckpt_units"] == units and p["ckpt_kept"] == keep
+            sizes[keep] = p["device_bytes"]
+    per_unit = sizes[2] - sizes[1]   # one unit of snapshots less one checkpoint
+    assert per_unit > 0
+    kept, middle = [], None
+    for j in range(0, 40, 2):        # the planner's estimate includes a fixed reserve above the arrays
+        budget = int(sizes[1] + (64 << 20) + j * per_unit)
+        try:
+            sim = psv.Simulation(c, storage=psv._lib.STORE_CHECKPOINT, ckpt_interval=S, mem_budget=budget)
+        except psv._lib.FwiError:    # below what one unit kept with a checkpoint per boundary needs
+            continue
+        with sim:
+            p = sim.plan()
+            assert p["device_bytes"] <= budget and p["ckpt_slots"] == max(units - p["ckpt_kept"] - 1, 0)
+            if p["batch"] < 2:       # the budget holds one shot only (which then keeps more units)
+                continue
+            kept.append(p["ckpt_kept"])
+            if middle is None and 1 < p["ckpt_kept"] < units:
+                middle = budget
+    _print(f"kept units over rising budgets (batch 2): {kept}")
+    assert kept == sorted(kept) and kept[-1] == units and middle is not None
+    res = {}
+    for name, kw in (("full", dict(storage=psv._lib.STORE_FULL)),
+                     ("budget", dict(storage=psv._lib.STORE_CHECKPOINT, ckpt_interval=S, mem_budget=middle))):
+        with psv.Simulation(c, **kw) as sim:
+            sim.set_model(lam0, mu0, rho0)
+            sim.set_observed([o[0] for o in obs], [o[1] for o in obs])
+            res[name] = (sim.gradient(), sim.get_gradient())
+    np.testing.assert_array_equal(res["budget"][0], res["full"][0])
+    for i in range(3):
+        np.testing.assert_array_equal(res["budget"][1][i], res["full"][1][i])

@@ -45,6 +45,9 @@ def main(argv=None):
     ap.add_argument("--param", choices=("velocity", "lame"), default="velocity")
     ap.add_argument("--pcg", action="store_true", help="conjugate-gradient directions (PCG_PSV)")
     ap.add_argument("--smooth", type=int, default=0, metavar="HFS", help="Gaussian smoothing half width")
+    ap.add_argument("--batch", type=int, default=0,
+                    help="shots per launch (0: as many as memory allows); fewer shots leave memory for "
+                         "kept snapshot units, which cuts recomputation")
     ap.add_argument("--json", default="", help="also write the history as JSON to this file")
     args = ap.parse_args(argv)
 
@@ -66,7 +69,8 @@ def main(argv=None):
 
     hist = []
     param = psv._lib.PARAM_VELOCITY if args.param == "velocity" else psv._lib.PARAM_LAME
-    with psv.Simulation(true, grad_sign=1, param=param, pcg=int(args.pcg), smooth_hfs=args.smooth) as sim:
+    with psv.Simulation(true, grad_sign=1, param=param, pcg=int(args.pcg), smooth_hfs=args.smooth,
+                        batch=args.batch) as sim:
         t0 = time.perf_counter()
         uz, ux = sim.forward()
         sim.set_observed(uz, ux)

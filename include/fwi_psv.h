@@ -121,7 +121,9 @@ typedef struct {
 } fwi_acq;
 
 typedef struct {
-    int32_t batch;          /* shots advanced together in one launch; 0 = as many as memory allows */
+    int32_t batch;          /* shots advanced together in one launch; 0 = as many as memory allows.
+                               With CHECKPOINT storage a smaller batch leaves memory for more kept
+                               units (ckpt_keep); at 1001 x 2001, 8 shots per launch fill the GPU */
     int32_t storage;        /* enum fwi_storage */
     int32_t ckpt_interval;  /* time steps per checkpoint segment; 0 = chosen from memory */
     int64_t mem_budget;     /* device bytes the context may allocate; 0 = 90 % of free memory */
